@@ -1,0 +1,190 @@
+"""Benchmark: Yee Gcell-updates/s of the forward FDTD step on one MI355X.
+
+Workload (the dielectric-box configuration): 256^3 physical cells (281^3 Yee
+nodes with CPML 12 on all faces), 40 seeded dielectric boxes (eps ~ U[1,12],
+edge 8-64 cells), +z soft plane wave with a Gaussian pulse, two z-plane DFT flux
+monitors at 8 frequencies over f0 +- 20%, Courant 0.99.  A "step" is one FDTD
+time step of that scene (H+CPML, soft-source terms, E+CPML, monitor DFT).
+
+Prints ONE JSON line.  ``value`` is node-updates per second over the timed steps
+with all inputs resident in HBM; ``roofline`` prices the dominant kernel's
+algorithmic bytes per launch against 8 TB/s; ``cpu_baseline`` times the float32
+OpenMP transcription of the same update equations (oracle/fdtd_oracle.c) on a
+bounded sample of the same scene on the host cores.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12
+
+
+def algorithmic_bytes(plan, runner):
+    """Compulsory float32 bytes per step and per E/H launch.
+
+    Per step: every field component read and written once (8 B per sample), each
+    non-uniform coefficient array read once, each psi slab sample read and
+    written once, ADE pole state read+written and drive read, full-field DFT
+    accumulators read+written.  Per launch: the same restricted to what that
+    kernel must touch (its own family R+W, the other family read once)."""
+    import numpy as np
+    from witwin_maxwell_amd.fdtd.plan import COMPONENTS
+    numel = {c: int(np.prod(plan.field_shape(c))) for c in COMPONENTS}
+    e_n = sum(numel[c] for c in COMPONENTS[:3])
+    h_n = sum(numel[c] for c in COMPONENTS[3:])
+    coef_e = sum(4 * numel[c] for c in COMPONENTS[:3] for k in ("decay", "curl") if runner.uniform[(c, k)] is None)
+    coef_h = sum(4 * numel[c] for c in COMPONENTS[3:] for k in ("decay", "curl") if runner.uniform[(c, k)] is None)
+    psi_e = sum(8 * t.numel() for n, t in runner.psi.items() if n.startswith("psi_e"))
+    psi_h = sum(8 * t.numel() for n, t in runner.psi.items() if n.startswith("psi_h"))
+    ade = 0
+    for c, states in runner.ade_state.items():
+        for s in states:
+            ade += numel[c] * (4 + 8 + (8 if s["polarization"] is not None else 0))
+        if states:
+            ade += 4 * numel[c]
+    dft = sum(2 * 8 * len(plan.dft_frequencies) * numel[c] for c in COMPONENTS[:3]) if plan.dft_frequencies else 0
+    step = 8 * (e_n + h_n) + coef_e + coef_h + psi_e + psi_h + ade + dft
+    e_launch = 8 * e_n + 4 * h_n + coef_e + psi_e + ade
+    h_launch = 8 * h_n + 4 * e_n + coef_h + psi_h
+    return step, e_launch, h_launch
+
+
+def time_kernel(runner, which, reps):
+    """Average duration of one E or H launch, measured with HIP events on the
+    stream the kernel is launched on (torch's current stream)."""
+    import torch
+    from witwin_maxwell_amd.fdtd import native as nat
+    lib, prog = runner.lib, runner.program
+    stream = nat.current_stream()
+    ade = ctypes.byref(prog.ade) if prog.has_ade else None
+
+    def launch():
+        if which == "E":
+            rc = lib.mw_update_electric(ctypes.byref(prog.grid), ctypes.byref(prog.coeffs), ctypes.byref(prog.cpml),
+                                        ade, ctypes.byref(prog.fields), 7, stream)
+        else:
+            rc = lib.mw_update_magnetic(ctypes.byref(prog.grid), ctypes.byref(prog.coeffs), ctypes.byref(prog.cpml),
+                                        ctypes.byref(prog.fields), 7, stream)
+        nat.check(rc, lib)
+
+    for _ in range(3):
+        launch()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for _ in range(reps):
+        launch()
+    end.record()
+    end.synchronize()
+    return start.elapsed_time(end) * 1e-3 / reps
+
+
+def cpu_baseline(scene, kw, steps=8, warm=2):
+    """Float32 OpenMP transcription (oracle/fdtd_oracle.c) on the host cores."""
+    from oracle.driver import OracleRun
+    from witwin_maxwell_amd.fdtd.plan import compile_plan
+    import numpy as np
+    kw = dict(kw)
+    kw["time_steps"] = warm + steps
+    plan = compile_plan(scene, **kw)
+    run = OracleRun(plan, precision="f32")
+    for _ in range(warm):
+        run.step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run.step()
+    dt = time.perf_counter() - t0
+    nodes = int(np.prod(plan.shape))
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": nodes * steps / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"same {plan.shape[0]}x{plan.shape[1]}x{plan.shape[2]}-node scene, {steps} steps after "
+                      f"{warm} warm-up steps, float32 OpenMP ({threads} threads), {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--cells", type=int, default=256)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
+              f"(the FDTD path runs on a single device).", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+    from witwin_maxwell_amd import configs
+    from witwin_maxwell_amd.fdtd.plan import compile_plan
+    from witwin_maxwell_amd.fdtd.runtime import DeviceRun
+
+    total = args.warmup + args.steps
+    scene, kw = configs.dielectric_boxes(cells=args.cells, pml=12, n_boxes=40, steps=max(2000, total))
+    kw = dict(kw)
+    kw["time_steps"] = max(2000, total)
+    t0 = time.perf_counter()
+    plan = compile_plan(scene, **kw)
+    compile_s = time.perf_counter() - t0
+    runner = DeviceRun(plan, "cuda:0")
+    nodes = int(np.prod(plan.shape))
+    print(f"[bench] plan {plan.shape} compiled in {compile_s:.1f}s; warmup {args.warmup} steps", file=sys.stderr,
+          flush=True)
+    runner.run(args.warmup)
+    torch.cuda.synchronize()
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    runner.run(args.steps)
+    end.record()
+    end.synchronize()
+    torch.cuda.synchronize()
+    elapsed = start.elapsed_time(end) * 1e-3
+    ms_per_step = elapsed * 1e3 / args.steps
+    value = nodes * args.steps / elapsed / 1e9
+    print(f"[bench] {args.steps} steps in {elapsed:.3f}s ({ms_per_step:.4f} ms/step)", file=sys.stderr, flush=True)
+
+    step_bytes, e_bytes, h_bytes = algorithmic_bytes(plan, runner)
+    reps = max(20, min(200, args.steps // 5))
+    t_e = time_kernel(runner, "E", reps)
+    t_h = time_kernel(runner, "H", reps)
+    dominant, t_dom, b_dom = ("update_e_kernel", t_e, e_bytes) if t_e >= t_h else ("update_h_kernel", t_h, h_bytes)
+    achieved = b_dom / t_dom
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "traffic_dielectric_boxes_256.json")
+    if os.path.exists(tf):
+        try:
+            traffic = json.load(open(tf)).get(dominant)
+        except Exception:
+            traffic = None
+    roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": dominant,
+                "kernel_avg_us": t_dom * 1e6, "alg_bytes_per_launch": b_dom,
+                "step_alg_bytes_per_node": step_bytes / nodes,
+                "step_frac": step_bytes * args.steps / elapsed / HBM_PEAK}
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(scene, kw, steps=args.cpu_steps)
+    line = {"metric": "Yee Gcell-updates/s (% HBM peak)", "value": value, "unit": "Gcell-updates/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+            "config": {"workload": f"dielectric boxes {args.cells}^3 cells (+2x12 CPML), 40 boxes eps U[1,12], "
+                                   f"soft +z plane wave, 2 flux monitors x 8 freq",
+                       "nodes": list(plan.shape), "courant": 0.99},
+            "roofline": roofline, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

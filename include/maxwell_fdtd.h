@@ -1,0 +1,225 @@
+/*
+ * maxwell_fdtd.h -- C ABI of the MI355X (gfx950) FDTD runtime.
+ *
+ * Plain pointers, sizes and scalars only; every device pointer is a float32
+ * (or where stated float64/int32) buffer owned by the caller, laid out like the
+ * reference's torch tensors: [x][y][z], z fastest, component shapes
+ *   Ex (Nx-1,Ny,Nz)  Ey (Nx,Ny-1,Nz)  Ez (Nx,Ny,Nz-1)
+ *   Hx (Nx,Ny-1,Nz-1) Hy (Nx-1,Ny,Nz-1) Hz (Nx-1,Ny-1,Nz)
+ * (reference: witwin/maxwell/fdtd/runtime/stepping.py:1625-1630).  `stream` is a
+ * hipStream_t passed as void*; nothing here synchronises it.  Every entry point
+ * returns 0 on success or a negative MW_ERR_* code, with a message available
+ * from mw_last_error() (mirrors the reference's STD_TORCH_CHECK RuntimeErrors,
+ * witwin/maxwell/fdtd/cuda/tensors.h:10-55).
+ *
+ * The reference binds its kernels as torch ops
+ * (witwin/maxwell/fdtd/cuda/extension.cpp:2059-2212) called through
+ * NativeFDTDModule.<Name>(**kw).launchRaw() (fdtd/cuda/backend.py:81-115); each
+ * declaration below names the reference op family it replaces.
+ */
+#ifndef MAXWELL_FDTD_H
+#define MAXWELL_FDTD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MW_OK 0
+#define MW_ERR_ARGUMENT -1
+#define MW_ERR_LAUNCH -2
+#define MW_ERR_RUNTIME -3
+
+/* boundary face codes, witwin/maxwell/fdtd/cuda/kernels/common.cuh:45-50 */
+#define MW_BC_NONE 0
+#define MW_BC_PML 1
+#define MW_BC_PERIODIC 2
+#define MW_BC_PEC 4
+#define MW_BC_PMC 5
+
+#define MW_MAX_POLES 4
+#define MW_MAX_TERMS 16
+#define MW_MAX_PLANES 16
+
+/* Yee grid: node counts, per-face boundary codes and spacing reciprocals
+ * (dual spacing for E, length N; primal for H, length N-1;
+ * reference fdtd/runtime/initialization.py:131-139). */
+typedef struct mw_grid {
+  int32_t nx, ny, nz;
+  int32_t bc_lo[3], bc_hi[3];
+  const float* inv_e[3];
+  const float* inv_h[3];
+} mw_grid;
+
+/* Update coefficients per component, order Ex Ey Ez Hx Hy Hz.  A NULL array
+ * means the coefficient is uniform and *_value is used
+ * (reference runtime/materials.py:1550-1574 uniform fast path). */
+typedef struct mw_coeffs {
+  const float* decay[6];
+  const float* curl[6];
+  float decay_value[6];
+  float curl_value[6];
+} mw_coeffs;
+
+/* CPML: 1-D profiles per axis and 12 slab-stored psi arrays, order
+ * ex_y ex_z ey_x ey_z ez_x ez_y hx_y hx_z hy_x hy_z hz_x hz_y.  psi[q] spans
+ * psi_low[q] + psi_high_len[q] samples along its derivative axis
+ * (reference boundary/cpml.py:218-257, kernels/common.cuh:65-96). */
+typedef struct mw_cpml {
+  int32_t enabled;
+  int32_t pad;
+  const float* inv_kappa_e[3];
+  const float* b_e[3];
+  const float* c_e[3];
+  const float* inv_kappa_h[3];
+  const float* b_h[3];
+  const float* c_h[3];
+  float* psi[12];
+  int32_t psi_low[12];
+  int32_t psi_high_start[12];
+  int32_t psi_high_len[12];
+} mw_cpml;
+
+typedef struct mw_fields {
+  float* e[3];
+  float* h[3];
+} mw_fields;
+
+/* One Debye (0) / Drude (1) / Lorentz (2) pole on one E component; drive and the
+ * state arrays have the component's shape (reference kernels/dispersive.cu:18-70). */
+typedef struct mw_pole {
+  int32_t kind;
+  float decay;
+  float restoring;
+  int32_t pad;
+  const float* drive;
+  float* polarization; /* unused (may be NULL) for Drude */
+  float* current;
+} mw_pole;
+
+typedef struct mw_ade {
+  int32_t count[3];
+  float dt;
+  const float* inv_eps[3];
+  mw_pole poles[3][MW_MAX_POLES];
+} mw_ade;
+
+/* H <- H*decay - curl*(curl E) with CPML psi for the components in comp_mask
+ * (bit c = Hx,Hy,Hz).  Replaces updateMagneticField{Hx,Hy,Hz}{3D,CpmlCompressed3D,
+ * Standard3D} (kernels/magnetic.cu:20-212, launchers :368-921). */
+int mw_update_magnetic(const mw_grid* grid, const mw_coeffs* coeffs, const mw_cpml* cpml, const mw_fields* fields,
+                       int32_t comp_mask, void* stream);
+
+/* E <- E*decay + curl*(curl H) with boundary modes, CPML psi and (optional, may
+ * be NULL) fused ADE advance + polarisation-current subtraction.  Replaces
+ * updateElectricField{Ex,Ey,Ez}{Cpml3D,CpmlCompressed3D,Standard3D} plus
+ * update{Debye,Drude,Lorentz}Current3D / applyPolarizationCurrent3D
+ * (kernels/electric.cu:242-941, kernels/dispersive.cu:18-83). */
+int mw_update_electric(const mw_grid* grid, const mw_coeffs* coeffs, const mw_cpml* cpml, const mw_ade* ade,
+                       const mw_fields* fields, int32_t comp_mask, void* stream);
+
+/* field[o + p] += signal * patch[p].  Replaces addSourcePatch3D
+ * (kernels/sources.cu:135-154). */
+int mw_add_source_patch(float* field, int32_t fnx, int32_t fny, int32_t fnz, const float* patch, int32_t pnx,
+                        int32_t pny, int32_t pnz, int32_t oi, int32_t oj, int32_t ok, float signal, void* stream);
+
+/* field[o + p] += patch[p] * s(time - delay[p]) with the CW (0) / Gaussian pulse (1)
+ * / Ricker (2) waveform evaluated in double precision; delay is float64.
+ * Replaces addTimeShiftedSourcePatch3D (kernels/sources.cu:179-261). */
+int mw_add_time_shifted_patch(float* field, int32_t fnx, int32_t fny, int32_t fnz, const float* patch,
+                              const double* delay, int32_t pnx, int32_t pny, int32_t pnz, int32_t oi, int32_t oj,
+                              int32_t ok, double time, int32_t kind, double frequency, double fwidth,
+                              double amplitude, double phase, double delay0, void* stream);
+
+/* re[f][l] += v[l]*wcos[f]; im[f][l] += v[l]*wsin[f] for f < nf, weights in device
+ * memory.  Replaces accumulateDft* (kernels/spectral.cu:9-297). */
+int mw_accumulate_dft(const float* field, int64_t count, int32_t nf, const float* wcos, const float* wsin,
+                      float* re, float* im, void* stream);
+
+/* Plane observer: accumulate one axis-normal plane of a component
+ * (axis 0/1/2, plane index) into re/im of shape (nf, plane).  Replaces
+ * accumulatePlaneObserver3D (kernels/observers.cu:35-78). */
+int mw_accumulate_plane(const float* field, int32_t nx, int32_t ny, int32_t nz, int32_t axis, int32_t index,
+                        int32_t nf, const float* wcos, const float* wsin, float* re, float* im, void* stream);
+
+/* Point observers: accumulate count samples field[idx[p]] (linear indices) into
+ * re/im of shape (nf, count).  Replaces accumulatePointObservers3D
+ * (kernels/observers.cu:7-33). */
+int mw_accumulate_points(const float* field, const int32_t* linear_index, int32_t count, int32_t nf,
+                         const float* wcos, const float* wsin, float* re, float* im, void* stream);
+
+/* ---- multi-step program: the whole per-step sequence with device-side step
+ * tables, launched from native code (reference hot loop
+ * fdtd/runtime/stepping.py:2289-2357 and its graph runners :1818-2024). ---- */
+
+#define MW_TERM_ADDITIVE 0 /* value = signal_table[step] * patch            */
+#define MW_TERM_SHIFTED 1  /* value = patch * s(step*dt + t_offset - delay) */
+
+typedef struct mw_term {
+  int32_t kind;
+  int32_t component; /* 0..5 = Ex..Hz */
+  int32_t offset[3];
+  int32_t shape[3];
+  const float* patch;
+  const double* delay;        /* SHIFTED */
+  const float* signal_table;  /* ADDITIVE: one value per step */
+  int32_t wave_kind;          /* SHIFTED: 0 CW, 1 Gaussian, 2 Ricker */
+  int32_t pad;
+  double t_offset, frequency, fwidth, amplitude, phase, delay0;
+} mw_term;
+
+typedef struct mw_plane_obs {
+  int32_t component, axis, index, pad;
+  float* re; /* (nf, plane) */
+  float* im;
+} mw_plane_obs;
+
+typedef struct mw_program {
+  mw_grid grid;
+  mw_coeffs coeffs;
+  mw_cpml cpml;
+  mw_ade ade;
+  mw_fields fields;
+  double dt;
+  int32_t total_steps;
+  int32_t has_ade;
+  /* H-side (applied after the H update) and E-side terms; HOST arrays of at
+   * most MW_MAX_TERMS descriptors each (they travel as kernel arguments) */
+  int32_t n_h_terms, n_e_terms;
+  const mw_term* h_terms;
+  const mw_term* e_terms;
+  /* observers: weight tables are (total_steps, nf) float32 in device memory */
+  int32_t nf_obs, n_planes, n_points, pad2;
+  const float* obs_e_cos;
+  const float* obs_e_sin;
+  const float* obs_h_cos;
+  const float* obs_h_sin;
+  const mw_plane_obs* planes; /* HOST array of at most MW_MAX_PLANES */
+  const int32_t* point_component; /* device arrays, n_points */
+  const int32_t* point_linear;
+  float* point_re; /* (nf, n_points) */
+  float* point_im;
+  /* full-field DFT of E */
+  int32_t nf_dft, pad3;
+  const float* dft_cos;
+  const float* dft_sin;
+  float* dft_re[3]; /* (nf, component numel) */
+  float* dft_im[3];
+} mw_program;
+
+/* Advance `count` steps starting at step index `first` (t = n*dt). */
+int mw_run_steps(const mw_program* program, int32_t first, int32_t count, void* stream);
+
+const char* mw_last_error(void);
+int32_t mw_abi_version(void);
+/* sizeof of mw_grid, mw_coeffs, mw_cpml, mw_fields, mw_pole, mw_ade, mw_term,
+ * mw_plane_obs, mw_program (in that order) for binding layout checks; returns
+ * the number of entries. */
+int32_t mw_abi_struct_sizes(int64_t* out, int32_t capacity);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MAXWELL_FDTD_H */

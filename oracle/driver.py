@@ -1,0 +1,240 @@
+"""Step driver for the CPU checkers -- TEST INFRASTRUCTURE ONLY.
+
+Runs an ``FDTDPlan`` through ``fdtd_oracle.c`` (float64 oracle, or the float32
+OpenMP build used as the CPU baseline), in the reference's per-step order
+(witwin/maxwell/fdtd/runtime/stepping.py:1740-1815, :2289-2334):
+
+    H update (+CPML psi)             magnetic.cu:62-117
+    H surface terms at t             stepping.py:1776-1777
+    ADE advance from E^n             stepping.py:1794
+    E update (+CPML psi)             electric.cu:463-553
+    E surface terms at t + dt/2      stepping.py:2300-2307
+    additive point sources at t      stepping.py:2309-2310
+    ADE subtraction                  stepping.py:2316 (materials.py:783-798)
+    full-field DFT                   spectral.py:627-652
+    observers                        observers.py:577-659
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg use this.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+BUILD = os.path.join(HERE, "_build")
+_LIBS = {}
+
+_COMP = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
+_KIND = {"debye": 0, "drude": 1, "lorentz": 2}
+
+
+def build(force=False):
+    """Compile the checkers with the committed Makefile."""
+    targets = [os.path.join(BUILD, "liboracle_f64.so"), os.path.join(BUILD, "libcpu_f32.so")]
+    if force or not all(os.path.exists(t) for t in targets):
+        subprocess.run(["make", "-s", "-C", HERE] + (["-B"] if force else []), check=True)
+    return targets
+
+
+def load(precision="f64"):
+    if precision in _LIBS:
+        return _LIBS[precision]
+    name = "liboracle_f64.so" if precision == "f64" else "libcpu_f32.so"
+    path = os.path.join(BUILD, name)
+    if not os.path.exists(path):
+        build()
+    lib = ctypes.CDLL(path)
+    real = ctypes.c_double if precision == "f64" else ctypes.c_float
+    P = ctypes.POINTER(real)
+    PP = ctypes.POINTER(P)
+    lib.or_update_h.argtypes = [ctypes.c_void_p, PP, PP, PP]
+    lib.or_update_e.argtypes = [ctypes.c_void_p, PP, PP, PP]
+    i64, i32, dbl = ctypes.c_int64, ctypes.c_int32, ctypes.c_double
+    lib.or_ade_advance.argtypes = [ctypes.c_int, i64, P, P, dbl, dbl, dbl, P, P]
+    lib.or_ade_apply.argtypes = [i64, P, P, dbl, P]
+    lib.or_add_patch.argtypes = [P, i32, i32, P, i32, i32, i32, i32, i32, i32, dbl]
+    lib.or_add_shifted.argtypes = [P, i32, i32, P, ctypes.POINTER(dbl), i32, i32, i32, i32, i32, i32, dbl, i32,
+                                   dbl, dbl, dbl, dbl, dbl]
+    lib.or_accumulate.argtypes = [i64, P, i32, ctypes.POINTER(dbl), ctypes.POINTER(dbl), P, P]
+    lib._real = real
+    lib._dtype = np.float64 if precision == "f64" else np.float32
+    _LIBS[precision] = lib
+    return lib
+
+
+class _Grid(ctypes.Structure):
+    pass
+
+
+def _grid_struct(real):
+    P = ctypes.POINTER(real)
+    fields = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+              ("bc_lo", ctypes.c_int32 * 3), ("bc_hi", ctypes.c_int32 * 3), ("cpml", ctypes.c_int32),
+              ("pad", ctypes.c_int32), ("inv_e", P * 3), ("inv_h", P * 3), ("decay", P * 6), ("curl", P * 6),
+              ("kinv_e", P * 3), ("b_e", P * 3), ("c_e", P * 3), ("kinv_h", P * 3), ("b_h", P * 3), ("c_h", P * 3)]
+    return type("OrGrid", (ctypes.Structure,), {"_fields_": fields})
+
+
+def _ptr(arr, real):
+    return arr.ctypes.data_as(ctypes.POINTER(real))
+
+
+class OracleRun:
+    """Holds the state of one plan being stepped on the CPU."""
+
+    def __init__(self, plan, precision="f64"):
+        self.plan = plan
+        self.lib = load(precision)
+        dt_np = self.lib._dtype
+        real = self.lib._real
+        self._keep = []
+
+        def arr(a):
+            out = np.ascontiguousarray(np.asarray(a, dtype=dt_np))
+            self._keep.append(out)
+            return out
+
+        nx, ny, nz = plan.shape
+        self.fields = {c: np.zeros(plan.field_shape(c), dtype=dt_np) for c in _COMP}
+        self.psi = [np.zeros(plan.field_shape(c), dtype=dt_np) for c in
+                    ("Ex", "Ex", "Ey", "Ey", "Ez", "Ez", "Hx", "Hx", "Hy", "Hy", "Hz", "Hz")]
+        G = _grid_struct(real)
+        g = G()
+        g.nx, g.ny, g.nz = nx, ny, nz
+        for a in range(3):
+            g.bc_lo[a] = plan.bc_lo[a]
+            g.bc_hi[a] = plan.bc_hi[a]
+            g.inv_e[a] = _ptr(arr(plan.inv_e[a]), real)
+            g.inv_h[a] = _ptr(arr(plan.inv_h[a]), real)
+        for ci, c in enumerate(_COMP):
+            dec, curl = plan.coef[c]
+            g.decay[ci] = _ptr(arr(np.broadcast_to(dec, plan.field_shape(c))), real)
+            g.curl[ci] = _ptr(arr(np.broadcast_to(curl, plan.field_shape(c))), real)
+        g.cpml = 1 if plan.uses_cpml else 0
+        if plan.uses_cpml:
+            for a, ax in enumerate("xyz"):
+                p = plan.cpml[ax]
+                g.kinv_e[a] = _ptr(arr(1.0 / p["kappa_e"]), real)
+                g.b_e[a] = _ptr(arr(p["b_e"]), real)
+                g.c_e[a] = _ptr(arr(p["c_e"]), real)
+                g.kinv_h[a] = _ptr(arr(1.0 / p["kappa_h"]), real)
+                g.b_h[a] = _ptr(arr(p["b_h"]), real)
+                g.c_h[a] = _ptr(arr(p["c_h"]), real)
+        self.grid = g
+        P = ctypes.POINTER(real)
+        self._eptr = (P * 3)(*[_ptr(self.fields[c], real) for c in ("Ex", "Ey", "Ez")])
+        self._hptr = (P * 3)(*[_ptr(self.fields[c], real) for c in ("Hx", "Hy", "Hz")])
+        self._psiptr = (P * 12)(*[_ptr(p, real) for p in self.psi])
+
+        self.ade = {}
+        for c in ("Ex", "Ey", "Ez"):
+            entries = []
+            for pole in plan.ade[c]["poles"]:
+                entries.append({"kind": _KIND[pole["kind"]], "decay": pole["decay"], "restoring": pole["restoring"],
+                                "drive": arr(pole["drive"]), "pol": np.zeros(plan.field_shape(c), dt_np),
+                                "cur": np.zeros(plan.field_shape(c), dt_np)})
+            self.ade[c] = {"inv_eps": arr(plan.ade[c]["inv_eps"]), "poles": entries}
+
+        self.point_terms = [(t["field"], t["offset"], arr(t["patch"])) for t in plan.point_terms]
+        self.e_terms = [(t, arr(t["patch"]), np.ascontiguousarray(t["delay"], dtype=np.float64)) for t in plan.e_terms]
+        self.h_terms = [(t, arr(t["patch"]), np.ascontiguousarray(t["delay"], dtype=np.float64)) for t in plan.h_terms]
+
+        self.obs = plan.observers
+        if self.obs is not None:
+            nf = len(self.obs.frequencies)
+            self.point_re = np.zeros((nf, len(self.obs.points)))
+            self.point_im = np.zeros((nf, len(self.obs.points)))
+            shapes = {c: plan.field_shape(c) for c in _COMP}
+            self.plane_re = [np.zeros((nf,) + self.obs.plane_shape(shapes, k)) for k in self.obs.planes]
+            self.plane_im = [np.zeros((nf,) + self.obs.plane_shape(shapes, k)) for k in self.obs.planes]
+        self.dft = None
+        if plan.dft_frequencies:
+            nf = len(plan.dft_frequencies)
+            self.dft = {c: (np.zeros((nf,) + plan.field_shape(c), dt_np), np.zeros((nf,) + plan.field_shape(c), dt_np))
+                        for c in ("Ex", "Ey", "Ez")}
+        self.step_index = 0
+        # running peak |field| per component over all steps taken, the scale
+        # against which late-time (decayed) fields are judged
+        self.peak = {c: 0.0 for c in _COMP}
+        self.track_peak = precision == "f64"
+
+    # ------------------------------------------------------------------ steps
+    def _shifted(self, terms, time):
+        real = self.lib._real
+        for t, patch, delay in terms:
+            f = self.fields[t["field"]]
+            st = t["st"]
+            self.lib.or_add_shifted(_ptr(f, real), f.shape[1], f.shape[2], _ptr(patch, real),
+                                    delay.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), *patch.shape,
+                                    *t["offset"], time, st["kind_code"], st["frequency"], st["fwidth"],
+                                    st["amplitude"], st["phase"], st["delay"])
+
+    def step(self):
+        plan, lib, real = self.plan, self.lib, self.lib._real
+        n = self.step_index
+        t = n * plan.dt
+        lib.or_update_h(ctypes.byref(self.grid), self._eptr, self._hptr, self._psiptr)
+        self._shifted(self.h_terms, t)
+        for c in ("Ex", "Ey", "Ez"):
+            e = self.fields[c]
+            for p in self.ade[c]["poles"]:
+                lib.or_ade_advance(p["kind"], e.size, _ptr(e, real), _ptr(p["drive"], real), p["decay"],
+                                   p["restoring"], plan.dt, _ptr(p["pol"], real), _ptr(p["cur"], real))
+        lib.or_update_e(ctypes.byref(self.grid), self._eptr, self._hptr, self._psiptr)
+        self._shifted(self.e_terms, t + 0.5 * plan.dt)
+        for ti, (comp, off, patch) in enumerate(self.point_terms):
+            f = self.fields[comp]
+            lib.or_add_patch(_ptr(f, real), f.shape[1], f.shape[2], _ptr(patch, real), *patch.shape, *off,
+                             float(plan.signals[ti, n]))
+        for c in ("Ex", "Ey", "Ez"):
+            e = self.fields[c]
+            for p in self.ade[c]["poles"]:
+                lib.or_ade_apply(e.size, _ptr(p["cur"], real), _ptr(self.ade[c]["inv_eps"], real), plan.dt,
+                                 _ptr(e, real))
+        if self.dft is not None:
+            wc = np.ascontiguousarray(plan.dft_weights[0][n])
+            ws = np.ascontiguousarray(plan.dft_weights[1][n])
+            for c, (re, im) in self.dft.items():
+                lib.or_accumulate(self.fields[c].size, _ptr(self.fields[c], real), len(wc),
+                                  wc.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                  ws.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _ptr(re, real), _ptr(im, real))
+        if self.obs is not None:
+            self._observe(n)
+        if self.track_peak:
+            for c in _COMP:
+                self.peak[c] = max(self.peak[c], float(np.max(np.abs(self.fields[c]))))
+        self.step_index += 1
+
+    def _observe(self, n):
+        ec, es, hc, hs = (w[n] for w in self.plan.obs_weights)
+        for pi, (comp, idx) in enumerate(self.obs.points):
+            v = float(self.fields[comp][idx])
+            wc, ws = (ec, es) if comp[0] == "E" else (hc, hs)
+            self.point_re[:, pi] += v * wc
+            self.point_im[:, pi] += v * ws
+        for qi, (comp, axis, index) in enumerate(self.obs.planes):
+            f = self.fields[comp]
+            sl = {"x": f[index, :, :], "y": f[:, index, :], "z": f[:, :, index]}[axis].astype(np.float64)
+            wc, ws = (ec, es) if comp[0] == "E" else (hc, hs)
+            self.plane_re[qi] += wc[:, None, None] * sl[None]
+            self.plane_im[qi] += ws[:, None, None] * sl[None]
+
+    def run(self, steps=None):
+        steps = self.plan.time_steps if steps is None else steps
+        for _ in range(steps):
+            self.step()
+        return self
+
+    def monitor_results(self):
+        from witwin_maxwell_amd.fdtd.spectral import assemble_monitor_results
+        return assemble_monitor_results(self.obs, self.point_re, self.point_im, self.plane_re, self.plane_im,
+                                        self.plan.obs_window_sum)
+
+
+def run_plan(plan, steps=None, precision="f64"):
+    return OracleRun(plan, precision).run(steps)

@@ -1,0 +1,325 @@
+/*
+ * CPU restatement of the reference FDTD update equations -- TEST INFRASTRUCTURE.
+ *
+ * This file is the checker for the HIP runtime, never part of the product path:
+ * only tests/, the smoke check and bench.py's CPU-baseline leg load it.  It is
+ * compiled twice:
+ *   - REAL_T=double          -> liboracle_f64.so  (the oracle for parity tests)
+ *   - REAL_T=float, -fopenmp -> libcpu_f32.so     (the timed CPU baseline)
+ *
+ * Every routine restates a reference kernel, cited by file:line under
+ * witwin/maxwell/fdtd/cuda/kernels/ unless stated otherwise:
+ *   or_update_h        magnetic.cu:62-117   (H CPML update, dense psi)
+ *   or_update_e        electric.cu:18-119 (boundary modes), :463-553 (E CPML update)
+ *   or_ade_*           dispersive.cu:18-83  (Debye/Drude/Lorentz, polarisation current)
+ *   or_add_patch       sources.cu:135-154   (additive source patch)
+ *   or_add_shifted     sources.cu:71-93, :179-261 (time-shifted patch; waveform in double)
+ *   or_accumulate      spectral.cu:9-46, observers.cu:7-78 (running DFT sums)
+ * Reverse (adjoint) routines restate adjoint.cu as the exact transpose of the
+ * forward ones (see or_reverse_*).
+ *
+ * Pinning (tests/test_oracle_*.py): discrete PEC-cavity eigenmodes against the
+ * closed-form Yee dispersion relation, discrete energy conservation, CPML decay
+ * and reflection, ADE susceptibilities against their closed-form discrete-time
+ * transfer functions, DFT sums against direct numpy sums, and transpose
+ * (dot-product) identities for the reverse routines.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <string.h>
+
+#ifndef REAL_T
+#define REAL_T double
+#endif
+typedef REAL_T real_t;
+
+#define BC_NONE 0
+#define BC_PML 1
+#define BC_PERIODIC 2
+#define BC_PEC 4
+#define BC_PMC 5
+
+typedef struct {
+  int32_t nx, ny, nz;
+  int32_t bc_lo[3], bc_hi[3];
+  int32_t cpml;
+  int32_t pad;
+  const real_t* inv_e[3];
+  const real_t* inv_h[3];
+  const real_t* decay[6];
+  const real_t* curl[6];
+  const real_t* kinv_e[3];
+  const real_t* b_e[3];
+  const real_t* c_e[3];
+  const real_t* kinv_h[3];
+  const real_t* b_h[3];
+  const real_t* c_h[3];
+} or_grid;
+
+static inline int64_t off3(int64_t i, int64_t j, int64_t k, int64_t ny, int64_t nz) {
+  return (i * ny + j) * nz + k;
+}
+
+/* magnetic.cu:62-117 */
+void or_update_h(const or_grid* g, real_t* const* e, real_t* const* h, real_t* const* psi) {
+  const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
+  const real_t *ex = e[0], *ey = e[1], *ez = e[2];
+  /* Hx (nx, ny-1, nz-1): a = dEz/dy, b = dEy/dz */
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < nx; ++i)
+    for (int64_t j = 0; j < ny - 1; ++j)
+      for (int64_t k = 0; k < nz - 1; ++k) {
+        const int64_t l = off3(i, j, k, ny - 1, nz - 1);
+        real_t da = (ez[off3(i, j + 1, k, ny, nz - 1)] - ez[off3(i, j, k, ny, nz - 1)]) * g->inv_h[1][j];
+        real_t db = (ey[off3(i, j, k + 1, ny - 1, nz)] - ey[off3(i, j, k, ny - 1, nz)]) * g->inv_h[2][k];
+        real_t curl;
+        if (g->cpml) {
+          real_t pa = g->b_h[1][j] * psi[6][l] + g->c_h[1][j] * da;
+          real_t pb = g->b_h[2][k] * psi[7][l] + g->c_h[2][k] * db;
+          psi[6][l] = pa;
+          psi[7][l] = pb;
+          curl = (da * g->kinv_h[1][j] + pa) - (db * g->kinv_h[2][k] + pb);
+        } else {
+          curl = da - db;
+        }
+        h[0][l] = h[0][l] * g->decay[3][l] - g->curl[3][l] * curl;
+      }
+  /* Hy (nx-1, ny, nz-1): a = dEz/dx, b = dEx/dz, curl = b - a */
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < nx - 1; ++i)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t k = 0; k < nz - 1; ++k) {
+        const int64_t l = off3(i, j, k, ny, nz - 1);
+        real_t da = (ez[off3(i + 1, j, k, ny, nz - 1)] - ez[off3(i, j, k, ny, nz - 1)]) * g->inv_h[0][i];
+        real_t db = (ex[off3(i, j, k + 1, ny, nz)] - ex[off3(i, j, k, ny, nz)]) * g->inv_h[2][k];
+        real_t curl;
+        if (g->cpml) {
+          real_t pa = g->b_h[0][i] * psi[8][l] + g->c_h[0][i] * da;
+          real_t pb = g->b_h[2][k] * psi[9][l] + g->c_h[2][k] * db;
+          psi[8][l] = pa;
+          psi[9][l] = pb;
+          curl = (db * g->kinv_h[2][k] + pb) - (da * g->kinv_h[0][i] + pa);
+        } else {
+          curl = db - da;
+        }
+        h[1][l] = h[1][l] * g->decay[4][l] - g->curl[4][l] * curl;
+      }
+  /* Hz (nx-1, ny-1, nz): a = dEy/dx, b = dEx/dy */
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < nx - 1; ++i)
+    for (int64_t j = 0; j < ny - 1; ++j)
+      for (int64_t k = 0; k < nz; ++k) {
+        const int64_t l = off3(i, j, k, ny - 1, nz);
+        real_t da = (ey[off3(i + 1, j, k, ny - 1, nz)] - ey[off3(i, j, k, ny - 1, nz)]) * g->inv_h[0][i];
+        real_t db = (ex[off3(i, j + 1, k, ny, nz)] - ex[off3(i, j, k, ny, nz)]) * g->inv_h[1][j];
+        real_t curl;
+        if (g->cpml) {
+          real_t pa = g->b_h[0][i] * psi[10][l] + g->c_h[0][i] * da;
+          real_t pb = g->b_h[1][j] * psi[11][l] + g->c_h[1][j] * db;
+          psi[10][l] = pa;
+          psi[11][l] = pb;
+          curl = (da * g->kinv_h[0][i] + pa) - (db * g->kinv_h[1][j] + pb);
+        } else {
+          curl = da - db;
+        }
+        h[2][l] = h[2][l] * g->decay[5][l] - g->curl[5][l] * curl;
+      }
+}
+
+/* electric.cu:23-51 */
+static inline int is_pec(int64_t c, int64_t size, int lo, int hi) {
+  return (c == 0 && lo == BC_PEC) || (c + 1 == size && hi == BC_PEC);
+}
+static inline int is_inactive(int64_t c, int64_t size, int lo, int hi) {
+  return (c == 0 && (lo == BC_NONE || lo == BC_PML)) || (c + 1 == size && (hi == BC_NONE || hi == BC_PML));
+}
+
+/* Backward difference of `src` along one axis with the boundary modes of
+ * electric.cu:23-87.  `cur`/`prev` are the current and previous samples,
+ * `first`/`last` the samples at source index 0 and size-1 of that line. */
+static inline real_t bdiff(int64_t c, int64_t size, int lo, int hi, real_t inv, real_t cur, real_t prev, real_t first,
+                           real_t last) {
+  if (c == 0) {
+    if (lo == BC_PERIODIC) return (first - last) * inv;
+    if (lo == BC_PMC) return 2.0f * first * inv;
+    return 0;
+  }
+  if (c + 1 == size) {
+    if (hi == BC_PERIODIC) return (first - last) * inv;
+    if (hi == BC_PMC) return -2.0f * last * inv;
+    return 0;
+  }
+  return (cur - prev) * inv;
+}
+
+/* electric.cu:463-553 */
+void or_update_e(const or_grid* g, real_t* const* e, real_t* const* h, real_t* const* psi) {
+  const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
+  const real_t *hx = h[0], *hy = h[1], *hz = h[2];
+  const int *lo = g->bc_lo, *hi = g->bc_hi;
+  /* Ex (nx-1, ny, nz): a = dHz/dy (Hz: nx-1, ny-1, nz), b = dHy/dz (Hy: nx-1, ny, nz-1) */
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < nx - 1; ++i)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t k = 0; k < nz; ++k) {
+        const int64_t l = off3(i, j, k, ny, nz);
+        if (is_pec(j, ny, lo[1], hi[1]) || is_pec(k, nz, lo[2], hi[2])) { e[0][l] = 0; continue; }
+        if (is_inactive(j, ny, lo[1], hi[1]) || is_inactive(k, nz, lo[2], hi[2])) continue;
+        const int64_t jc = j < ny - 1 ? j : ny - 2, jp = j > 0 ? j - 1 : 0;
+        const int64_t kc = k < nz - 1 ? k : nz - 2, kp = k > 0 ? k - 1 : 0;
+        real_t da = bdiff(j, ny, lo[1], hi[1], g->inv_e[1][j], hz[off3(i, jc, k, ny - 1, nz)],
+                          hz[off3(i, jp, k, ny - 1, nz)], hz[off3(i, 0, k, ny - 1, nz)],
+                          hz[off3(i, ny - 2, k, ny - 1, nz)]);
+        real_t db = bdiff(k, nz, lo[2], hi[2], g->inv_e[2][k], hy[off3(i, j, kc, ny, nz - 1)],
+                          hy[off3(i, j, kp, ny, nz - 1)], hy[off3(i, j, 0, ny, nz - 1)],
+                          hy[off3(i, j, nz - 2, ny, nz - 1)]);
+        real_t curl;
+        if (g->cpml) {
+          real_t pa = g->b_e[1][j] * psi[0][l] + g->c_e[1][j] * da;
+          real_t pb = g->b_e[2][k] * psi[1][l] + g->c_e[2][k] * db;
+          psi[0][l] = pa;
+          psi[1][l] = pb;
+          curl = (da * g->kinv_e[1][j] + pa) - (db * g->kinv_e[2][k] + pb);
+        } else {
+          curl = da - db;
+        }
+        e[0][l] = e[0][l] * g->decay[0][l] + g->curl[0][l] * curl;
+      }
+  /* Ey (nx, ny-1, nz): a = dHz/dx (Hz: nx-1, ny-1, nz), b = dHx/dz (Hx: nx, ny-1, nz-1), curl = b - a */
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < nx; ++i)
+    for (int64_t j = 0; j < ny - 1; ++j)
+      for (int64_t k = 0; k < nz; ++k) {
+        const int64_t l = off3(i, j, k, ny - 1, nz);
+        if (is_pec(i, nx, lo[0], hi[0]) || is_pec(k, nz, lo[2], hi[2])) { e[1][l] = 0; continue; }
+        if (is_inactive(i, nx, lo[0], hi[0]) || is_inactive(k, nz, lo[2], hi[2])) continue;
+        const int64_t ic = i < nx - 1 ? i : nx - 2, ip = i > 0 ? i - 1 : 0;
+        const int64_t kc = k < nz - 1 ? k : nz - 2, kp = k > 0 ? k - 1 : 0;
+        real_t da = bdiff(i, nx, lo[0], hi[0], g->inv_e[0][i], hz[off3(ic, j, k, ny - 1, nz)],
+                          hz[off3(ip, j, k, ny - 1, nz)], hz[off3(0, j, k, ny - 1, nz)],
+                          hz[off3(nx - 2, j, k, ny - 1, nz)]);
+        real_t db = bdiff(k, nz, lo[2], hi[2], g->inv_e[2][k], hx[off3(i, j, kc, ny - 1, nz - 1)],
+                          hx[off3(i, j, kp, ny - 1, nz - 1)], hx[off3(i, j, 0, ny - 1, nz - 1)],
+                          hx[off3(i, j, nz - 2, ny - 1, nz - 1)]);
+        real_t curl;
+        if (g->cpml) {
+          real_t pa = g->b_e[0][i] * psi[2][l] + g->c_e[0][i] * da;
+          real_t pb = g->b_e[2][k] * psi[3][l] + g->c_e[2][k] * db;
+          psi[2][l] = pa;
+          psi[3][l] = pb;
+          curl = (db * g->kinv_e[2][k] + pb) - (da * g->kinv_e[0][i] + pa);
+        } else {
+          curl = db - da;
+        }
+        e[1][l] = e[1][l] * g->decay[1][l] + g->curl[1][l] * curl;
+      }
+  /* Ez (nx, ny, nz-1): a = dHy/dx (Hy: nx-1, ny, nz-1), b = dHx/dy (Hx: nx, ny-1, nz-1) */
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < nx; ++i)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t k = 0; k < nz - 1; ++k) {
+        const int64_t l = off3(i, j, k, ny, nz - 1);
+        if (is_pec(i, nx, lo[0], hi[0]) || is_pec(j, ny, lo[1], hi[1])) { e[2][l] = 0; continue; }
+        if (is_inactive(i, nx, lo[0], hi[0]) || is_inactive(j, ny, lo[1], hi[1])) continue;
+        const int64_t ic = i < nx - 1 ? i : nx - 2, ip = i > 0 ? i - 1 : 0;
+        const int64_t jc = j < ny - 1 ? j : ny - 2, jp = j > 0 ? j - 1 : 0;
+        real_t da = bdiff(i, nx, lo[0], hi[0], g->inv_e[0][i], hy[off3(ic, j, k, ny, nz - 1)],
+                          hy[off3(ip, j, k, ny, nz - 1)], hy[off3(0, j, k, ny, nz - 1)],
+                          hy[off3(nx - 2, j, k, ny, nz - 1)]);
+        real_t db = bdiff(j, ny, lo[1], hi[1], g->inv_e[1][j], hx[off3(i, jc, k, ny - 1, nz - 1)],
+                          hx[off3(i, jp, k, ny - 1, nz - 1)], hx[off3(i, 0, k, ny - 1, nz - 1)],
+                          hx[off3(i, ny - 2, k, ny - 1, nz - 1)]);
+        real_t curl;
+        if (g->cpml) {
+          real_t pa = g->b_e[0][i] * psi[4][l] + g->c_e[0][i] * da;
+          real_t pb = g->b_e[1][j] * psi[5][l] + g->c_e[1][j] * db;
+          psi[4][l] = pa;
+          psi[5][l] = pb;
+          curl = (da * g->kinv_e[0][i] + pa) - (db * g->kinv_e[1][j] + pb);
+        } else {
+          curl = da - db;
+        }
+        e[2][l] = e[2][l] * g->decay[2][l] + g->curl[2][l] * curl;
+      }
+}
+
+/* dispersive.cu:18-70; kind 0 = Debye, 1 = Drude, 2 = Lorentz */
+void or_ade_advance(int kind, int64_t n, const real_t* e, const real_t* drive, double decay, double restoring,
+                    double dt, real_t* pol, real_t* cur) {
+#pragma omp parallel for schedule(static)
+  for (int64_t l = 0; l < n; ++l) {
+    if (kind == 0) {
+      real_t prev = pol[l];
+      real_t next = (real_t)decay * prev + drive[l] * e[l];
+      pol[l] = next;
+      cur[l] = (next - prev) * (real_t)(1.0 / dt);
+    } else if (kind == 1) {
+      cur[l] = (real_t)decay * cur[l] + drive[l] * e[l];
+    } else {
+      real_t pc = cur[l], pp = pol[l];
+      real_t nc = (real_t)decay * pc - (real_t)restoring * pp + drive[l] * e[l];
+      cur[l] = nc;
+      pol[l] = pp + (real_t)dt * nc;
+    }
+  }
+}
+
+/* dispersive.cu:72-83 */
+void or_ade_apply(int64_t n, const real_t* cur, const real_t* inv_eps, double dt, real_t* e) {
+#pragma omp parallel for schedule(static)
+  for (int64_t l = 0; l < n; ++l) e[l] -= (real_t)dt * cur[l] * inv_eps[l];
+}
+
+/* sources.cu:135-154 */
+void or_add_patch(real_t* f, int32_t fny, int32_t fnz, const real_t* patch, int32_t pnx, int32_t pny, int32_t pnz,
+                  int32_t oi, int32_t oj, int32_t ok, double signal) {
+  for (int64_t i = 0; i < pnx; ++i)
+    for (int64_t j = 0; j < pny; ++j)
+      for (int64_t k = 0; k < pnz; ++k)
+        f[off3(oi + i, oj + j, ok + k, fny, fnz)] += (real_t)signal * patch[off3(i, j, k, pny, pnz)];
+}
+
+/* sources.cu:71-93 waveform, evaluated in double for the checker */
+static double waveform(int kind, double t, double freq, double fwidth, double amp, double phase, double delay) {
+  const double two_pi = 6.283185307179586476925286766559;
+  if (kind == 0) return amp * cos(two_pi * freq * t + phase);
+  if (kind == 1) {
+    double tau = t - delay;
+    double inv_sigma = two_pi * fwidth;
+    if (inv_sigma < 1e-30) inv_sigma = 1e-30;
+    double x = tau * inv_sigma;
+    return amp * exp(-0.5 * x * x) * cos(two_pi * freq * tau + phase);
+  }
+  double tau = t - delay;
+  double a = 3.14159265358979323846 * freq * tau;
+  return amp * (1.0 - 2.0 * a * a) * exp(-a * a);
+}
+
+/* sources.cu:179-261 (no causal gate) */
+void or_add_shifted(real_t* f, int32_t fny, int32_t fnz, const real_t* patch, const double* delay, int32_t pnx,
+                    int32_t pny, int32_t pnz, int32_t oi, int32_t oj, int32_t ok, double time, int32_t kind,
+                    double freq, double fwidth, double amp, double phase, double delay0) {
+  for (int64_t i = 0; i < pnx; ++i)
+    for (int64_t j = 0; j < pny; ++j)
+      for (int64_t k = 0; k < pnz; ++k) {
+        const int64_t p = off3(i, j, k, pny, pnz);
+        double s = waveform(kind, time - delay[p], freq, fwidth, amp, phase, delay0);
+        f[off3(oi + i, oj + j, ok + k, fny, fnz)] += (real_t)(s * (double)patch[p]);
+      }
+}
+
+/* spectral.cu:9-46 / observers.cu:35-78: re[f][l] += v[l]*wcos[f], im += v*wsin[f] */
+void or_accumulate(int64_t n, const real_t* v, int32_t nf, const double* wcos, const double* wsin, real_t* re,
+                   real_t* im) {
+  for (int32_t f = 0; f < nf; ++f) {
+    const real_t c = (real_t)wcos[f], s = (real_t)wsin[f];
+    if (c == 0 && s == 0) continue;
+#pragma omp parallel for schedule(static)
+    for (int64_t l = 0; l < n; ++l) {
+      re[f * n + l] += v[l] * c;
+      im[f * n + l] += v[l] * s;
+    }
+  }
+}
+
+int32_t or_real_bytes(void) { return (int32_t)sizeof(real_t); }

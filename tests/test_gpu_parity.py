@@ -1,0 +1,155 @@
+"""HIP runtime vs the float64 oracle on the same seeded plans (normalised max error).
+
+Tolerance: fields, psi, ADE state and DFT/monitor sums <= 1e-5 of the reference
+field's peak (FP32 device arithmetic vs FP64 oracle).
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import witwin_maxwell_amd as mw
+from witwin_maxwell_amd import configs
+from witwin_maxwell_amd.fdtd.cpml import PSI_NAMES
+from witwin_maxwell_amd.fdtd.plan import COMPONENTS, compile_plan
+
+from conftest import family_err, norm_err
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-5
+
+
+def _run_both(scene, kw, steps):
+    """Run oracle and device side by side; fields are also compared at mid-run,
+    while the pulse is still inside the grid."""
+    from oracle.driver import OracleRun
+    from witwin_maxwell_amd.fdtd.runtime import DeviceRun
+    kw = dict(kw)
+    kw.pop("theta", None)
+    kw["time_steps"] = steps
+    plan = compile_plan(scene, **kw)
+    ref = OracleRun(plan)
+    dev = DeviceRun(plan, "cuda")
+    mid = steps // 3
+    ref.run(mid)
+    dev.run(mid)
+    torch.cuda.synchronize()
+    _check_fields(ref, dev)
+    _check_psi(plan, ref, dev)
+    ref.run(steps - mid)
+    dev.run()
+    torch.cuda.synchronize()
+    return plan, ref, dev
+
+
+def _check_fields(ref, dev, tol=TOL):
+    """Family error normalised by the family's peak over the run so far: a
+    component that has decayed (pulse gone, PML-absorbed) is judged at the scale
+    float32 could represent it, max(|x|) over time times ~1e-7 per step."""
+    errs = {}
+    for fam in (COMPONENTS[:3], COMPONENTS[3:]):
+        scale = max(ref.peak[c] for c in fam)
+        errs.update(family_err({c: dev.fields[c].double().cpu().numpy() for c in fam},
+                               {c: ref.fields[c] for c in fam}, scale=scale))
+    assert max(errs.values()) <= tol, errs
+    return errs
+
+
+def _check_psi(plan, ref, dev, tol=TOL):
+    """psi compared mid-run, while the pulse is inside the absorbing layers
+    (after it has been absorbed psi decays towards round-off)."""
+    if not plan.uses_cpml:
+        return
+    for fam in (range(0, 6), range(6, 12)):
+        errs = family_err({PSI_NAMES[q]: dev.psi_dense(PSI_NAMES[q]).double().cpu().numpy() for q in fam},
+                          {PSI_NAMES[q]: ref.psi[q] for q in fam})
+        assert max(errs.values()) <= tol, errs
+
+
+def _check_monitors(ref, dev, tol=TOL):
+    mr, md = ref.monitor_results(), dev.monitor_results()
+    for name, entry in mr.items():
+        for fam in ("E", "H"):
+            comps = [c for c in entry["fields"] if c[0] == fam]
+            if not comps:
+                continue
+            def data(e, c):
+                return e["components"][c]["data"] if isinstance(e["components"][c], dict) else e["components"][c]
+            errs = family_err({c: data(md[name], c) for c in comps}, {c: data(entry, c) for c in comps})
+            assert max(errs.values()) <= tol, (name, errs)
+        if "flux" in entry:
+            assert norm_err(md[name]["flux"], entry["flux"]) <= 1e-4, (name, md[name]["flux"], entry["flux"])
+
+
+def test_vacuum_dipole_cpml():
+    print("case: vacuum dipole 20 cells CPML 8, 300 steps", flush=True)
+    scene, kw = configs.vacuum_dipole(cells=20, pml=8)
+    plan, ref, dev = _run_both(scene, kw, 300)
+    _check_fields(ref, dev)
+    _check_monitors(ref, dev)
+    for f in range(len(plan.dft_frequencies)):
+        d = {c + p: (dev.dft_re if p == "re" else dev.dft_im)[c][f].double().cpu().numpy()
+             for c in ("Ex", "Ey", "Ez") for p in ("re", "im")}
+        r = {c + p: ref.dft[c][0 if p == "re" else 1][f] for c in ("Ex", "Ey", "Ez") for p in ("re", "im")}
+        errs = family_err(d, r)
+        assert max(errs.values()) <= TOL, errs
+
+
+def test_dielectric_boxes_soft_plane_wave():
+    print("case: dielectric boxes 32 cells, soft plane wave, 250 steps", flush=True)
+    scene, kw = configs.dielectric_boxes(cells=32, pml=8, n_boxes=6, box_range=(4, 12), seed=3)
+    plan, ref, dev = _run_both(scene, kw, 250)
+    assert plan.h_terms and plan.e_terms
+    _check_fields(ref, dev)
+    _check_monitors(ref, dev)
+
+
+def test_dispersive_drude_lorentz():
+    print("case: Drude + 2 Lorentz spheres 28x28x24, 250 steps", flush=True)
+    scene, kw = configs.dispersive_spheres(cells=(28, 28, 24), pml=8, fill=0.3, radius=(3, 6), seed=1,
+                                           injection="soft", pulse_delay=3e-15)
+    plan, ref, dev = _run_both(scene, kw, 400)
+    assert plan.dispersive
+    _check_fields(ref, dev)
+    d, r = {}, {}
+    for c in ("Ex", "Ey", "Ez"):
+        for i, (p, st) in enumerate(zip(ref.ade[c]["poles"], dev.ade_state[c])):
+            d[(c, i)] = st["current"].double().cpu().numpy()
+            r[(c, i)] = p["cur"]
+    errs = family_err(d, r)
+    assert max(errs.values()) <= TOL, errs
+    _check_monitors(ref, dev)
+
+
+@pytest.mark.parametrize("kind", ["pec", "pmc", "periodic"])
+def test_boundary_modes(kind):
+    print(f"case: boundary {kind}, 18^3 cells, dipole, 200 steps", flush=True)
+    dx = configs.DX
+    L = 9 * dx
+    scene = mw.Scene(domain=mw.Domain(((-L, L),) * 3), grid=mw.GridSpec.uniform(dx),
+                     boundary=mw.BoundarySpec.faces(default=kind, z="pml", num_layers=6) if kind != "periodic"
+                     else mw.BoundarySpec.faces(default="periodic", z="pml", num_layers=6))
+    scene.add_source(mw.PointDipole((0.3 * dx, 0.2 * dx, 0.1 * dx), (1.0, 0.5, 0.25), profile="ideal",
+                                    source_time=mw.GaussianPulse(configs.F0, configs.F0 / 4)))
+    scene.add_monitor(mw.PointMonitor("p", (2 * dx, -dx, 0.0), fields=("Ex", "Hz")))
+    plan, ref, dev = _run_both(scene, {"frequencies": (configs.F0,), "courant": 0.99}, 200)
+    _check_fields(ref, dev)
+    _check_monitors(ref, dev)
+
+
+def test_conductive_and_gaussian_dipole():
+    print("case: conductive box + gaussian-profile dipole, 150 steps", flush=True)
+    dx = configs.DX
+    L = 10 * dx
+    scene = mw.Scene(domain=mw.Domain(((-L, L),) * 3), grid=mw.GridSpec.uniform(dx),
+                     boundary=mw.BoundarySpec.pml(6))
+    scene.add_structure(mw.Structure(mw.Box((2 * dx, 0, 0), (6 * dx, 8 * dx, 8 * dx)),
+                                     mw.Material(eps_r=4.0, sigma_e=2e4)))
+    scene.add_source(mw.PointDipole((-3 * dx, 0, 0), "Ey", width=1.5 * dx,
+                                    source_time=mw.GaussianPulse(configs.F0, configs.F0 / 3)))
+    scene.add_monitor(mw.PlaneMonitor("pl", axis="x", position=4.3 * dx, fields=("Ey", "Ez", "Hy", "Hz"),
+                                      compute_flux=True))
+    plan, ref, dev = _run_both(scene, {"frequencies": (configs.F0, 1.1 * configs.F0), "courant": 0.95}, 150)
+    _check_fields(ref, dev)
+    _check_monitors(ref, dev)

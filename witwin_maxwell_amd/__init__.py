@@ -1,0 +1,34 @@
+"""MI355X-native FDTD runtime with the witwin.maxwell ``Scene -> Simulation.fdtd(...) -> Result`` API.
+
+The time-step loop (Yee E/H, CPML psi, Debye/Drude/Lorentz ADE, soft/TFSF sources,
+running DFT) and its adjoint run as hand-written HIP kernels for gfx950 behind a
+C ABI (``include/maxwell_fdtd.h``); this package is the host side.
+"""
+
+from .constants import C_0, EPSILON_0, ETA_0, MU_0
+from .geometry import Box, Sphere, Structure
+from .media import DebyePole, DrudePole, LorentzPole, Material
+from .monitors import FluxMonitor, PlaneMonitor, PointMonitor
+from .scene import BoundarySpec, Domain, GridSpec, MaterialRegion, Scene, SceneModule, prepare_scene
+from .sources import CW, TFSF, GaussianPulse, PlaneWave, PointDipole
+
+__all__ = [
+    "C_0", "EPSILON_0", "ETA_0", "MU_0",
+    "Box", "Sphere", "Structure",
+    "Material", "DebyePole", "DrudePole", "LorentzPole",
+    "PointMonitor", "PlaneMonitor", "FluxMonitor",
+    "Scene", "SceneModule", "MaterialRegion", "Domain", "GridSpec", "BoundarySpec", "prepare_scene",
+    "GaussianPulse", "CW", "PointDipole", "PlaneWave", "TFSF",
+]
+
+
+def __getattr__(name):
+    # Simulation/Result pull in the native runtime; import them lazily so that the
+    # scene API stays usable for compile-only work.
+    if name in ("Simulation", "FDTDConfig", "TimeConfig", "SpectralSampler"):
+        from . import simulation
+        return getattr(simulation, name)
+    if name == "Result":
+        from .result import Result
+        return Result
+    raise AttributeError(name)

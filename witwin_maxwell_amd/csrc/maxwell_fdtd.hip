@@ -1,0 +1,363 @@
+// C ABI entry points of the FDTD runtime (declarations: include/maxwell_fdtd.h).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cstdio>
+#include <string>
+
+#include "maxwell_fdtd.h"
+#include "terms.h"
+#include "yee.h"
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_error = buf;
+  return code;
+}
+
+int check_launch(const char* what) {
+  hipError_t err = hipGetLastError();
+  if (err != hipSuccess) return fail(MW_ERR_LAUNCH, "%s: launch failed: %s", what, hipGetErrorString(err));
+  return MW_OK;
+}
+
+constexpr int64_t kMaxElems = (int64_t(1) << 31) - 1;
+
+int check_grid(const mw_grid* g) {
+  if (!g) return fail(MW_ERR_ARGUMENT, "grid is NULL");
+  if (g->nx < 2 || g->ny < 2 || g->nz < 2) return fail(MW_ERR_ARGUMENT, "grid needs >= 2 nodes per axis, got (%d, %d, %d)", g->nx, g->ny, g->nz);
+  if ((int64_t)g->nx * g->ny * g->nz > kMaxElems) return fail(MW_ERR_ARGUMENT, "grid has more than 2^31 nodes");
+  for (int a = 0; a < 3; ++a) {
+    if (!g->inv_e[a] || !g->inv_h[a]) return fail(MW_ERR_ARGUMENT, "grid spacing array %d is NULL", a);
+    for (int s = 0; s < 2; ++s) {
+      const int m = s ? g->bc_hi[a] : g->bc_lo[a];
+      if (m != MW_BC_NONE && m != MW_BC_PML && m != MW_BC_PERIODIC && m != MW_BC_PEC && m != MW_BC_PMC)
+        return fail(MW_ERR_ARGUMENT, "unsupported boundary code %d on axis %d", m, a);
+    }
+  }
+  return MW_OK;
+}
+
+void component_shape(const mw_grid* g, int c, int out[3]) {
+  out[0] = g->nx; out[1] = g->ny; out[2] = g->nz;
+  switch (c) {
+    case 0: out[0] -= 1; break;
+    case 1: out[1] -= 1; break;
+    case 2: out[2] -= 1; break;
+    case 3: out[1] -= 1; out[2] -= 1; break;
+    case 4: out[0] -= 1; out[2] -= 1; break;
+    default: out[0] -= 1; out[1] -= 1; break;
+  }
+}
+
+// psi q -> (component, axis)
+constexpr int kPsiComp[12] = {0, 0, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5};
+constexpr int kPsiAxis[12] = {1, 2, 0, 2, 0, 1, 1, 2, 0, 2, 0, 1};
+
+int check_cpml(const mw_grid* g, const mw_cpml* c, bool electric) {
+  if (!c || !c->enabled) return MW_OK;
+  for (int a = 0; a < 3; ++a) {
+    if (electric && (!c->inv_kappa_e[a] || !c->b_e[a] || !c->c_e[a])) return fail(MW_ERR_ARGUMENT, "CPML E profile %d is NULL", a);
+    if (!electric && (!c->inv_kappa_h[a] || !c->b_h[a] || !c->c_h[a])) return fail(MW_ERR_ARGUMENT, "CPML H profile %d is NULL", a);
+  }
+  for (int q = electric ? 0 : 6; q < (electric ? 6 : 12); ++q) {
+    int shp[3];
+    component_shape(g, kPsiComp[q], shp);
+    const int size = shp[kPsiAxis[q]];
+    const int low = c->psi_low[q], hs = c->psi_high_start[q], hl = c->psi_high_len[q];
+    if (low < 0 || hl < 0 || low > size || hs < low || hs + hl > size)
+      return fail(MW_ERR_ARGUMENT, "psi %d slab layout (%d, %d, %d) does not fit axis size %d", q, low, hs, hl, size);
+    if ((low + hl) > 0 && !c->psi[q]) return fail(MW_ERR_ARGUMENT, "psi %d is NULL", q);
+  }
+  return MW_OK;
+}
+
+int pick_xchunk(const mw_grid* g) {
+  // enough blocks to fill 256 CUs several times over, long enough x-runs to
+  // amortise the carried-neighbour preload
+  const int64_t cols = (int64_t)((g->nz + mw::kTileZ - 1) / mw::kTileZ) * ((g->ny + mw::kTileY - 1) / mw::kTileY);
+  int chunk = 16;
+  while (chunk > 2 && cols * ((g->nx + chunk - 1) / chunk) < 4096) chunk /= 2;
+  return chunk;
+}
+
+int launch_h(const mw_grid* g, const mw_coeffs* cf, const mw_cpml* cp, const mw_fields* f, int mask, hipStream_t s) {
+  const int chunk = pick_xchunk(g);
+  dim3 block(mw::kTileZ, mw::kTileY, 1);
+  dim3 grid((g->nz + mw::kTileZ - 1) / mw::kTileZ, (g->ny + mw::kTileY - 1) / mw::kTileY, (g->nx + chunk - 1) / chunk);
+  mw_cpml cpv{};
+  if (cp) cpv = *cp;
+  if (cpv.enabled) hipLaunchKernelGGL((mw::update_h_kernel<true>), grid, block, 0, s, *g, *cf, cpv, *f, mask, chunk);
+  else hipLaunchKernelGGL((mw::update_h_kernel<false>), grid, block, 0, s, *g, *cf, cpv, *f, mask, chunk);
+  return check_launch("update_magnetic");
+}
+
+int launch_e(const mw_grid* g, const mw_coeffs* cf, const mw_cpml* cp, const mw_ade* ad, const mw_fields* f, int mask,
+             hipStream_t s) {
+  const int chunk = pick_xchunk(g);
+  dim3 block(mw::kTileZ, mw::kTileY, 1);
+  dim3 grid((g->nz + mw::kTileZ - 1) / mw::kTileZ, (g->ny + mw::kTileY - 1) / mw::kTileY, (g->nx + chunk - 1) / chunk);
+  mw_cpml cpv{};
+  if (cp) cpv = *cp;
+  mw_ade adv{};
+  const bool ade = ad && (ad->count[0] + ad->count[1] + ad->count[2]) > 0;
+  if (ade) adv = *ad;
+  if (cpv.enabled) {
+    if (ade) hipLaunchKernelGGL((mw::update_e_kernel<true, true>), grid, block, 0, s, *g, *cf, cpv, adv, *f, mask, chunk);
+    else hipLaunchKernelGGL((mw::update_e_kernel<true, false>), grid, block, 0, s, *g, *cf, cpv, adv, *f, mask, chunk);
+  } else {
+    if (ade) hipLaunchKernelGGL((mw::update_e_kernel<false, true>), grid, block, 0, s, *g, *cf, cpv, adv, *f, mask, chunk);
+    else hipLaunchKernelGGL((mw::update_e_kernel<false, false>), grid, block, 0, s, *g, *cf, cpv, adv, *f, mask, chunk);
+  }
+  return check_launch("update_electric");
+}
+
+int check_fields(const mw_fields* f, bool e, bool h) {
+  if (!f) return fail(MW_ERR_ARGUMENT, "fields is NULL");
+  for (int c = 0; c < 3; ++c) {
+    if (!f->e[c] || !f->h[c]) return fail(MW_ERR_ARGUMENT, "field pointer %d is NULL", c);
+  }
+  (void)e; (void)h;
+  return MW_OK;
+}
+
+int check_ade(const mw_grid* g, const mw_ade* a) {
+  if (!a) return MW_OK;
+  for (int c = 0; c < 3; ++c) {
+    if (a->count[c] < 0 || a->count[c] > MW_MAX_POLES) return fail(MW_ERR_ARGUMENT, "ADE pole count %d out of range", a->count[c]);
+    if (a->count[c] > 0 && !a->inv_eps[c]) return fail(MW_ERR_ARGUMENT, "ADE inv_eps %d is NULL", c);
+    for (int p = 0; p < a->count[c]; ++p) {
+      const mw_pole& pl = a->poles[c][p];
+      if (pl.kind < 0 || pl.kind > 2) return fail(MW_ERR_ARGUMENT, "ADE pole kind %d invalid", pl.kind);
+      if (!pl.drive || !pl.current || (pl.kind != 1 && !pl.polarization))
+        return fail(MW_ERR_ARGUMENT, "ADE pole (%d, %d) has a NULL state array", c, p);
+    }
+  }
+  (void)g;
+  return MW_OK;
+}
+
+int grid_blocks(int64_t n) { return (int)((n + 255) / 256); }
+
+mw::FieldSet field_set(const mw_grid* g, const mw_fields* f) {
+  mw::FieldSet fs;
+  for (int c = 0; c < 6; ++c) {
+    fs.ptr[c] = c < 3 ? f->e[c] : f->h[c - 3];
+    component_shape(g, c, fs.shape[c]);
+  }
+  return fs;
+}
+
+int check_terms(const mw_grid* g, const mw_term* t, int n, const char* which, int total_steps) {
+  if (n < 0 || n > MW_MAX_TERMS) return fail(MW_ERR_ARGUMENT, "%s: %d terms (max %d)", which, n, MW_MAX_TERMS);
+  if (n > 0 && !t) return fail(MW_ERR_ARGUMENT, "%s: term array is NULL", which);
+  for (int i = 0; i < n; ++i) {
+    const mw_term& x = t[i];
+    if (x.component < 0 || x.component > 5) return fail(MW_ERR_ARGUMENT, "%s[%d]: bad component", which, i);
+    int shp[3];
+    component_shape(g, x.component, shp);
+    for (int a = 0; a < 3; ++a) {
+      if (x.shape[a] <= 0 || x.offset[a] < 0 || x.offset[a] + x.shape[a] > shp[a])
+        return fail(MW_ERR_ARGUMENT, "%s[%d]: patch [%d, %d) exceeds axis %d of size %d", which, i, x.offset[a],
+                    x.offset[a] + x.shape[a], a, shp[a]);
+    }
+    if (!x.patch) return fail(MW_ERR_ARGUMENT, "%s[%d]: patch is NULL", which, i);
+    if (x.kind == MW_TERM_ADDITIVE && !x.signal_table) return fail(MW_ERR_ARGUMENT, "%s[%d]: signal table is NULL", which, i);
+    if (x.kind == MW_TERM_SHIFTED && !x.delay) return fail(MW_ERR_ARGUMENT, "%s[%d]: delay is NULL", which, i);
+    if (x.kind != MW_TERM_ADDITIVE && x.kind != MW_TERM_SHIFTED) return fail(MW_ERR_ARGUMENT, "%s[%d]: bad kind", which, i);
+  }
+  (void)total_steps;
+  return MW_OK;
+}
+
+int launch_terms(const mw_grid* g, const mw_fields* f, const mw_term* t, int n, int step, double dt, hipStream_t s) {
+  if (n <= 0) return MW_OK;
+  mw::TermSet set{};
+  set.count = n;
+  int64_t cells = 0;
+  for (int i = 0; i < n; ++i) {
+    set.t[i] = t[i];
+    cells = std::max<int64_t>(cells, (int64_t)t[i].shape[0] * t[i].shape[1] * t[i].shape[2]);
+  }
+  dim3 grid(grid_blocks(cells), n, 1);
+  hipLaunchKernelGGL(mw::apply_terms_kernel, grid, dim3(256), 0, s, set, field_set(g, f), step, dt);
+  return check_launch("apply_terms");
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* mw_last_error(void) { return g_error.c_str(); }
+
+int32_t mw_abi_version(void) { return 1; }
+
+int32_t mw_abi_struct_sizes(int64_t* out, int32_t capacity) {
+  const int64_t sizes[] = {(int64_t)sizeof(mw_grid), (int64_t)sizeof(mw_coeffs), (int64_t)sizeof(mw_cpml),
+                           (int64_t)sizeof(mw_fields), (int64_t)sizeof(mw_pole), (int64_t)sizeof(mw_ade),
+                           (int64_t)sizeof(mw_term), (int64_t)sizeof(mw_plane_obs), (int64_t)sizeof(mw_program)};
+  const int32_t n = (int32_t)(sizeof(sizes) / sizeof(sizes[0]));
+  for (int32_t i = 0; i < n && i < capacity; ++i) out[i] = sizes[i];
+  return n;
+}
+
+int mw_update_magnetic(const mw_grid* grid, const mw_coeffs* coeffs, const mw_cpml* cpml, const mw_fields* fields,
+                       int32_t comp_mask, void* stream) {
+  int rc;
+  if ((rc = check_grid(grid)) || (rc = check_fields(fields, false, true)) || (rc = check_cpml(grid, cpml, false))) return rc;
+  if (!coeffs) return fail(MW_ERR_ARGUMENT, "coeffs is NULL");
+  return launch_h(grid, coeffs, cpml, fields, comp_mask & 7, (hipStream_t)stream);
+}
+
+int mw_update_electric(const mw_grid* grid, const mw_coeffs* coeffs, const mw_cpml* cpml, const mw_ade* ade,
+                       const mw_fields* fields, int32_t comp_mask, void* stream) {
+  int rc;
+  if ((rc = check_grid(grid)) || (rc = check_fields(fields, true, false)) || (rc = check_cpml(grid, cpml, true)) ||
+      (rc = check_ade(grid, ade)))
+    return rc;
+  if (!coeffs) return fail(MW_ERR_ARGUMENT, "coeffs is NULL");
+  return launch_e(grid, coeffs, cpml, ade, fields, comp_mask & 7, (hipStream_t)stream);
+}
+
+int mw_add_source_patch(float* field, int32_t fnx, int32_t fny, int32_t fnz, const float* patch, int32_t pnx,
+                        int32_t pny, int32_t pnz, int32_t oi, int32_t oj, int32_t ok, float signal, void* stream) {
+  if (!field || !patch) return fail(MW_ERR_ARGUMENT, "add_source_patch: NULL pointer");
+  if (pnx <= 0 || pny <= 0 || pnz <= 0 || oi < 0 || oj < 0 || ok < 0 || oi + pnx > fnx || oj + pny > fny ||
+      ok + pnz > fnz)
+    return fail(MW_ERR_ARGUMENT, "add_source_patch: patch (%d,%d,%d)+(%d,%d,%d) outside field (%d,%d,%d)", oi, oj, ok,
+                pnx, pny, pnz, fnx, fny, fnz);
+  const int64_t cells = (int64_t)pnx * pny * pnz;
+  hipLaunchKernelGGL(mw::add_patch_kernel, dim3(grid_blocks(cells)), dim3(256), 0, (hipStream_t)stream, field, fny, fnz,
+                     patch, pnx, pny, pnz, oi, oj, ok, signal);
+  return check_launch("add_source_patch");
+}
+
+int mw_add_time_shifted_patch(float* field, int32_t fnx, int32_t fny, int32_t fnz, const float* patch,
+                              const double* delay, int32_t pnx, int32_t pny, int32_t pnz, int32_t oi, int32_t oj,
+                              int32_t ok, double time, int32_t kind, double frequency, double fwidth,
+                              double amplitude, double phase, double delay0, void* stream) {
+  if (!field || !patch || !delay) return fail(MW_ERR_ARGUMENT, "add_time_shifted_patch: NULL pointer");
+  if (pnx <= 0 || pny <= 0 || pnz <= 0 || oi < 0 || oj < 0 || ok < 0 || oi + pnx > fnx || oj + pny > fny ||
+      ok + pnz > fnz)
+    return fail(MW_ERR_ARGUMENT, "add_time_shifted_patch: patch outside field");
+  if (kind < 0 || kind > 2) return fail(MW_ERR_ARGUMENT, "add_time_shifted_patch: waveform kind %d", kind);
+  const int64_t cells = (int64_t)pnx * pny * pnz;
+  hipLaunchKernelGGL(mw::add_shifted_kernel, dim3(grid_blocks(cells)), dim3(256), 0, (hipStream_t)stream, field, fny,
+                     fnz, patch, delay, pnx, pny, pnz, oi, oj, ok, time, kind, frequency, fwidth, amplitude, phase,
+                     delay0);
+  return check_launch("add_time_shifted_patch");
+}
+
+int mw_accumulate_dft(const float* field, int64_t count, int32_t nf, const float* wcos, const float* wsin, float* re,
+                      float* im, void* stream) {
+  if (!field || !wcos || !wsin || !re || !im) return fail(MW_ERR_ARGUMENT, "accumulate_dft: NULL pointer");
+  if (count <= 0 || count > kMaxElems || nf <= 0) return fail(MW_ERR_ARGUMENT, "accumulate_dft: bad sizes");
+  hipLaunchKernelGGL(mw::accumulate_kernel, dim3(grid_blocks(count)), dim3(256), 0, (hipStream_t)stream, field, count,
+                     nf, wcos, wsin, re, im);
+  return check_launch("accumulate_dft");
+}
+
+int mw_accumulate_plane(const float* field, int32_t nx, int32_t ny, int32_t nz, int32_t axis, int32_t index,
+                        int32_t nf, const float* wcos, const float* wsin, float* re, float* im, void* stream) {
+  if (!field || !wcos || !wsin || !re || !im) return fail(MW_ERR_ARGUMENT, "accumulate_plane: NULL pointer");
+  if (axis < 0 || axis > 2 || nf <= 0) return fail(MW_ERR_ARGUMENT, "accumulate_plane: bad axis/nf");
+  const int size = axis == 0 ? nx : (axis == 1 ? ny : nz);
+  if (index < 0 || index >= size) return fail(MW_ERR_ARGUMENT, "accumulate_plane: index %d outside [0,%d)", index, size);
+  const int64_t n = axis == 0 ? (int64_t)ny * nz : (axis == 1 ? (int64_t)nx * nz : (int64_t)nx * ny);
+  hipLaunchKernelGGL(mw::accumulate_plane_single_kernel, dim3(grid_blocks(n)), dim3(256), 0, (hipStream_t)stream, field,
+                     nx, ny, nz, axis, index, nf, wcos, wsin, re, im);
+  return check_launch("accumulate_plane");
+}
+
+int mw_accumulate_points(const float* field, const int32_t* linear_index, int32_t count, int32_t nf, const float* wcos,
+                         const float* wsin, float* re, float* im, void* stream) {
+  if (!field || !linear_index || !wcos || !wsin || !re || !im) return fail(MW_ERR_ARGUMENT, "accumulate_points: NULL pointer");
+  if (count <= 0 || nf <= 0) return fail(MW_ERR_ARGUMENT, "accumulate_points: bad sizes");
+  hipLaunchKernelGGL(mw::accumulate_points_single_kernel, dim3((count + 63) / 64), dim3(64), 0, (hipStream_t)stream,
+                     field, linear_index, count, nf, wcos, wsin, re, im);
+  return check_launch("accumulate_points");
+}
+
+int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream) {
+  if (!p) return fail(MW_ERR_ARGUMENT, "program is NULL");
+  int rc;
+  const mw_grid* g = &p->grid;
+  if ((rc = check_grid(g)) || (rc = check_fields(&p->fields, true, true)) || (rc = check_cpml(g, &p->cpml, true)) ||
+      (rc = check_cpml(g, &p->cpml, false)) || (rc = check_ade(g, p->has_ade ? &p->ade : nullptr)))
+    return rc;
+  if (first < 0 || count < 0 || first + count > p->total_steps)
+    return fail(MW_ERR_ARGUMENT, "steps [%d, %d) outside the program's %d steps", first, first + count, p->total_steps);
+  if ((rc = check_terms(g, p->h_terms, p->n_h_terms, "h_terms", p->total_steps)) ||
+      (rc = check_terms(g, p->e_terms, p->n_e_terms, "e_terms", p->total_steps)))
+    return rc;
+  if (p->n_planes < 0 || p->n_planes > MW_MAX_PLANES) return fail(MW_ERR_ARGUMENT, "%d planes (max %d)", p->n_planes, MW_MAX_PLANES);
+  mw::PlaneSet planes{};
+  planes.count = p->n_planes;
+  int64_t plane_cells = 0;
+  for (int q = 0; q < p->n_planes; ++q) {
+    const mw_plane_obs& o = p->planes[q];
+    if (o.component < 0 || o.component > 5 || o.axis < 0 || o.axis > 2 || !o.re || !o.im)
+      return fail(MW_ERR_ARGUMENT, "plane %d descriptor invalid", q);
+    int shp[3];
+    component_shape(g, o.component, shp);
+    if (o.index < 0 || o.index >= shp[o.axis]) return fail(MW_ERR_ARGUMENT, "plane %d index %d outside axis", q, o.index);
+    const int64_t n = o.axis == 0 ? (int64_t)shp[1] * shp[2] : (o.axis == 1 ? (int64_t)shp[0] * shp[2] : (int64_t)shp[0] * shp[1]);
+    plane_cells = std::max(plane_cells, n);
+    planes.p[q] = o;
+  }
+  if ((p->n_planes > 0 || p->n_points > 0) && (p->nf_obs <= 0 || !p->obs_e_cos || !p->obs_e_sin || !p->obs_h_cos || !p->obs_h_sin))
+    return fail(MW_ERR_ARGUMENT, "observer weight tables missing");
+  if (p->n_points > 0 && (!p->point_component || !p->point_linear || !p->point_re || !p->point_im))
+    return fail(MW_ERR_ARGUMENT, "point observer arrays missing");
+  if (p->nf_dft > 0) {
+    if (!p->dft_cos || !p->dft_sin) return fail(MW_ERR_ARGUMENT, "DFT weight tables missing");
+    for (int c = 0; c < 3; ++c)
+      if (!p->dft_re[c] || !p->dft_im[c]) return fail(MW_ERR_ARGUMENT, "DFT accumulator %d missing", c);
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const mw::FieldSet fs = field_set(g, &p->fields);
+  const mw_ade* ade = p->has_ade ? &p->ade : nullptr;
+  for (int n = first; n < first + count; ++n) {
+    if ((rc = launch_h(g, &p->coeffs, &p->cpml, &p->fields, 7, s))) return rc;
+    if ((rc = launch_terms(g, &p->fields, p->h_terms, p->n_h_terms, n, p->dt, s))) return rc;
+    if ((rc = launch_e(g, &p->coeffs, &p->cpml, ade, &p->fields, 7, s))) return rc;
+    if ((rc = launch_terms(g, &p->fields, p->e_terms, p->n_e_terms, n, p->dt, s))) return rc;
+    if (p->nf_dft > 0) {
+      for (int c = 0; c < 3; ++c) {
+        int shp[3];
+        component_shape(g, c, shp);
+        const int64_t cnt = (int64_t)shp[0] * shp[1] * shp[2];
+        hipLaunchKernelGGL(mw::accumulate_kernel, dim3(grid_blocks(cnt)), dim3(256), 0, s, fs.ptr[c], cnt, p->nf_dft,
+                           p->dft_cos + (int64_t)n * p->nf_dft, p->dft_sin + (int64_t)n * p->nf_dft, p->dft_re[c],
+                           p->dft_im[c]);
+      }
+      if ((rc = check_launch("accumulate_dft"))) return rc;
+    }
+    if (p->n_planes > 0) {
+      const int64_t off = (int64_t)n * p->nf_obs;
+      hipLaunchKernelGGL(mw::accumulate_planes_kernel, dim3(grid_blocks(plane_cells), p->n_planes), dim3(256), 0, s,
+                         planes, fs, p->nf_obs, p->obs_e_cos + off, p->obs_e_sin + off, p->obs_h_cos + off,
+                         p->obs_h_sin + off);
+      if ((rc = check_launch("accumulate_planes"))) return rc;
+    }
+    if (p->n_points > 0) {
+      const int64_t off = (int64_t)n * p->nf_obs;
+      hipLaunchKernelGGL(mw::accumulate_points_kernel, dim3((p->n_points + 63) / 64), dim3(64), 0, s, fs, p->n_points,
+                         p->point_component, p->point_linear, p->nf_obs, p->obs_e_cos + off, p->obs_e_sin + off,
+                         p->obs_h_cos + off, p->obs_h_sin + off, p->point_re, p->point_im);
+      if ((rc = check_launch("accumulate_points"))) return rc;
+    }
+  }
+  return MW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,187 @@
+// Source terms and running-DFT accumulation kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "maxwell_fdtd.h"
+
+namespace mw {
+
+// Waveforms of kernels/sources.cu:71-93, evaluated in double: the argument
+// t - delay spans thousands of periods, where float32 loses ~1e-5 of phase.
+__device__ __forceinline__ double waveform(int kind, double t, double freq, double fwidth, double amp, double phase,
+                                           double delay0) {
+  const double two_pi = 6.283185307179586476925286766559;
+  if (kind == 0) return amp * cos(two_pi * freq * t + phase);
+  const double tau = t - delay0;
+  if (kind == 1) {
+    const double inv_sigma = fmax(two_pi * fwidth, 1e-30);
+    const double x = tau * inv_sigma;
+    return amp * exp(-0.5 * x * x) * cos(two_pi * freq * tau + phase);
+  }
+  const double a = 3.14159265358979323846 * freq * tau;
+  return amp * (1.0 - 2.0 * a * a) * exp(-a * a);
+}
+
+struct TermSet {
+  int32_t count;
+  int32_t pad;
+  mw_term t[MW_MAX_TERMS];
+};
+
+struct FieldSet {
+  float* ptr[6];
+  int32_t shape[6][3];
+};
+
+// One launch applies every term of a set: blockIdx.y = term, x = patch cell.
+__global__ __launch_bounds__(256) void apply_terms_kernel(TermSet set, FieldSet fs, int step, double dt) {
+  const int ti = blockIdx.y;
+  if (ti >= set.count) return;
+  const mw_term& t = set.t[ti];
+  const int64_t cells = (int64_t)t.shape[0] * t.shape[1] * t.shape[2];
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= cells) return;
+  const int pz = (int)(p % t.shape[2]);
+  const int py = (int)((p / t.shape[2]) % t.shape[1]);
+  const int px = (int)(p / ((int64_t)t.shape[2] * t.shape[1]));
+  const int c = t.component;
+  const int fy = fs.shape[c][1], fz = fs.shape[c][2];
+  const uint32_t o = ((uint32_t)(t.offset[0] + px) * fy + (t.offset[1] + py)) * fz + (t.offset[2] + pz);
+  float value;
+  if (t.kind == MW_TERM_ADDITIVE) {
+    value = t.signal_table[step] * t.patch[p];
+  } else {
+    const double time = (double)step * dt + t.t_offset - t.delay[p];
+    value = (float)(waveform(t.wave_kind, time, t.frequency, t.fwidth, t.amplitude, t.phase, t.delay0) *
+                    (double)t.patch[p]);
+  }
+  fs.ptr[c][o] += value;
+}
+
+__global__ __launch_bounds__(256) void add_patch_kernel(float* field, int fy, int fz, const float* patch, int pnx,
+                                                         int pny, int pnz, int oi, int oj, int ok, float signal) {
+  const int64_t cells = (int64_t)pnx * pny * pnz;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= cells) return;
+  const int pz = (int)(p % pnz);
+  const int py = (int)((p / pnz) % pny);
+  const int px = (int)(p / ((int64_t)pnz * pny));
+  field[((uint32_t)(oi + px) * fy + (oj + py)) * fz + (ok + pz)] += signal * patch[p];
+}
+
+__global__ __launch_bounds__(256) void add_shifted_kernel(float* field, int fy, int fz, const float* patch,
+                                                           const double* delay, int pnx, int pny, int pnz, int oi,
+                                                           int oj, int ok, double time, int kind, double freq,
+                                                           double fwidth, double amp, double phase, double delay0) {
+  const int64_t cells = (int64_t)pnx * pny * pnz;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= cells) return;
+  const int pz = (int)(p % pnz);
+  const int py = (int)((p / pnz) % pny);
+  const int px = (int)(p / ((int64_t)pnz * pny));
+  const double s = waveform(kind, time - delay[p], freq, fwidth, amp, phase, delay0);
+  field[((uint32_t)(oi + px) * fy + (oj + py)) * fz + (ok + pz)] += (float)(s * (double)patch[p]);
+}
+
+// ------------------------------------------------------------ spectral sums
+// re[f][l] += v[l]*wc[f]; the weights of one step are read once per block.
+__global__ __launch_bounds__(256) void accumulate_kernel(const float* __restrict__ v, int64_t n, int nf,
+                                                         const float* __restrict__ wc, const float* __restrict__ ws,
+                                                         float* __restrict__ re, float* __restrict__ im) {
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n) return;
+  const float x = v[l];
+  for (int f = 0; f < nf; ++f) {
+    const float c = wc[f], s = ws[f];
+    re[f * n + l] += x * c;
+    im[f * n + l] += x * s;
+  }
+}
+
+struct PlaneSet {
+  int32_t count;
+  int32_t pad;
+  mw_plane_obs p[MW_MAX_PLANES];
+};
+
+// All plane observers of one step in one launch: blockIdx.y = plane.
+__global__ __launch_bounds__(256) void accumulate_planes_kernel(PlaneSet set, FieldSet fs, int nf,
+                                                                const float* __restrict__ e_cos,
+                                                                const float* __restrict__ e_sin,
+                                                                const float* __restrict__ h_cos,
+                                                                const float* __restrict__ h_sin) {
+  const int qi = blockIdx.y;
+  if (qi >= set.count) return;
+  const mw_plane_obs& q = set.p[qi];
+  const int c = q.component;
+  const int sx = fs.shape[c][0], sy = fs.shape[c][1], sz = fs.shape[c][2];
+  int a, b;
+  if (q.axis == 0) { a = sy; b = sz; } else if (q.axis == 1) { a = sx; b = sz; } else { a = sx; b = sy; }
+  const int64_t n = (int64_t)a * b;
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n) return;
+  const int ia = (int)(l / b), ib = (int)(l % b);
+  uint32_t o;
+  if (q.axis == 0) o = ((uint32_t)q.index * sy + ia) * sz + ib;
+  else if (q.axis == 1) o = ((uint32_t)ia * sy + q.index) * sz + ib;
+  else o = ((uint32_t)ia * sy + ib) * sz + q.index;
+  const float x = fs.ptr[c][o];
+  const float* wc = c < 3 ? e_cos : h_cos;
+  const float* ws = c < 3 ? e_sin : h_sin;
+  for (int f = 0; f < nf; ++f) {
+    q.re[f * n + l] += x * wc[f];
+    q.im[f * n + l] += x * ws[f];
+  }
+}
+
+__global__ __launch_bounds__(64) void accumulate_points_kernel(FieldSet fs, int count, const int32_t* comp,
+                                                               const int32_t* lin, int nf, const float* e_cos,
+                                                               const float* e_sin, const float* h_cos,
+                                                               const float* h_sin, float* re, float* im) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= count) return;
+  const int c = comp[p];
+  const float x = fs.ptr[c][lin[p]];
+  const float* wc = c < 3 ? e_cos : h_cos;
+  const float* ws = c < 3 ? e_sin : h_sin;
+  for (int f = 0; f < nf; ++f) {
+    re[f * count + p] += x * wc[f];
+    im[f * count + p] += x * ws[f];
+  }
+}
+
+__global__ __launch_bounds__(64) void accumulate_points_single_kernel(const float* field, const int32_t* lin,
+                                                                      int count, int nf, const float* wc,
+                                                                      const float* ws, float* re, float* im) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= count) return;
+  const float x = field[lin[p]];
+  for (int f = 0; f < nf; ++f) {
+    re[f * count + p] += x * wc[f];
+    im[f * count + p] += x * ws[f];
+  }
+}
+
+__global__ __launch_bounds__(256) void accumulate_plane_single_kernel(const float* field, int sx, int sy, int sz,
+                                                                      int axis, int index, int nf, const float* wc,
+                                                                      const float* ws, float* re, float* im) {
+  int a, b;
+  if (axis == 0) { a = sy; b = sz; } else if (axis == 1) { a = sx; b = sz; } else { a = sx; b = sy; }
+  const int64_t n = (int64_t)a * b;
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n) return;
+  const int ia = (int)(l / b), ib = (int)(l % b);
+  uint32_t o;
+  if (axis == 0) o = ((uint32_t)index * sy + ia) * sz + ib;
+  else if (axis == 1) o = ((uint32_t)ia * sy + index) * sz + ib;
+  else o = ((uint32_t)ia * sy + ib) * sz + index;
+  const float x = field[o];
+  for (int f = 0; f < nf; ++f) {
+    re[f * n + l] += x * wc[f];
+    im[f * n + l] += x * ws[f];
+  }
+}
+
+}  // namespace mw

@@ -1,0 +1,231 @@
+"""Material compilation: node permittivity, Yee-edge averaging, update
+coefficients and ADE pole templates.
+
+Follows
+  * node->edge averaging ``average_node_to_component`` (fdtd/runtime/materials.py:399-407)
+    and ``average_node_to_magnetic_component`` (:420-456);
+  * semi-implicit lossy coefficients ``_electric_update_coefficients`` /
+    ``_magnetic_update_coefficients`` (:942-1008) with CPML's ``pml_decay=None``
+    (:1577-1594);
+  * ADE constants ``_build_family_dispersive_templates`` (:481-600);
+  * ``MaterialRegion`` density preparation, trilinear sampling and blending
+    (compiler/materials.py:1310-1470).
+
+All host arithmetic is float64; the device receives float32 copies.  Structure
+occupancy is the hard-voxel node rasteriser of ``geometry.rasterize_nodes`` (the
+reference's occupancy lives in the absent ``witwin.core``).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+import torch.nn.functional as F
+
+from ..constants import EPSILON_0, MU_0
+from ..geometry import Box, rasterize_nodes
+
+E_COMPONENTS = ("Ex", "Ey", "Ez")
+H_COMPONENTS = ("Hx", "Hy", "Hz")
+
+
+def average_node_to_component(node, component):
+    """Two-node average onto the E edge (materials.py:399-407)."""
+    if component == "Ex":
+        return 0.5 * (node[:-1, :, :] + node[1:, :, :])
+    if component == "Ey":
+        return 0.5 * (node[:, :-1, :] + node[:, 1:, :])
+    if component == "Ez":
+        return 0.5 * (node[:, :, :-1] + node[:, :, 1:])
+    raise ValueError(component)
+
+
+def average_node_to_magnetic_component(node, component):
+    """Four-node average onto the H face (materials.py:420-456)."""
+    if component == "Hx":
+        return 0.25 * (node[:, :-1, :-1] + node[:, :-1, 1:] + node[:, 1:, :-1] + node[:, 1:, 1:])
+    if component == "Hy":
+        return 0.25 * (node[:-1, :, :-1] + node[:-1, :, 1:] + node[1:, :, :-1] + node[1:, :, 1:])
+    if component == "Hz":
+        return 0.25 * (node[:-1, :-1, :] + node[:-1, 1:, :] + node[1:, :-1, :] + node[1:, 1:, :])
+    raise ValueError(component)
+
+
+def component_shape(component, shape):
+    nx, ny, nz = shape
+    return {"Ex": (nx - 1, ny, nz), "Ey": (nx, ny - 1, nz), "Ez": (nx, ny, nz - 1),
+            "Hx": (nx, ny - 1, nz - 1), "Hy": (nx - 1, ny, nz - 1), "Hz": (nx - 1, ny - 1, nz)}[component]
+
+
+class NodeMaterialModel:
+    """Node-sampled eps_r, mu_r, sigma_e and pole weights for a prepared scene."""
+
+    def __init__(self, prepared):
+        self.prepared = prepared
+        shape = prepared.shape
+        self.eps_r = np.ones(shape)
+        self.mu_r = np.ones(shape)
+        self.sigma_e = np.zeros(shape)
+        self.poles = []  # list of (kind, pole, node weight array)
+        self.region_mask = np.zeros(shape, dtype=bool)
+        x, y, z = prepared.x_nodes64, prepared.y_nodes64, prepared.z_nodes64
+        pole_index = {}
+        for structure in prepared.scene.structures:
+            mask = rasterize_nodes(structure.geometry, x, y, z)
+            if not mask.any():
+                continue
+            m = structure.material
+            self.eps_r[mask] = m.eps_r
+            self.mu_r[mask] = m.mu_r
+            self.sigma_e[mask] = m.sigma_e
+            # a later structure replaces whatever poles an earlier one put on these nodes
+            for _, _, weight in self.poles:
+                weight[mask] = 0.0
+            for kind, plist in (("debye", m.debye_poles), ("drude", m.drude_poles), ("lorentz", m.lorentz_poles)):
+                for pole in plist:
+                    key = (kind, pole)
+                    if key not in pole_index:
+                        pole_index[key] = len(self.poles)
+                        self.poles.append((kind, pole, np.zeros(shape)))
+                    self.poles[pole_index[key]][2][mask] = 1.0
+        self.poles = [p for p in self.poles if p[2].any()]
+
+    @property
+    def dispersive(self):
+        return bool(self.poles)
+
+
+def prepare_region_density(prepared, region) -> torch.Tensor:
+    """Normalise, filter and project a region density (compiler/materials.py:1310-1360)."""
+    if not isinstance(region.geometry, Box):
+        raise ValueError("MaterialRegion currently supports Box geometry only.")
+    density = region.density.to(dtype=torch.float64)
+    lo, hi = region.bounds
+    if not np.isclose(lo, 0.0) or not np.isclose(hi, 1.0):
+        density = (density - lo) / max(hi - lo, 1e-12)
+    density = density.clamp(0.0, 1.0)
+    if region.filter_radius is not None:
+        spacing = [float(np.min(prepared.primal(a))) for a in "xyz"]
+        kernel = tuple(max(1, 2 * int(np.ceil(region.filter_radius / s)) + 1) for s in spacing)
+        kernel = tuple(min(k, n if n % 2 == 1 else max(1, n - 1)) for k, n in zip(kernel, density.shape))
+        if kernel != (1, 1, 1):
+            density = F.avg_pool3d(density[None, None], kernel_size=kernel, stride=1,
+                                   padding=tuple(k // 2 for k in kernel))[0, 0]
+    if region.projection_beta is not None:
+        beta = float(region.projection_beta)
+        mid = density.new_tensor(0.5)
+        num = torch.tanh(beta * mid) + torch.tanh(beta * (density - mid))
+        den = torch.tanh(beta * mid) + torch.tanh(beta * (1.0 - mid))
+        density = num / den
+    return density.clamp(0.0, 1.0)
+
+
+def sample_region_density(region, density, x, y, z):
+    """Trilinear ``grid_sample`` of the density at node coordinates inside the
+    region's box window (compiler/materials.py:1363-1390); returns the window
+    slices and the sampled float64 tensor."""
+    (bx0, bx1), (by0, by1), (bz0, bz1) = region.geometry.bounds
+    i0, i1 = np.searchsorted(x, bx0, "left"), np.searchsorted(x, bx1, "right")
+    j0, j1 = np.searchsorted(y, by0, "left"), np.searchsorted(y, by1, "right")
+    k0, k1 = np.searchsorted(z, bz0, "left"), np.searchsorted(z, bz1, "right")
+    if i1 <= i0 or j1 <= j0 or k1 <= k0:
+        return None, None
+    cx, cy, cz = region.geometry.position
+    sx, sy, sz = region.geometry.size
+    dev = density.device
+    xs = torch.as_tensor(2.0 * (x[i0:i1] - cx) / sx, dtype=torch.float64, device=dev)
+    ys = torch.as_tensor(2.0 * (y[j0:j1] - cy) / sy, dtype=torch.float64, device=dev)
+    zs = torch.as_tensor(2.0 * (z[k0:k1] - cz) / sz, dtype=torch.float64, device=dev)
+    gx, gy, gz = torch.meshgrid(xs, ys, zs, indexing="ij")
+    grid = torch.stack((gx, gy, gz), dim=-1)[None]
+    texture = density.permute(2, 1, 0)[None, None]
+    sampled = F.grid_sample(texture, grid, mode="bilinear", padding_mode="border", align_corners=False)[0, 0]
+    return (slice(i0, i1), slice(j0, j1), slice(k0, k1)), sampled
+
+
+def node_permittivity(prepared, model: NodeMaterialModel, region_densities=None) -> torch.Tensor:
+    """Node relative permittivity as a float64 torch tensor; design regions are
+    blended in with hard occupancy (compiler/materials.py:1393-1470), keeping the
+    autograd graph back to the region densities."""
+    eps = torch.as_tensor(model.eps_r, dtype=torch.float64)
+    regions = list(prepared.scene.material_regions)
+    if not regions:
+        return eps
+    eps = eps.clone()
+    x, y, z = prepared.x_nodes64, prepared.y_nodes64, prepared.z_nodes64
+    densities = region_densities if region_densities is not None else [None] * len(regions)
+    for region, dens in zip(regions, densities):
+        if dens is None:
+            dens = prepare_region_density(prepared, region)
+        window, sampled = sample_region_density(region, dens.to("cpu"), x, y, z)
+        if window is None:
+            continue
+        occ = torch.as_tensor(region.geometry.contains(x[window[0]][:, None, None], y[window[1]][None, :, None],
+                                                       z[window[2]][None, None, :]), dtype=torch.float64)
+        lo_e, hi_e = region.eps_bounds
+        value = lo_e + sampled * (hi_e - lo_e)
+        eps[window] = eps[window] * (1.0 - occ) + occ * value
+        model.region_mask[window] |= occ.numpy().astype(bool)
+    return eps
+
+
+def update_coefficients(prepared, model: NodeMaterialModel, eps_node: np.ndarray, dt: float):
+    """Per-component (decay, curl) in float64 (materials.py:942-1008, :1577-1594)."""
+    coef = {}
+    eps_edges = {}
+    for comp in E_COMPONENTS:
+        eps = average_node_to_component(eps_node, comp) * EPSILON_0
+        sig = average_node_to_component(model.sigma_e, comp)
+        eps_edges[comp] = eps
+        if np.any(model.sigma_e != 0):
+            half = 0.5 * sig * dt / eps
+            coef[comp] = ((1.0 - half) / (1.0 + half), (dt / eps) / (1.0 + half))
+        else:
+            coef[comp] = (np.ones_like(eps), dt / eps)
+    for comp in H_COMPONENTS:
+        mu = average_node_to_magnetic_component(model.mu_r, comp) * MU_0
+        coef[comp] = (np.ones_like(mu), dt / mu)
+    return coef, eps_edges
+
+
+def dispersive_templates(model: NodeMaterialModel, eps_edges, dt: float):
+    """ADE constants per E component (materials.py:481-600): decay/restoring
+    scalars and the edge-averaged drive arrays, plus 1/eps_inf."""
+    out = {}
+    for comp in E_COMPONENTS:
+        entries = []
+        for kind, pole, weight_node in model.poles:
+            weight = average_node_to_component(weight_node, comp)
+            if kind == "debye":
+                decay = (2.0 * pole.tau - dt) / (2.0 * pole.tau + dt)
+                scale = 2.0 * EPSILON_0 * pole.delta_eps * dt / (2.0 * pole.tau + dt)
+                entries.append({"kind": "debye", "decay": decay, "restoring": 0.0, "drive": weight * scale})
+            elif kind == "drude":
+                wp = 2.0 * np.pi * pole.plasma_frequency
+                g = 2.0 * np.pi * pole.gamma
+                denom = 2.0 + g * dt
+                entries.append({"kind": "drude", "decay": (2.0 - g * dt) / denom, "restoring": 0.0,
+                                "drive": weight * (2.0 * EPSILON_0 * wp * wp * dt / denom)})
+            else:
+                w0 = 2.0 * np.pi * pole.resonance_frequency
+                g = 2.0 * np.pi * pole.gamma
+                denom = 2.0 + g * dt
+                entries.append({"kind": "lorentz", "decay": (2.0 - g * dt) / denom,
+                                "restoring": 2.0 * w0 * w0 * dt / denom,
+                                "drive": weight * (2.0 * EPSILON_0 * pole.delta_eps * w0 * w0 * dt / denom)})
+        out[comp] = {"inv_eps": 1.0 / eps_edges[comp], "poles": entries}
+    return out
+
+
+def material_characteristic_frequency(scene) -> float:
+    """Largest pole frequency in Hz (fdtd/runtime/initialization.py:24-69)."""
+    best = 0.0
+    for s in scene.structures:
+        m = s.material
+        for p in m.debye_poles:
+            best = max(best, 1.0 / (2.0 * np.pi * p.tau))
+        for p in m.drude_poles:
+            best = max(best, p.plasma_frequency, p.gamma)
+        for p in m.lorentz_poles:
+            best = max(best, p.resonance_frequency, p.gamma)
+    return best

@@ -1,0 +1,287 @@
+"""Device runtime: uploads an ``FDTDPlan`` into HBM and drives the native step loop.
+
+Replaces the reference's per-kernel Python orchestration
+(fdtd/runtime/stepping.py:2153-2416) with one native call per chunk of steps
+(``mw_run_steps``); per-step host scalars (source signals, DFT/observer weights)
+become float32 step tables in device memory, so a chunk carries no host input.
+Torch owns every buffer; the kernels run on torch's current stream.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import torch
+
+from . import native as nat
+from .cpml import PSI_NAMES, PSI_SPECS, compact_shape
+from .plan import COMPONENTS, FDTDPlan
+
+_KIND = {"debye": 0, "drude": 1, "lorentz": 2}
+
+
+def _f32(a, device):
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=device)
+
+
+def _uniform_value(arr):
+    a = np.asarray(arr, dtype=np.float32)
+    if a.size and a.min() == a.max():
+        return float(a.flat[0])
+    return None
+
+
+class DeviceRun:
+    def __init__(self, plan: FDTDPlan, device="cuda"):
+        self.plan = plan
+        self.device = torch.device(device)
+        if self.device.type != "cuda":
+            raise RuntimeError("the FDTD runtime runs on a HIP device; got " + str(self.device))
+        self.lib = nat.load_library()
+        dev = self.device
+        self.keep = []
+        nx, ny, nz = plan.shape
+        self.fields = {c: torch.zeros(plan.field_shape(c), dtype=torch.float32, device=dev) for c in COMPONENTS}
+
+        prog = nat.MwProgram()
+        g = prog.grid
+        g.nx, g.ny, g.nz = nx, ny, nz
+        for a in range(3):
+            g.bc_lo[a] = plan.bc_lo[a]
+            g.bc_hi[a] = plan.bc_hi[a]
+            g.inv_e[a] = self._dev(plan.inv_e[a])
+            g.inv_h[a] = self._dev(plan.inv_h[a])
+        cf = prog.coeffs
+        self.uniform = {}
+        for ci, c in enumerate(COMPONENTS):
+            dec, curl = plan.coef[c]
+            for name, arr, tgt_arr, tgt_val in (("decay", dec, cf.decay, cf.decay_value),
+                                                ("curl", curl, cf.curl, cf.curl_value)):
+                u = _uniform_value(arr)
+                self.uniform[(c, name)] = u
+                if u is not None:
+                    tgt_arr[ci] = None
+                    tgt_val[ci] = u
+                else:
+                    tgt_arr[ci] = self._dev(np.broadcast_to(arr, plan.field_shape(c)))
+                    tgt_val[ci] = 0.0
+        cp = prog.cpml
+        self.psi = {}
+        if plan.uses_cpml:
+            cp.enabled = 1
+            for a, ax in enumerate("xyz"):
+                p = plan.cpml[ax]
+                cp.inv_kappa_e[a] = self._dev(1.0 / p["kappa_e"])
+                cp.b_e[a] = self._dev(p["b_e"])
+                cp.c_e[a] = self._dev(p["c_e"])
+                cp.inv_kappa_h[a] = self._dev(1.0 / p["kappa_h"])
+                cp.b_h[a] = self._dev(p["b_h"])
+                cp.c_h[a] = self._dev(p["c_h"])
+            for q, name in enumerate(PSI_NAMES):
+                comp, _ = PSI_SPECS[name]
+                lay = plan.psi_layout[name]
+                shape = compact_shape(plan.field_shape(comp), lay)
+                t = torch.zeros(shape, dtype=torch.float32, device=dev)
+                self.psi[name] = t
+                cp.psi[q] = nat.ptr(t) if t.numel() else None
+                cp.psi_low[q] = lay[1]
+                cp.psi_high_start[q] = lay[2]
+                cp.psi_high_len[q] = lay[3]
+        ad = prog.ade
+        ad.dt = plan.dt
+        self.ade_state = {}
+        has_ade = False
+        for ci, c in enumerate(("Ex", "Ey", "Ez")):
+            poles = plan.ade[c]["poles"]
+            if len(poles) > nat.MW_MAX_POLES:
+                raise ValueError(f"at most {nat.MW_MAX_POLES} poles per component")
+            ad.count[ci] = len(poles)
+            if poles:
+                has_ade = True
+                ad.inv_eps[ci] = self._dev(plan.ade[c]["inv_eps"])
+            states = []
+            for pi, pole in enumerate(poles):
+                pl = ad.poles[ci][pi]
+                pl.kind = _KIND[pole["kind"]]
+                pl.decay = pole["decay"]
+                pl.restoring = pole["restoring"]
+                pl.drive = self._dev(pole["drive"])
+                cur = torch.zeros(plan.field_shape(c), dtype=torch.float32, device=dev)
+                pol = torch.zeros_like(cur) if pole["kind"] != "drude" else None
+                pl.current = nat.ptr(cur)
+                pl.polarization = nat.ptr(pol) if pol is not None else None
+                states.append({"current": cur, "polarization": pol})
+            self.ade_state[c] = states
+        prog.has_ade = 1 if has_ade else 0
+        f = prog.fields
+        for a in range(3):
+            f.e[a] = nat.ptr(self.fields[COMPONENTS[a]])
+            f.h[a] = nat.ptr(self.fields[COMPONENTS[a + 3]])
+        prog.dt = plan.dt
+        prog.total_steps = plan.time_steps
+
+        # ---- source terms
+        cidx = {c: i for i, c in enumerate(COMPONENTS)}
+        e_terms, h_terms = [], []
+        for ti, t in enumerate(plan.point_terms):
+            e_terms.append(self._term_additive(cidx[t["field"]], t, plan.signals[ti]))
+        for t in plan.e_terms:
+            e_terms.append(self._term_shifted(cidx[t["field"]], t, 0.5 * plan.dt))
+        for t in plan.h_terms:
+            h_terms.append(self._term_shifted(cidx[t["field"]], t, 0.0))
+        if len(e_terms) > nat.MW_MAX_TERMS or len(h_terms) > nat.MW_MAX_TERMS:
+            raise ValueError(f"at most {nat.MW_MAX_TERMS} source terms per field family")
+        self._e_terms = (nat.MwTerm * max(1, len(e_terms)))(*e_terms)
+        self._h_terms = (nat.MwTerm * max(1, len(h_terms)))(*h_terms)
+        prog.n_e_terms = len(e_terms)
+        prog.n_h_terms = len(h_terms)
+        prog.e_terms = ctypes.cast(self._e_terms, ctypes.POINTER(nat.MwTerm))
+        prog.h_terms = ctypes.cast(self._h_terms, ctypes.POINTER(nat.MwTerm))
+
+        # ---- observers
+        obs = plan.observers
+        self.point_re = self.point_im = None
+        self.plane_re, self.plane_im = [], []
+        planes = []
+        if obs is not None:
+            nf = len(obs.frequencies)
+            prog.nf_obs = nf
+            ec, es, hc, hs = plan.obs_weights
+            prog.obs_e_cos = self._dev(ec)
+            prog.obs_e_sin = self._dev(es)
+            prog.obs_h_cos = self._dev(hc)
+            prog.obs_h_sin = self._dev(hs)
+            shapes = {c: plan.field_shape(c) for c in COMPONENTS}
+            for (comp, axis, index) in obs.planes:
+                pshape = obs.plane_shape(shapes, (comp, axis, index))
+                re = torch.zeros((nf,) + pshape, dtype=torch.float32, device=dev)
+                im = torch.zeros_like(re)
+                self.plane_re.append(re)
+                self.plane_im.append(im)
+                po = nat.MwPlaneObs()
+                po.component = cidx[comp]
+                po.axis = "xyz".index(axis)
+                po.index = index
+                po.re = nat.ptr(re)
+                po.im = nat.ptr(im)
+                planes.append(po)
+            if len(planes) > nat.MW_MAX_PLANES:
+                raise ValueError(f"at most {nat.MW_MAX_PLANES} observer planes")
+            if obs.points:
+                comps = np.array([cidx[c] for c, _ in obs.points], dtype=np.int32)
+                lin = np.array([np.ravel_multi_index(idx, plan.field_shape(c)) for c, idx in obs.points],
+                               dtype=np.int32)
+                self.point_comp_t = torch.as_tensor(comps, device=dev)
+                self.point_lin_t = torch.as_tensor(lin, device=dev)
+                self.point_re = torch.zeros((nf, len(obs.points)), dtype=torch.float32, device=dev)
+                self.point_im = torch.zeros_like(self.point_re)
+                prog.point_component = nat.ptr(self.point_comp_t)
+                prog.point_linear = nat.ptr(self.point_lin_t)
+                prog.point_re = nat.ptr(self.point_re)
+                prog.point_im = nat.ptr(self.point_im)
+                prog.n_points = len(obs.points)
+        self._planes = (nat.MwPlaneObs * max(1, len(planes)))(*planes)
+        prog.planes = ctypes.cast(self._planes, ctypes.POINTER(nat.MwPlaneObs))
+        prog.n_planes = len(planes)
+
+        # ---- full-field DFT
+        self.dft_re, self.dft_im = {}, {}
+        if plan.dft_frequencies:
+            nf = len(plan.dft_frequencies)
+            prog.nf_dft = nf
+            prog.dft_cos = self._dev(plan.dft_weights[0])
+            prog.dft_sin = self._dev(plan.dft_weights[1])
+            for a, c in enumerate(("Ex", "Ey", "Ez")):
+                re = torch.zeros((nf,) + plan.field_shape(c), dtype=torch.float32, device=dev)
+                im = torch.zeros_like(re)
+                self.dft_re[c] = re
+                self.dft_im[c] = im
+                prog.dft_re[a] = nat.ptr(re)
+                prog.dft_im[a] = nat.ptr(im)
+        self.program = prog
+        self.step_index = 0
+
+    # ------------------------------------------------------------ helpers
+    def _dev(self, arr, dtype=np.float32):
+        t = torch.as_tensor(np.ascontiguousarray(arr, dtype=dtype), device=self.device)
+        self.keep.append(t)
+        return nat.ptr(t)
+
+    def _term_common(self, comp, t):
+        term = nat.MwTerm()
+        term.component = comp
+        for a in range(3):
+            term.offset[a] = int(t["offset"][a])
+            term.shape[a] = int(t["patch"].shape[a])
+        term.patch = self._dev(t["patch"])
+        return term
+
+    def _term_additive(self, comp, t, signal):
+        term = self._term_common(comp, t)
+        term.kind = 0
+        term.signal_table = self._dev(signal)
+        return term
+
+    def _term_shifted(self, comp, t, t_offset):
+        term = self._term_common(comp, t)
+        term.kind = 1
+        term.delay = self._dev(t["delay"], dtype=np.float64)
+        st = t["st"]
+        term.wave_kind = st["kind_code"]
+        term.t_offset = t_offset
+        term.frequency = st["frequency"]
+        term.fwidth = st["fwidth"]
+        term.amplitude = st["amplitude"]
+        term.phase = st["phase"]
+        term.delay0 = st["delay"]
+        return term
+
+    # ------------------------------------------------------------ stepping
+    def run(self, count=None):
+        count = self.plan.time_steps - self.step_index if count is None else int(count)
+        if count <= 0:
+            return self
+        with torch.cuda.device(self.device):
+            rc = self.lib.mw_run_steps(ctypes.byref(self.program), self.step_index, count, nat.current_stream())
+        nat.check(rc, self.lib)
+        self.step_index += count
+        return self
+
+    def reset(self):
+        for t in list(self.fields.values()) + list(self.psi.values()):
+            t.zero_()
+        for states in self.ade_state.values():
+            for s in states:
+                s["current"].zero_()
+                if s["polarization"] is not None:
+                    s["polarization"].zero_()
+        for t in self.plane_re + self.plane_im + list(self.dft_re.values()) + list(self.dft_im.values()):
+            t.zero_()
+        if self.point_re is not None:
+            self.point_re.zero_()
+            self.point_im.zero_()
+        self.step_index = 0
+
+    # ------------------------------------------------------------ outputs
+    def monitor_results(self):
+        from .spectral import assemble_monitor_results
+        obs = self.plan.observers
+        if obs is None:
+            return {}
+        pr = self.point_re.double().cpu().numpy() if self.point_re is not None else np.zeros((len(obs.frequencies), 0))
+        pi = self.point_im.double().cpu().numpy() if self.point_im is not None else np.zeros((len(obs.frequencies), 0))
+        return assemble_monitor_results(obs, pr, pi, [t.double().cpu().numpy() for t in self.plane_re],
+                                        [t.double().cpu().numpy() for t in self.plane_im], self.plan.obs_window_sum)
+
+    def psi_dense(self, name):
+        """Expand a slab psi array to the field shape (psi is zero outside the slabs)."""
+        comp, axis = PSI_SPECS[name]
+        _, low, hs, hl = self.plan.psi_layout[name]
+        dense = torch.zeros(self.plan.field_shape(comp), dtype=torch.float32, device=self.device)
+        t = self.psi[name]
+        if low:
+            dense.narrow(axis, 0, low).copy_(t.narrow(axis, 0, low))
+        if hl:
+            dense.narrow(axis, hs, hl).copy_(t.narrow(axis, low, hl))
+        return dense

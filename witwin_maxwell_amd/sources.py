@@ -1,0 +1,198 @@
+"""Sources and source-time waveforms.
+
+``GaussianPulse``/``CW`` follow witwin/maxwell/sources.py:200-250 and the
+formulas of witwin/maxwell/waveforms.py:30-53 (sigma_t = 1/(2*pi*fwidth), default
+delay 6*sigma_t, characteristic frequency f0 + 3*fwidth).  ``PointDipole`` and
+``PlaneWave`` follow sources.py:549+ and ``TFSF`` the slab form used by
+fdtd/excitation/tfsf_*.py.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass
+
+POINT_DIPOLE_REFERENCE_WIDTH = 0.02  # metres, sources.py:21
+
+KIND_CODE = {"cw": 0, "gaussian_pulse": 1, "ricker_wavelet": 2}
+
+
+def gaussian_sigma_t(fwidth: float) -> float:
+    return 1.0 / (2.0 * math.pi * fwidth)
+
+
+def gaussian_pulse_signal(t, *, frequency, fwidth, amplitude, phase, delay):
+    """A exp(-tau^2/(2 sigma^2)) cos(2 pi f tau + phi), waveforms.py:40-53."""
+    sigma_t = gaussian_sigma_t(fwidth)
+    tau = float(t) - float(delay)
+    return amplitude * math.exp(-0.5 * (tau / sigma_t) ** 2) * math.cos(2.0 * math.pi * frequency * tau + phase)
+
+
+def cw_signal(t, *, frequency, amplitude, phase):
+    return amplitude * math.cos(2.0 * math.pi * frequency * float(t) + phase)
+
+
+@dataclass(frozen=True)
+class GaussianPulse:
+    frequency: float
+    fwidth: float
+    amplitude: float = 1.0
+    phase: float = 0.0
+    delay: float | None = None
+    kind: str = "gaussian_pulse"
+
+    def __post_init__(self):
+        f = float(self.frequency)
+        fw = float(self.fwidth)
+        if f <= 0 or fw <= 0:
+            raise ValueError("GaussianPulse frequency and fwidth must be > 0")
+        object.__setattr__(self, "frequency", f)
+        object.__setattr__(self, "fwidth", fw)
+        object.__setattr__(self, "amplitude", float(self.amplitude))
+        object.__setattr__(self, "phase", float(self.phase))
+        object.__setattr__(self, "delay", 6.0 * gaussian_sigma_t(fw) if self.delay is None else float(self.delay))
+        object.__setattr__(self, "kind", "gaussian_pulse")
+
+    @property
+    def sigma_t(self):
+        return gaussian_sigma_t(self.fwidth)
+
+    def evaluate(self, t: float) -> float:
+        return gaussian_pulse_signal(t, frequency=self.frequency, fwidth=self.fwidth, amplitude=self.amplitude,
+                                     phase=self.phase, delay=self.delay)
+
+    @property
+    def characteristic_frequency(self) -> float:
+        return max(self.frequency, self.frequency + 3.0 * self.fwidth)
+
+
+@dataclass(frozen=True)
+class CW:
+    frequency: float
+    amplitude: float = 1.0
+    phase: float = 0.0
+    kind: str = "cw"
+
+    def __post_init__(self):
+        if float(self.frequency) <= 0:
+            raise ValueError("CW frequency must be > 0")
+        object.__setattr__(self, "frequency", float(self.frequency))
+        object.__setattr__(self, "amplitude", float(self.amplitude))
+        object.__setattr__(self, "phase", float(self.phase))
+
+    @property
+    def delay(self):
+        return 0.0
+
+    @property
+    def fwidth(self):
+        return 0.0
+
+    def evaluate(self, t: float) -> float:
+        return cw_signal(t, frequency=self.frequency, amplitude=self.amplitude, phase=self.phase)
+
+    @property
+    def characteristic_frequency(self) -> float:
+        return self.frequency
+
+
+def source_time_dict(st) -> dict:
+    """Compiled source-time record, the fields the device waveform needs."""
+    return {"kind": st.kind, "kind_code": KIND_CODE[st.kind], "frequency": float(st.frequency),
+            "fwidth": float(getattr(st, "fwidth", 0.0)), "amplitude": float(st.amplitude),
+            "phase": float(getattr(st, "phase", 0.0)), "delay": float(st.delay),
+            "characteristic_frequency": float(st.characteristic_frequency)}
+
+
+def polarization_vector(polarization):
+    if isinstance(polarization, str):
+        table = {"ex": (1.0, 0.0, 0.0), "ey": (0.0, 1.0, 0.0), "ez": (0.0, 0.0, 1.0),
+                 "x": (1.0, 0.0, 0.0), "y": (0.0, 1.0, 0.0), "z": (0.0, 0.0, 1.0)}
+        key = polarization.strip().lower()
+        if key not in table:
+            raise ValueError(f"Unknown polarization {polarization!r}")
+        return table[key]
+    vec = tuple(float(v) for v in polarization)
+    if len(vec) != 3:
+        raise ValueError("polarization must have three components")
+    return vec
+
+
+@dataclass(frozen=True)
+class PointDipole:
+    position: tuple
+    polarization: tuple
+    width: float = POINT_DIPOLE_REFERENCE_WIDTH
+    profile: str = "gaussian"
+    source_time: object = None
+    name: str | None = None
+    kind: str = "point_dipole"
+
+    def __init__(self, position, polarization="Ez", width=POINT_DIPOLE_REFERENCE_WIDTH, profile="gaussian",
+                 source_time=None, name=None):
+        pos = tuple(float(v) for v in position)
+        if len(pos) != 3:
+            raise ValueError("position must have three components")
+        prof = str(profile).lower()
+        if prof not in ("gaussian", "ideal"):
+            raise ValueError("PointDipole profile must be 'gaussian' or 'ideal'")
+        if float(width) <= 0:
+            raise ValueError("width must be > 0.")
+        object.__setattr__(self, "position", pos)
+        object.__setattr__(self, "polarization", polarization_vector(polarization))
+        object.__setattr__(self, "width", float(width))
+        object.__setattr__(self, "profile", prof)
+        object.__setattr__(self, "source_time", source_time)
+        object.__setattr__(self, "name", name)
+        object.__setattr__(self, "kind", "point_dipole")
+
+
+@dataclass(frozen=True)
+class TFSF:
+    """Total-field/scattered-field slab injection: the incident wave is confined to
+    the region between ``bounds`` along ``axis`` (fdtd/excitation/tfsf_*.py)."""
+
+    axis: str = "z"
+    bounds: tuple | None = None
+
+    @classmethod
+    def slab(cls, axis="z", bounds=None):
+        return cls(axis=str(axis).lower(), bounds=None if bounds is None else (float(bounds[0]), float(bounds[1])))
+
+
+def _normalize(v):
+    m = math.sqrt(sum(c * c for c in v))
+    if m <= 1e-12:
+        raise ValueError("vector must be non-zero")
+    return tuple(c / m for c in v)
+
+
+@dataclass(frozen=True)
+class PlaneWave:
+    direction: tuple
+    polarization: tuple
+    source_time: object = None
+    injection: object = "soft"
+    injection_axis: str | None = None
+    name: str | None = None
+    kind: str = "plane_wave"
+
+    def __init__(self, direction=(0.0, 0.0, 1.0), polarization=(1.0, 0.0, 0.0), source_time=None,
+                 injection="soft", injection_axis=None, name=None):
+        d = _normalize(tuple(float(v) for v in direction))
+        p = _normalize(tuple(float(v) for v in polarization))
+        if abs(sum(a * b for a, b in zip(d, p))) > 1e-6:
+            raise ValueError("PlaneWave polarization must be transverse to direction")
+        if not isinstance(injection, TFSF) and str(injection).lower() != "soft":
+            raise ValueError("PlaneWave injection must be 'soft' or TFSF(...).")
+        object.__setattr__(self, "direction", d)
+        object.__setattr__(self, "polarization", p)
+        object.__setattr__(self, "source_time", source_time)
+        object.__setattr__(self, "injection", injection if isinstance(injection, TFSF) else "soft")
+        object.__setattr__(self, "injection_axis", None if injection_axis is None else str(injection_axis).lower())
+        object.__setattr__(self, "name", name)
+        object.__setattr__(self, "kind", "plane_wave")
+
+
+__all__ = ["GaussianPulse", "CW", "PointDipole", "PlaneWave", "TFSF", "POINT_DIPOLE_REFERENCE_WIDTH",
+           "gaussian_pulse_signal", "cw_signal", "source_time_dict", "polarization_vector"]
