@@ -58,6 +58,29 @@ def algorithmic_bytes(plan, runner):
     return step, e_launch, h_launch
 
 
+def time_fused(runner, reps):
+    """Average duration of one single-pass step launch: the program is copied with
+    its monitors detached, so each step is exactly one kernel on torch's current
+    stream, bracketed by HIP events."""
+    import torch
+    from witwin_maxwell_amd.fdtd import native as nat
+    prog = nat.MwProgram()
+    ctypes.memmove(ctypes.byref(prog), ctypes.byref(runner.program), ctypes.sizeof(prog))
+    prog.n_planes = 0
+    prog.n_points = 0
+    first = runner.step_index
+    reps = min(reps, runner.plan.time_steps - first - 2)
+    stream = nat.current_stream()
+    nat.check(runner.lib.mw_run_steps(ctypes.byref(prog), first, 2, stream), runner.lib)
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    nat.check(runner.lib.mw_run_steps(ctypes.byref(prog), first + 2, reps, stream), runner.lib)
+    end.record()
+    end.synchronize()
+    return start.elapsed_time(end) * 1e-3 / reps
+
+
 def time_kernel(runner, which, reps):
     """Average duration of one E or H launch, measured with HIP events on the
     stream the kernel is launched on (torch's current stream)."""
@@ -118,6 +141,9 @@ def main():
     ap.add_argument("--cells", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--two-kernel", action="store_true", help="use the two-launch (H, E) step instead of the single pass")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="also run torch copies of 256 MiB (counter calibration for rocprofv3 --pmc runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
@@ -137,7 +163,7 @@ def main():
     t0 = time.perf_counter()
     plan = compile_plan(scene, **kw)
     compile_s = time.perf_counter() - t0
-    runner = DeviceRun(plan, "cuda:0")
+    runner = DeviceRun(plan, "cuda:0", fused=not args.two_kernel)
     nodes = int(np.prod(plan.shape))
     print(f"[bench] plan {plan.shape} compiled in {compile_s:.1f}s; warmup {args.warmup} steps", file=sys.stderr,
           flush=True)
@@ -156,9 +182,12 @@ def main():
 
     step_bytes, e_bytes, h_bytes = algorithmic_bytes(plan, runner)
     reps = max(20, min(200, args.steps // 5))
-    t_e = time_kernel(runner, "E", reps)
-    t_h = time_kernel(runner, "H", reps)
-    dominant, t_dom, b_dom = ("update_e_kernel", t_e, e_bytes) if t_e >= t_h else ("update_h_kernel", t_h, h_bytes)
+    if runner.fused:
+        dominant, t_dom, b_dom = "fused_step_kernel", time_fused(runner, reps), step_bytes
+    else:
+        t_e = time_kernel(runner, "E", reps)
+        t_h = time_kernel(runner, "H", reps)
+        dominant, t_dom, b_dom = ("update_e_kernel", t_e, e_bytes) if t_e >= t_h else ("update_h_kernel", t_h, h_bytes)
     achieved = b_dom / t_dom
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_dielectric_boxes_256.json")
@@ -172,6 +201,13 @@ def main():
                 "kernel_avg_us": t_dom * 1e6, "alg_bytes_per_launch": b_dom,
                 "step_alg_bytes_per_node": step_bytes / nodes,
                 "step_frac": step_bytes * args.steps / elapsed / HBM_PEAK}
+    if args.copy_probe:
+        src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0").fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)

@@ -167,6 +167,9 @@ typedef struct mw_term {
   int32_t wave_kind;          /* SHIFTED: 0 CW, 1 Gaussian, 2 Ricker */
   int32_t pad;
   double t_offset, frequency, fwidth, amplitude, phase, delay0;
+  float* values;              /* SHIFTED, single-pass mode: device buffer of
+                                 patch-size floats the step fills with
+                                 patch * s(t - delay) before the fused kernel */
 } mw_term;
 
 typedef struct mw_plane_obs {
@@ -206,9 +209,20 @@ typedef struct mw_program {
   const float* dft_sin;
   float* dft_re[3]; /* (nf, component numel) */
   float* dft_im[3];
+  /* Single-pass mode (fused = 1): E, H and the H-side psi are double-buffered.
+   * At step n the state lives in `fields` / cpml.psi[6..11] when n is even and in
+   * `fields_alt` / psi_h_alt when n is odd.  Not available with periodic faces.
+   * h_terms_dev / e_terms_dev are device copies of h_terms / e_terms. */
+  int32_t fused, pad4;
+  mw_fields fields_alt;
+  float* psi_h_alt[6];
+  const mw_term* h_terms_dev;
+  const mw_term* e_terms_dev;
 } mw_program;
 
-/* Advance `count` steps starting at step index `first` (t = n*dt). */
+/* Advance `count` steps starting at step index `first` (t = n*dt).  Replaces the
+ * reference's per-step Python loop with its graph-captured field/tail blocks
+ * (fdtd/runtime/stepping.py:2289-2357, :1818-2024). */
 int mw_run_steps(const mw_program* program, int32_t first, int32_t count, void* stream);
 
 const char* mw_last_error(void);

@@ -219,7 +219,7 @@ class OracleRun:
             self.point_im[:, pi] += v * ws
         for qi, (comp, axis, index) in enumerate(self.obs.planes):
             f = self.fields[comp]
-            sl = {"x": f[index, :, :], "y": f[:, index, :], "z": f[:, :, index]}[axis].astype(np.float64)
+            sl = np.take(f, index, axis="xyz".index(axis)).astype(np.float64)
             wc, ws = (ec, es) if comp[0] == "E" else (hc, hs)
             self.plane_re[qi] += wc[:, None, None] * sl[None]
             self.plane_im[qi] += ws[:, None, None] * sl[None]

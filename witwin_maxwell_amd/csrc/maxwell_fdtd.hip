@@ -4,9 +4,11 @@
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
+#include <cstring>
 #include <string>
 
 #include "maxwell_fdtd.h"
+#include "fused.h"
 #include "terms.h"
 #include "yee.h"
 
@@ -193,6 +195,166 @@ int launch_terms(const mw_grid* g, const mw_fields* f, const mw_term* t, int n, 
   return check_launch("apply_terms");
 }
 
+int coef_mode(const mw_coeffs& c) {
+  // coefficient mode of the lean interior path (see fused.h)
+  bool decay_one = true, h_curl_scalar = true, e_curl_scalar = true, e_curl_array = true;
+  for (int q = 0; q < 6; ++q)
+    if (c.decay[q] || c.decay_value[q] != 1.0f) decay_one = false;
+  for (int q = 3; q < 6; ++q)
+    if (c.curl[q]) h_curl_scalar = false;
+  for (int q = 0; q < 3; ++q) {
+    if (c.curl[q]) e_curl_scalar = false;
+    else e_curl_array = false;
+  }
+  if (decay_one && h_curl_scalar && e_curl_scalar) return 2;
+  if (decay_one && h_curl_scalar && e_curl_array) return 1;
+  return 0;
+}
+
+template <bool CPML, bool ADE, bool DFT>
+void dispatch_general(dim3 grid, dim3 block, hipStream_t s, const mw::FusedArgs& a, int /*mode*/) {
+  hipLaunchKernelGGL((mw::fused_step_kernel<CPML, ADE, DFT>), grid, block, 0, s, a);
+}
+
+template <bool ADE, bool DFT>
+void dispatch_lean(dim3 grid, dim3 block, hipStream_t s, const mw::FusedArgs& a, int mode) {
+  if (mode == 2) hipLaunchKernelGGL((mw::lean_step_kernel<ADE, DFT, 2>), grid, block, 0, s, a);
+  else if (mode == 1) hipLaunchKernelGGL((mw::lean_step_kernel<ADE, DFT, 1>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((mw::lean_step_kernel<ADE, DFT, 0>), grid, block, 0, s, a);
+}
+
+// Tile origin and lean-box extent along one axis.  Tile t covers `width`
+// samples starting at off + width*t; the lean box holds the tiles whose samples,
+// plus `lead` samples before them (the halo), all lie in [lo, hi).
+struct AxisTiling {
+  int off, count, lean0, lean1;
+};
+AxisTiling tile_axis(int size, int width, int lead, int lo, int hi) {
+  AxisTiling t;
+  const int first = lo + lead;  // first owned sample of the first lean tile
+  const int r = first % width;
+  t.off = r ? r - width : 0;
+  t.count = (size - t.off + width - 1) / width;
+  t.lean0 = (first - t.off) / width;
+  int n = 0;
+  while (first + width * (n + 1) <= hi) ++n;
+  t.lean1 = std::max(t.lean0, std::min(t.lean0 + n, t.count));
+  return t;
+}
+
+int launch_shifted_values(const mw_term* t, int n, int step, double dt, hipStream_t s) {
+  mw::TermSet set{};
+  int64_t cells = 0;
+  for (int i = 0; i < n; ++i) {
+    if (t[i].kind != MW_TERM_SHIFTED) continue;
+    set.t[set.count++] = t[i];
+    cells = std::max<int64_t>(cells, (int64_t)t[i].shape[0] * t[i].shape[1] * t[i].shape[2]);
+  }
+  if (set.count == 0) return MW_OK;
+  hipLaunchKernelGGL(mw::shifted_values_kernel, dim3(grid_blocks(cells), set.count), dim3(256), 0, s, set, step, dt);
+  return check_launch("shifted_values");
+}
+
+int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_fields) {
+  int rc;
+  if ((rc = launch_shifted_values(p->h_terms, p->n_h_terms, n, p->dt, s))) return rc;
+  if ((rc = launch_shifted_values(p->e_terms, p->n_e_terms, n, p->dt, s))) return rc;
+  mw::FusedArgs a;
+  memset(&a, 0, sizeof(a));
+  a.g = p->grid;
+  a.cf = p->coeffs;
+  a.cp = p->cpml;
+  const bool odd = (n & 1) != 0;
+  a.in = odd ? p->fields_alt : p->fields;
+  a.out = odd ? p->fields : p->fields_alt;
+  if (p->cpml.enabled) {
+    for (int q = 0; q < 6; ++q) {
+      a.cp.psi[6 + q] = odd ? p->psi_h_alt[q] : p->cpml.psi[6 + q];
+      a.psi_h_out[q] = odd ? p->cpml.psi[6 + q] : p->psi_h_alt[q];
+    }
+  }
+  const bool ade = p->has_ade && (p->ade.count[0] + p->ade.count[1] + p->ade.count[2]) > 0;
+  if (ade) a.ad = p->ade;
+  a.h_terms = p->h_terms_dev;
+  a.e_terms = p->e_terms_dev;
+  a.n_h_terms = p->n_h_terms;
+  a.n_e_terms = p->n_e_terms;
+  a.nf_dft = p->nf_dft;
+  a.step = n;
+  if (p->nf_dft > 0) {
+    a.dft_cos = p->dft_cos + (int64_t)n * p->nf_dft;
+    a.dft_sin = p->dft_sin + (int64_t)n * p->nf_dft;
+    for (int c = 0; c < 3; ++c) { a.dft_re[c] = p->dft_re[c]; a.dft_im[c] = p->dft_im[c]; }
+  }
+  a.dt = p->dt;
+  const mw_grid* g = &p->grid;
+  const mw_cpml& cp = p->cpml;
+  // lean ranges [lo, hi) per axis: strictly interior and outside every psi slab
+  auto low = [&](int q) { return cp.enabled ? cp.psi_low[q] : 0; };
+  auto high = [&](int q, int size) { return (cp.enabled && cp.psi_high_len[q] > 0) ? cp.psi_high_start[q] : size; };
+  const int ilo = std::max({1, low(8), low(2)}), ihi = std::min({g->nx - 1, high(8, g->nx - 1), high(2, g->nx)});
+  const int jlo = std::max({2, low(6) + 1, low(0)}), jhi = std::min({g->ny - 1, high(6, g->ny - 1), high(0, g->ny)});
+  const int klo = std::max({1, low(7), low(1)}), khi = std::min({g->nz - 1, high(7, g->nz - 1), high(1, g->nz)});
+  // z: owned columns start one after the halo lane; rows: row j-1 is recomputed
+  // (jlo already accounts for it); planes: one halo plane before each chunk
+  const AxisTiling tz = tile_axis(g->nz, mw::kFOwnCols, 1, klo, khi);
+  const AxisTiling ty = tile_axis(g->ny, mw::kFRows, 0, jlo, jhi);
+  const int cols = tz.count * ty.count;
+  const int want_chunks = std::max(1, (4096 + cols - 1) / cols);
+  const int target = std::max(8, std::min(32, (g->nx + want_chunks - 1) / want_chunks));
+  // the chunk near the target that leaves the fewest interior planes to the
+  // general kernel (they are the ones the lean box cannot cover)
+  int chunk = target;
+  {
+    const int span = ihi - (ilo + 1);
+    int best = span >= 0 ? span % target : 0;
+    for (int c = std::max(8, target - 8); c <= std::min(32, target + 8) && span >= 8; ++c) {
+      const int left = span % c;
+      if (left < best || (left == best && std::abs(c - target) < std::abs(chunk - target))) { best = left; chunk = c; }
+    }
+  }
+  const AxisTiling tx = tile_axis(g->nx, chunk, 1, ilo, ihi);
+  a.z_off = tz.off;
+  a.y_off = ty.off;
+  a.x_off = tx.off;
+  a.tiles_z = tz.count;
+  a.tiles_y = ty.count;
+  a.tiles_x = tx.count;
+  a.xchunk = chunk;
+  a.lz0 = tz.lean0; a.lz1 = tz.lean1;
+  a.ly0 = ty.lean0; a.ly1 = ty.lean1;
+  a.lx0 = tx.lean0; a.lx1 = tx.lean1;
+  int nlean = (a.lz1 - a.lz0) * (a.ly1 - a.ly0) * (a.lx1 - a.lx0);
+  if (nlean <= 0) { nlean = 0; a.lz0 = a.lz1 = a.ly0 = a.ly1 = a.lx0 = a.lx1 = 0; }
+  const int nblocks = a.tiles_z * a.tiles_y * a.tiles_x;
+  dim3 block(mw::kFLanes, mw::kFRows, 1);
+  const bool cpml = p->cpml.enabled != 0, dft = p->nf_dft > 0;
+  const int mode = coef_mode(p->coeffs);
+  dim3 grid(((nblocks + 7) / 8) * 8);
+  if (cpml) {
+    if (ade) { if (dft) dispatch_general<true, true, true>(grid, block, s, a, mode); else dispatch_general<true, true, false>(grid, block, s, a, mode); }
+    else { if (dft) dispatch_general<true, false, true>(grid, block, s, a, mode); else dispatch_general<true, false, false>(grid, block, s, a, mode); }
+  } else {
+    if (ade) { if (dft) dispatch_general<false, true, true>(grid, block, s, a, mode); else dispatch_general<false, true, false>(grid, block, s, a, mode); }
+    else { if (dft) dispatch_general<false, false, true>(grid, block, s, a, mode); else dispatch_general<false, false, false>(grid, block, s, a, mode); }
+  }
+  if ((rc = check_launch("fused_step (general)"))) return rc;
+  if (nlean > 0) {
+    dim3 lgrid(((nlean + 7) / 8) * 8);
+    if (ade) { if (dft) dispatch_lean<true, true>(lgrid, block, s, a, mode); else dispatch_lean<true, false>(lgrid, block, s, a, mode); }
+    else { if (dft) dispatch_lean<false, true>(lgrid, block, s, a, mode); else dispatch_lean<false, false>(lgrid, block, s, a, mode); }
+    if ((rc = check_launch("fused_step (lean)"))) return rc;
+  }
+  *out_fields = a.out;
+  return MW_OK;
+}
+
+bool has_periodic(const mw_grid* g) {
+  for (int a = 0; a < 3; ++a)
+    if (g->bc_lo[a] == MW_BC_PERIODIC || g->bc_hi[a] == MW_BC_PERIODIC) return true;
+  return false;
+}
+
 }  // namespace
 
 extern "C" {
@@ -323,15 +485,36 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
     for (int c = 0; c < 3; ++c)
       if (!p->dft_re[c] || !p->dft_im[c]) return fail(MW_ERR_ARGUMENT, "DFT accumulator %d missing", c);
   }
+  if (p->fused) {
+    if (has_periodic(g)) return fail(MW_ERR_ARGUMENT, "the single-pass step does not support periodic faces");
+    for (int c = 0; c < 3; ++c)
+      if (!p->fields_alt.e[c] || !p->fields_alt.h[c]) return fail(MW_ERR_ARGUMENT, "fields_alt %d is NULL", c);
+    if (p->cpml.enabled)
+      for (int q = 0; q < 6; ++q)
+        if ((p->cpml.psi_low[6 + q] + p->cpml.psi_high_len[6 + q]) > 0 && !p->psi_h_alt[q])
+          return fail(MW_ERR_ARGUMENT, "psi_h_alt %d is NULL", q);
+    if ((p->n_h_terms > 0 && !p->h_terms_dev) || (p->n_e_terms > 0 && !p->e_terms_dev))
+      return fail(MW_ERR_ARGUMENT, "device term descriptors missing");
+    for (int f = 0; f < 2; ++f) {
+      const mw_term* t = f ? p->e_terms : p->h_terms;
+      for (int i = 0; i < (f ? p->n_e_terms : p->n_h_terms); ++i)
+        if (t[i].kind == MW_TERM_SHIFTED && !t[i].values) return fail(MW_ERR_ARGUMENT, "term %d has no value buffer", i);
+    }
+  }
   hipStream_t s = (hipStream_t)stream;
-  const mw::FieldSet fs = field_set(g, &p->fields);
   const mw_ade* ade = p->has_ade ? &p->ade : nullptr;
   for (int n = first; n < first + count; ++n) {
-    if ((rc = launch_h(g, &p->coeffs, &p->cpml, &p->fields, 7, s))) return rc;
-    if ((rc = launch_terms(g, &p->fields, p->h_terms, p->n_h_terms, n, p->dt, s))) return rc;
-    if ((rc = launch_e(g, &p->coeffs, &p->cpml, ade, &p->fields, 7, s))) return rc;
-    if ((rc = launch_terms(g, &p->fields, p->e_terms, p->n_e_terms, n, p->dt, s))) return rc;
-    if (p->nf_dft > 0) {
+    mw_fields cur = p->fields;
+    if (p->fused) {
+      if ((rc = launch_fused(p, n, s, &cur))) return rc;
+    } else {
+      if ((rc = launch_h(g, &p->coeffs, &p->cpml, &p->fields, 7, s))) return rc;
+      if ((rc = launch_terms(g, &p->fields, p->h_terms, p->n_h_terms, n, p->dt, s))) return rc;
+      if ((rc = launch_e(g, &p->coeffs, &p->cpml, ade, &p->fields, 7, s))) return rc;
+      if ((rc = launch_terms(g, &p->fields, p->e_terms, p->n_e_terms, n, p->dt, s))) return rc;
+    }
+    const mw::FieldSet fs = field_set(g, &cur);
+    if (p->nf_dft > 0 && !p->fused) {
       for (int c = 0; c < 3; ++c) {
         int shp[3];
         component_shape(g, c, shp);

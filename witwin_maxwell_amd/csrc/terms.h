@@ -60,6 +60,22 @@ __global__ __launch_bounds__(256) void apply_terms_kernel(TermSet set, FieldSet 
   fs.ptr[c][o] += value;
 }
 
+// Per-step waveform * patch of every time-shifted term of a set, written to the
+// term's value buffer (the fused step adds these instead of evaluating the
+// double-precision waveform per sample).
+__global__ __launch_bounds__(256) void shifted_values_kernel(TermSet set, int step, double dt) {
+  const int ti = blockIdx.y;
+  if (ti >= set.count) return;
+  const mw_term& t = set.t[ti];
+  if (t.kind != MW_TERM_SHIFTED) return;
+  const int64_t cells = (int64_t)t.shape[0] * t.shape[1] * t.shape[2];
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= cells) return;
+  const double time = (double)step * dt + t.t_offset - t.delay[p];
+  t.values[p] = (float)(waveform(t.wave_kind, time, t.frequency, t.fwidth, t.amplitude, t.phase, t.delay0) *
+                        (double)t.patch[p]);
+}
+
 __global__ __launch_bounds__(256) void add_patch_kernel(float* field, int fy, int fz, const float* patch, int pnx,
                                                          int pny, int pnz, int oi, int oj, int ok, float signal) {
   const int64_t cells = (int64_t)pnx * pny * pnz;

@@ -97,8 +97,16 @@ def cpml_profile_1d(positions, nodes, thickness, dt, vacuum, sigma_scale, impeda
 
 
 def build_cpml_profiles(prepared, dt, config=None):
-    """Per-axis E (node) and H (half-node) profiles (cpml.py:380-426)."""
+    """Per-axis E (node) and H (half-node) profiles (cpml.py:380-426).
+
+    The reference grades the high-side H samples at their cell's outer node
+    (``nodes[1:]``), half a cell deeper than where they sit; at normal incidence
+    that high face reflects ~6% at 12 layers / 30 cells per wavelength while the
+    low face reflects ~1e-5.  It is reproduced by default for parity;
+    ``config={"h_grading": "staggered"}`` grades both faces at the true H
+    positions instead."""
     cfg = dict(DEFAULT_CPML_CONFIG)
+    staggered = bool(config and config.get("h_grading") == "staggered")
     if config:
         cfg.update({k: v for k, v in config.items() if k in DEFAULT_CPML_CONFIG})
     scale = legacy_strength_scale(prepared.pml_strength)
@@ -113,7 +121,7 @@ def build_cpml_profiles(prepared, dt, config=None):
         ke, be, ce = cpml_profile_1d(nodes, nodes, thickness, dt, EPSILON_0, scale, eta0, cfg,
                                      apply_low=lo, apply_high=hi)
         kh, bh, ch = cpml_profile_1d(half, nodes, thickness, dt, MU_0, scale * (MU_0 / EPSILON_0), eta0, cfg,
-                                     high_positions=nodes[1:], apply_low=lo, apply_high=hi)
+                                     high_positions=None if staggered else nodes[1:], apply_low=lo, apply_high=hi)
         out[axis] = {"kappa_e": ke, "b_e": be, "c_e": ce, "kappa_h": kh, "b_h": bh, "c_h": ch}
     return out
 

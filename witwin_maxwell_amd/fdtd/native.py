@@ -64,7 +64,8 @@ class MwTerm(ctypes.Structure):
                 ("shape", ctypes.c_int32 * 3), ("patch", vp), ("delay", vp), ("signal_table", vp),
                 ("wave_kind", ctypes.c_int32), ("pad", ctypes.c_int32),
                 ("t_offset", ctypes.c_double), ("frequency", ctypes.c_double), ("fwidth", ctypes.c_double),
-                ("amplitude", ctypes.c_double), ("phase", ctypes.c_double), ("delay0", ctypes.c_double)]
+                ("amplitude", ctypes.c_double), ("phase", ctypes.c_double), ("delay0", ctypes.c_double),
+                ("values", vp)]
 
 
 class MwPlaneObs(ctypes.Structure):
@@ -81,7 +82,9 @@ class MwProgram(ctypes.Structure):
                 ("pad2", ctypes.c_int32), ("obs_e_cos", vp), ("obs_e_sin", vp), ("obs_h_cos", vp),
                 ("obs_h_sin", vp), ("planes", ctypes.POINTER(MwPlaneObs)), ("point_component", vp),
                 ("point_linear", vp), ("point_re", vp), ("point_im", vp), ("nf_dft", ctypes.c_int32),
-                ("pad3", ctypes.c_int32), ("dft_cos", vp), ("dft_sin", vp), ("dft_re", vp * 3), ("dft_im", vp * 3)]
+                ("pad3", ctypes.c_int32), ("dft_cos", vp), ("dft_sin", vp), ("dft_re", vp * 3), ("dft_im", vp * 3),
+                ("fused", ctypes.c_int32), ("pad4", ctypes.c_int32), ("fields_alt", MwFields), ("psi_h_alt", vp * 6),
+                ("h_terms_dev", vp), ("e_terms_dev", vp)]
 
 
 STRUCTS = (MwGrid, MwCoeffs, MwCpml, MwFields, MwPole, MwAde, MwTerm, MwPlaneObs, MwProgram)

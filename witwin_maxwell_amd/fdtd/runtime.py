@@ -33,7 +33,13 @@ def _uniform_value(arr):
 
 
 class DeviceRun:
-    def __init__(self, plan: FDTDPlan, device="cuda"):
+    """One scene resident in HBM.
+
+    ``fused=None`` picks the single-pass kernel (one launch per step, E/H and the
+    H-side psi double-buffered) unless a face is periodic, where the two-kernel
+    in-place path runs instead."""
+
+    def __init__(self, plan: FDTDPlan, device="cuda", fused=None):
         self.plan = plan
         self.device = torch.device(device)
         if self.device.type != "cuda":
@@ -42,7 +48,15 @@ class DeviceRun:
         dev = self.device
         self.keep = []
         nx, ny, nz = plan.shape
-        self.fields = {c: torch.zeros(plan.field_shape(c), dtype=torch.float32, device=dev) for c in COMPONENTS}
+        periodic = 2 in plan.bc_lo or 2 in plan.bc_hi
+        if fused is None:
+            fused = not periodic
+        if fused and periodic:
+            raise ValueError("the single-pass step does not support periodic faces")
+        self.fused = bool(fused)
+        self._fields = [{c: torch.zeros(plan.field_shape(c), dtype=torch.float32, device=dev) for c in COMPONENTS}]
+        if self.fused:
+            self._fields.append({c: torch.zeros_like(t) for c, t in self._fields[0].items()})
 
         prog = nat.MwProgram()
         g = prog.grid
@@ -78,6 +92,7 @@ class DeviceRun:
                 cp.inv_kappa_h[a] = self._dev(1.0 / p["kappa_h"])
                 cp.b_h[a] = self._dev(p["b_h"])
                 cp.c_h[a] = self._dev(p["c_h"])
+            self._psi_alt = {}
             for q, name in enumerate(PSI_NAMES):
                 comp, _ = PSI_SPECS[name]
                 lay = plan.psi_layout[name]
@@ -88,6 +103,10 @@ class DeviceRun:
                 cp.psi_low[q] = lay[1]
                 cp.psi_high_start[q] = lay[2]
                 cp.psi_high_len[q] = lay[3]
+                if self.fused and q >= 6:
+                    alt = torch.zeros_like(t)
+                    self._psi_alt[name] = alt
+                    prog.psi_h_alt[q - 6] = nat.ptr(alt) if alt.numel() else None
         ad = prog.ade
         ad.dt = plan.dt
         self.ade_state = {}
@@ -114,10 +133,11 @@ class DeviceRun:
                 states.append({"current": cur, "polarization": pol})
             self.ade_state[c] = states
         prog.has_ade = 1 if has_ade else 0
-        f = prog.fields
-        for a in range(3):
-            f.e[a] = nat.ptr(self.fields[COMPONENTS[a]])
-            f.h[a] = nat.ptr(self.fields[COMPONENTS[a + 3]])
+        for f, fd in ((prog.fields, self._fields[0]), (prog.fields_alt, self._fields[-1])):
+            for a in range(3):
+                f.e[a] = nat.ptr(fd[COMPONENTS[a]])
+                f.h[a] = nat.ptr(fd[COMPONENTS[a + 3]])
+        prog.fused = 1 if self.fused else 0
         prog.dt = plan.dt
         prog.total_steps = plan.time_steps
 
@@ -138,6 +158,12 @@ class DeviceRun:
         prog.n_h_terms = len(h_terms)
         prog.e_terms = ctypes.cast(self._e_terms, ctypes.POINTER(nat.MwTerm))
         prog.h_terms = ctypes.cast(self._h_terms, ctypes.POINTER(nat.MwTerm))
+        # device copies of the descriptors for the single-pass kernel
+        for arr, attr in ((self._e_terms, "e_terms_dev"), (self._h_terms, "h_terms_dev")):
+            raw = torch.frombuffer(bytearray(ctypes.string_at(ctypes.addressof(arr), ctypes.sizeof(arr))),
+                                   dtype=torch.uint8).to(dev)
+            self.keep.append(raw)
+            setattr(prog, attr, nat.ptr(raw))
 
         # ---- observers
         obs = plan.observers
@@ -202,6 +228,16 @@ class DeviceRun:
         self.program = prog
         self.step_index = 0
 
+    @property
+    def fields(self):
+        """Current field tensors (the double buffer holding step ``step_index``)."""
+        return self._fields[self.step_index & 1] if self.fused else self._fields[0]
+
+    def psi_current(self, name):
+        if self.fused and name.startswith("psi_h") and (self.step_index & 1):
+            return self._psi_alt[name]
+        return self.psi[name]
+
     # ------------------------------------------------------------ helpers
     def _dev(self, arr, dtype=np.float32):
         t = torch.as_tensor(np.ascontiguousarray(arr, dtype=dtype), device=self.device)
@@ -227,6 +263,9 @@ class DeviceRun:
         term = self._term_common(comp, t)
         term.kind = 1
         term.delay = self._dev(t["delay"], dtype=np.float64)
+        vals = torch.zeros(t["patch"].shape, dtype=torch.float32, device=self.device)
+        self.keep.append(vals)
+        term.values = nat.ptr(vals)
         st = t["st"]
         term.wave_kind = st["kind_code"]
         term.t_offset = t_offset
@@ -249,7 +288,10 @@ class DeviceRun:
         return self
 
     def reset(self):
-        for t in list(self.fields.values()) + list(self.psi.values()):
+        for fd in self._fields:
+            for t in fd.values():
+                t.zero_()
+        for t in list(self.psi.values()) + list(getattr(self, "_psi_alt", {}).values()):
             t.zero_()
         for states in self.ade_state.values():
             for s in states:
@@ -279,7 +321,7 @@ class DeviceRun:
         comp, axis = PSI_SPECS[name]
         _, low, hs, hl = self.plan.psi_layout[name]
         dense = torch.zeros(self.plan.field_shape(comp), dtype=torch.float32, device=self.device)
-        t = self.psi[name]
+        t = self.psi_current(name)
         if low:
             dense.narrow(axis, 0, low).copy_(t.narrow(axis, 0, low))
         if hl:
