@@ -225,6 +225,58 @@ typedef struct mw_program {
  * (fdtd/runtime/stepping.py:2289-2357, :1818-2024). */
 int mw_run_steps(const mw_program* program, int32_t first, int32_t count, void* stream);
 
+/* ---- adjoint (reverse-mode) ----
+ * Design window: the node box [lo, hi) over which dL/d(decay_E) and dL/d(curl_E)
+ * are accumulated, each as three z-fastest (hi-lo) boxes, one per E component.
+ * The box must satisfy 1 <= lo, hi <= N-1 and lie outside every E-side CPML
+ * slab, so every window sample is an updated, psi-free sample.  A capture
+ * stores E^n on [lo, hi) and H^{n+1/2} on [lo-1, hi) per replayed step
+ * (replaces the reference's segment replay that keeps whole states,
+ * fdtd/adjoint/bridge.py:728-853, fdtd/checkpoint.py:365-440). */
+typedef struct mw_window {
+  int32_t lo[3], hi[3];
+  int32_t capacity; /* steps the stores hold */
+  int32_t pad;
+  float* e_store;   /* (capacity + 1) x 3 x W,  W  = prod(hi - lo)     */
+  float* h_store;   /* capacity x 3 x Wh,       Wh = prod(hi - lo + 1) */
+  float* g_decay[3]; /* W each, accumulated */
+  float* g_curl[3];
+} mw_window;
+
+/* Adjoint state (same shapes as the forward state; psi in the program's slab
+ * layout) and the gradients of the observer sums that seed it (NULL = zero;
+ * layouts of point_re/im, planes[q].re/im and dft_re/im). */
+typedef struct mw_adjoint {
+  mw_fields fields;
+  float* psi[12];
+  float* polarization[3][MW_MAX_POLES];
+  float* current[3][MW_MAX_POLES];
+  const float* g_point_re;
+  const float* g_point_im;
+  const float* g_plane_re[MW_MAX_PLANES];
+  const float* g_plane_im[MW_MAX_PLANES];
+  const float* g_dft_re[3];
+  const float* g_dft_im[3];
+} mw_adjoint;
+
+/* Replay steps [first, first+count) of a program (observers are not
+ * accumulated) and capture the design window: E^first..E^{first+count} into
+ * e_store[0..count], H^{n+1/2} into h_store[n-first].  The program's state must
+ * hold step `first` on entry.  Replaces _replay_segment_states / _step_state
+ * (fdtd/adjoint/core.py:3495-3565). */
+int mw_capture_steps(const mw_program* program, int32_t first, int32_t count, const mw_window* window,
+                     void* stream);
+
+/* Reverse steps first+count-1 down to first: seeds of each step's observers,
+ * then the transposed H/E/CPML/ADE updates; with a window (may be NULL) also the
+ * coefficient gradients, reading the capture of [first, first+count).  The
+ * adjoint must hold the adjoint of the state after step first+count-1 on entry.
+ * Replaces reverse_phase{1,2,3} / reverse_cpml_phase_* / reverse_*_current and
+ * seed_inject_* (fdtd/adjoint/native.py:48-727, kernels/adjoint.cu:149-420,
+ * :898-979, :1224-1452, :1737-1870). */
+int mw_reverse_steps(const mw_program* program, const mw_adjoint* adjoint, const mw_window* window, int32_t first,
+                     int32_t count, void* stream);
+
 const char* mw_last_error(void);
 int32_t mw_abi_version(void);
 /* sizeof of mw_grid, mw_coeffs, mw_cpml, mw_fields, mw_pole, mw_ade, mw_term,

@@ -61,6 +61,7 @@ def load(precision="f64"):
     lib.or_add_shifted.argtypes = [P, i32, i32, P, ctypes.POINTER(dbl), i32, i32, i32, i32, i32, i32, dbl, i32,
                                    dbl, dbl, dbl, dbl, dbl]
     lib.or_accumulate.argtypes = [i64, P, i32, ctypes.POINTER(dbl), ctypes.POINTER(dbl), P, P]
+    lib.or_reverse_step.argtypes = [ctypes.c_void_p, PP, PP, PP, PP, ctypes.c_void_p, PP, PP, PP, PP, PP]
     lib._real = real
     lib._dtype = np.float64 if precision == "f64" else np.float32
     _LIBS[precision] = lib
@@ -78,6 +79,14 @@ def _grid_struct(real):
               ("pad", ctypes.c_int32), ("inv_e", P * 3), ("inv_h", P * 3), ("decay", P * 6), ("curl", P * 6),
               ("kinv_e", P * 3), ("b_e", P * 3), ("c_e", P * 3), ("kinv_h", P * 3), ("b_h", P * 3), ("c_h", P * 3)]
     return type("OrGrid", (ctypes.Structure,), {"_fields_": fields})
+
+
+def _ade_adj_struct(real):
+    P = ctypes.POINTER(real)
+    fields = [("count", ctypes.c_int32 * 3), ("kind", (ctypes.c_int32 * 4) * 3), ("decay", (ctypes.c_double * 4) * 3),
+              ("restoring", (ctypes.c_double * 4) * 3), ("drive", (P * 4) * 3), ("apol", (P * 4) * 3),
+              ("acur", (P * 4) * 3), ("inv_eps", P * 3), ("dt", ctypes.c_double)]
+    return type("OrAdeAdj", (ctypes.Structure,), {"_fields_": fields})
 
 
 def _ptr(arr, real):
@@ -162,6 +171,7 @@ class OracleRun:
         # against which late-time (decayed) fields are judged
         self.peak = {c: 0.0 for c in _COMP}
         self.track_peak = precision == "f64"
+        self.trajectory = None  # list of per-step forward states when recording for the adjoint
 
     # ------------------------------------------------------------------ steps
     def _shifted(self, terms, time):
@@ -178,6 +188,7 @@ class OracleRun:
         plan, lib, real = self.plan, self.lib, self.lib._real
         n = self.step_index
         t = n * plan.dt
+        e_before = {c: self.fields[c].copy() for c in _COMP[:3]} if self.trajectory is not None else None
         lib.or_update_h(ctypes.byref(self.grid), self._eptr, self._hptr, self._psiptr)
         self._shifted(self.h_terms, t)
         for c in ("Ex", "Ey", "Ez"):
@@ -208,6 +219,11 @@ class OracleRun:
         if self.track_peak:
             for c in _COMP:
                 self.peak[c] = max(self.peak[c], float(np.max(np.abs(self.fields[c]))))
+        if self.trajectory is not None:
+            # E^n, H^{n+1/2} and the E-side psi^{n+1}: what the coefficient
+            # gradients of step n need on the way back
+            self.trajectory.append((e_before, {c: self.fields[c].copy() for c in _COMP[3:]},
+                                    [p.copy() for p in self.psi[:6]]))
         self.step_index += 1
 
     def _observe(self, n):
@@ -238,3 +254,102 @@ class OracleRun:
 
 def run_plan(plan, steps=None, precision="f64"):
     return OracleRun(plan, precision).run(steps)
+
+
+class OracleAdjoint:
+    """Reverse sweep of a recorded ``OracleRun`` (float64): the exact transpose of
+    the forward steps (fdtd_oracle.c ``or_reverse_step``), seeded by the
+    gradients of the observer sums (the transpose of ``_observe`` and the
+    full-field DFT), accumulating dL/d(decay_E) and dL/d(curl_E).
+
+    Follows the reverse phases of fdtd/adjoint/native.py:48-727 and the seed
+    injection of fdtd/adjoint/seeds.py:922-990."""
+
+    def __init__(self, run: OracleRun):
+        self.run = run
+        plan, lib = run.plan, run.lib
+        real = lib._real
+        dt_np = lib._dtype
+        self.ae = [np.zeros(plan.field_shape(c), dt_np) for c in _COMP[:3]]
+        self.ah = [np.zeros(plan.field_shape(c), dt_np) for c in _COMP[3:]]
+        self.apsi = [np.zeros_like(p) for p in run.psi]
+        self.tmp = [np.zeros_like(a) for a in self.ae]
+        self.g_decay = [np.zeros_like(a) for a in self.ae]
+        self.g_curl = [np.zeros_like(a) for a in self.ae]
+        P = ctypes.POINTER(real)
+        self._pp = lambda arrs: (P * len(arrs))(*[_ptr(a, real) for a in arrs])
+        self.adj_poles = {}
+        A = _ade_adj_struct(real)
+        ad = A()
+        ad.dt = plan.dt
+        any_pole = False
+        for ci, c in enumerate(_COMP[:3]):
+            entry = run.ade[c]
+            ad.inv_eps[ci] = _ptr(entry["inv_eps"], real)
+            poles = []
+            for pi, p in enumerate(entry["poles"]):
+                any_pole = True
+                apol = np.zeros(plan.field_shape(c), dt_np)
+                acur = np.zeros(plan.field_shape(c), dt_np)
+                poles.append({"pol": apol, "cur": acur})
+                ad.kind[ci][pi] = p["kind"]
+                ad.decay[ci][pi] = p["decay"]
+                ad.restoring[ci][pi] = p["restoring"]
+                ad.drive[ci][pi] = _ptr(p["drive"], real)
+                ad.apol[ci][pi] = _ptr(apol, real)
+                ad.acur[ci][pi] = _ptr(acur, real)
+            ad.count[ci] = len(poles)
+            self.adj_poles[c] = poles
+        self.ad = ad if any_pole else None
+
+    # transpose of OracleRun._observe and the full-field DFT at step n
+    def seed(self, n, grads):
+        plan, run = self.run.plan, self.run
+        fields = dict(zip(_COMP, self.ae + self.ah))
+        if run.obs is not None and plan.obs_weights is not None:
+            ec, es, hc, hs = (w[n] for w in plan.obs_weights)
+            g_pre, g_pim = grads.get("point_re"), grads.get("point_im")
+            for pi, (comp, idx) in enumerate(run.obs.points):
+                wc, ws = (ec, es) if comp[0] == "E" else (hc, hs)
+                v = 0.0
+                if g_pre is not None:
+                    v += float(np.dot(g_pre[:, pi], wc))
+                if g_pim is not None:
+                    v += float(np.dot(g_pim[:, pi], ws))
+                fields[comp][idx] += v
+            for qi, (comp, axis, index) in enumerate(run.obs.planes):
+                wc, ws = (ec, es) if comp[0] == "E" else (hc, hs)
+                acc = 0.0
+                if grads.get("plane_re") is not None:
+                    acc = acc + np.tensordot(wc, grads["plane_re"][qi], axes=(0, 0))
+                if grads.get("plane_im") is not None:
+                    acc = acc + np.tensordot(ws, grads["plane_im"][qi], axes=(0, 0))
+                sl = [slice(None)] * 3
+                sl["xyz".index(axis)] = index
+                fields[comp][tuple(sl)] += acc
+        if plan.dft_weights is not None and grads.get("dft") is not None:
+            wc, ws = plan.dft_weights[0][n], plan.dft_weights[1][n]
+            for ci, c in enumerate(_COMP[:3]):
+                gre, gim = grads["dft"][c]
+                self.ae[ci] += np.tensordot(wc, gre, axes=(0, 0)) + np.tensordot(ws, gim, axes=(0, 0))
+
+    def reverse_step(self, n, gradients=True):
+        run = self.run
+        real = run.lib._real
+        fe = fh = fpsi = None
+        if gradients:
+            e_n, h_half, psi_e = run.trajectory[n]
+            fe = self._pp([e_n[c] for c in _COMP[:3]])
+            fh = self._pp([h_half[c] for c in _COMP[3:]])
+            fpsi = self._pp(psi_e)
+        run.lib.or_reverse_step(ctypes.byref(run.grid), self._pp(self.ae), self._pp(self.ah), self._pp(self.apsi),
+                                self._pp(self.tmp), ctypes.byref(self.ad) if self.ad is not None else None,
+                                fe, fh, fpsi, self._pp(self.g_decay) if gradients else None,
+                                self._pp(self.g_curl) if gradients else None)
+
+    def run_reverse(self, grads):
+        steps = len(self.run.trajectory)
+        for n in range(steps - 1, -1, -1):
+            self.seed(n, grads)
+            self.reverse_step(n)
+        return self

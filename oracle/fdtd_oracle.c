@@ -323,3 +323,242 @@ void or_accumulate(int64_t n, const real_t* v, int32_t nf, const double* wcos, c
 }
 
 int32_t or_real_bytes(void) { return (int32_t)sizeof(real_t); }
+
+/* ------------------------------------------------------------------ reverse
+ * Exact transpose of one forward step in the driver's order (H update, ADE
+ * advance, E update, ADE subtraction), restating adjoint.cu:149-420 (standard),
+ * :1224-1452 (CPML psi) and :898-979 (ADE) in scatter form: every forward read
+ * x -> y becomes lambda_x += d y/d x * lambda_y.  Source terms are constants and
+ * drop out.  On entry ae/ah/apsi/ade hold the adjoint of the state after step n
+ * (E^{n+1}, H^{n+1/2}, psi^{n+1}, ADE state^{n+1}) with the step's observation
+ * seeds already added; on exit they hold the adjoint of the state before it.
+ *
+ * When fe/fh/fpsi (E^n, H^{n+1/2}, psi_E^{n+1} of the forward run) are given, the
+ * gradients of the loss with respect to the E decay and curl coefficients are
+ * accumulated into g_decay/g_curl (E-shaped): dL/dcurl += lambda_post * curl^n,
+ * dL/ddecay += lambda_post * E^n on every updated sample. */
+
+typedef struct {
+  int32_t count[3];
+  int32_t kind[3][4];
+  double decay[3][4];
+  double restoring[3][4];
+  const real_t* drive[3][4];
+  real_t* apol[3][4];
+  real_t* acur[3][4];
+  const real_t* inv_eps[3];
+  double dt;
+} or_ade_adj;
+
+/* transpose of bdiff: scatter w into the H line (same index rules) */
+static inline void bdiff_t(int64_t c, int64_t size, int lo, int hi, real_t w, real_t* cur, real_t* prev, real_t* first,
+                           real_t* last) {
+  if (c == 0) {
+    if (lo == BC_PERIODIC) { *first += w; *last -= w; }
+    else if (lo == BC_PMC) *first += 2.0f * w;
+    return;
+  }
+  if (c + 1 == size) {
+    if (hi == BC_PERIODIC) { *first += w; *last -= w; }
+    else if (hi == BC_PMC) *last -= 2.0f * w;
+    return;
+  }
+  *cur += w;
+  *prev -= w;
+}
+
+/* one derivative term of an updated sample: psi adjoint and the weight of d */
+static inline real_t term_t(const or_grid* g, int cpml, int e_side, int axis, int64_t coord, real_t s, real_t mu,
+                            real_t* apsi_l) {
+  if (!cpml) return s * mu;
+  const real_t kinv = e_side ? g->kinv_e[axis][coord] : g->kinv_h[axis][coord];
+  const real_t b = e_side ? g->b_e[axis][coord] : g->b_h[axis][coord];
+  const real_t c = e_side ? g->c_e[axis][coord] : g->c_h[axis][coord];
+  const real_t tot = *apsi_l + s * mu;
+  *apsi_l = b * tot;
+  return s * mu * kinv + c * tot;
+}
+
+void or_reverse_step(const or_grid* g, real_t* const* ae, real_t* const* ah, real_t* const* apsi, real_t* const* tmp,
+                     const or_ade_adj* ad, real_t* const* fe, real_t* const* fh, real_t* const* fpsi,
+                     real_t* const* g_decay, real_t* const* g_curl) {
+  const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
+  const int *lo = g->bc_lo, *hi = g->bc_hi;
+  const int cp = g->cpml;
+  real_t *hx = ah[0], *hy = ah[1], *hz = ah[2];
+  /* ---- E side: ADE subtraction, E update and ADE advance, transposed ---- */
+  for (int c = 0; c < 3; ++c) {
+    const int64_t sx = c == 0 ? nx - 1 : nx, sy = c == 1 ? ny - 1 : ny, sz = c == 2 ? nz - 1 : nz;
+    for (int64_t i = 0; i < sx; ++i)
+      for (int64_t j = 0; j < sy; ++j)
+        for (int64_t k = 0; k < sz; ++k) {
+          const int64_t l = off3(i, j, k, sy, sz);
+          const real_t lam = ae[c][l];
+          real_t out = 0;
+          /* transverse axes (a, b) of this component and the sign of each term */
+          const int ax_a = c == 0 ? 1 : 0, ax_b = c == 2 ? 1 : 2;
+          const int64_t ca = ax_a == 0 ? i : j, cb = ax_b == 1 ? j : k;
+          const int64_t na = ax_a == 0 ? nx : ny, nb = ax_b == 1 ? ny : nz;
+          const real_t sa = c == 1 ? -1.0f : 1.0f, sb = -sa;
+          const int pec = is_pec(ca, na, lo[ax_a], hi[ax_a]) || is_pec(cb, nb, lo[ax_b], hi[ax_b]);
+          const int inactive = is_inactive(ca, na, lo[ax_a], hi[ax_a]) || is_inactive(cb, nb, lo[ax_b], hi[ax_b]);
+          if (pec) {
+            out = 0;
+          } else if (inactive) {
+            out = lam;
+          } else {
+            out = g->decay[c][l] * lam;
+            const real_t mu = g->curl[c][l] * lam;
+            const real_t wa = term_t(g, cp, 1, ax_a, ca, sa, mu, &apsi[2 * c][l]) * g->inv_e[ax_a][ca];
+            const real_t wb = term_t(g, cp, 1, ax_b, cb, sb, mu, &apsi[2 * c + 1][l]) * g->inv_e[ax_b][cb];
+            /* H lines read by the two differences: a = d(H_pa)/d(ax_a), b = d(H_pb)/d(ax_b) */
+            if (c == 0) { /* Ex: a = dHz/dy, b = dHy/dz */
+              const int64_t jc = j < ny - 1 ? j : ny - 2, jp = j > 0 ? j - 1 : 0;
+              const int64_t kc = k < nz - 1 ? k : nz - 2, kp = k > 0 ? k - 1 : 0;
+              bdiff_t(j, ny, lo[1], hi[1], wa, &hz[off3(i, jc, k, ny - 1, nz)], &hz[off3(i, jp, k, ny - 1, nz)],
+                      &hz[off3(i, 0, k, ny - 1, nz)], &hz[off3(i, ny - 2, k, ny - 1, nz)]);
+              bdiff_t(k, nz, lo[2], hi[2], wb, &hy[off3(i, j, kc, ny, nz - 1)], &hy[off3(i, j, kp, ny, nz - 1)],
+                      &hy[off3(i, j, 0, ny, nz - 1)], &hy[off3(i, j, nz - 2, ny, nz - 1)]);
+            } else if (c == 1) { /* Ey: a = dHz/dx, b = dHx/dz */
+              const int64_t ic = i < nx - 1 ? i : nx - 2, ip = i > 0 ? i - 1 : 0;
+              const int64_t kc = k < nz - 1 ? k : nz - 2, kp = k > 0 ? k - 1 : 0;
+              bdiff_t(i, nx, lo[0], hi[0], wa, &hz[off3(ic, j, k, ny - 1, nz)], &hz[off3(ip, j, k, ny - 1, nz)],
+                      &hz[off3(0, j, k, ny - 1, nz)], &hz[off3(nx - 2, j, k, ny - 1, nz)]);
+              bdiff_t(k, nz, lo[2], hi[2], wb, &hx[off3(i, j, kc, ny - 1, nz - 1)],
+                      &hx[off3(i, j, kp, ny - 1, nz - 1)], &hx[off3(i, j, 0, ny - 1, nz - 1)],
+                      &hx[off3(i, j, nz - 2, ny - 1, nz - 1)]);
+            } else { /* Ez: a = dHy/dx, b = dHx/dy */
+              const int64_t ic = i < nx - 1 ? i : nx - 2, ip = i > 0 ? i - 1 : 0;
+              const int64_t jc = j < ny - 1 ? j : ny - 2, jp = j > 0 ? j - 1 : 0;
+              bdiff_t(i, nx, lo[0], hi[0], wa, &hy[off3(ic, j, k, ny, nz - 1)], &hy[off3(ip, j, k, ny, nz - 1)],
+                      &hy[off3(0, j, k, ny, nz - 1)], &hy[off3(nx - 2, j, k, ny, nz - 1)]);
+              bdiff_t(j, ny, lo[1], hi[1], wb, &hx[off3(i, jc, k, ny - 1, nz - 1)],
+                      &hx[off3(i, jp, k, ny - 1, nz - 1)], &hx[off3(i, 0, k, ny - 1, nz - 1)],
+                      &hx[off3(i, ny - 2, k, ny - 1, nz - 1)]);
+            }
+            if (fe && g_curl) {
+              /* curl^n of the forward update at this sample, recomputed */
+              const real_t* fa = c == 0 ? fh[2] : (c == 1 ? fh[2] : fh[1]);
+              const real_t* fb = c == 0 ? fh[1] : (c == 1 ? fh[0] : fh[0]);
+              real_t da, db;
+              if (c == 0) {
+                const int64_t jc = j < ny - 1 ? j : ny - 2, jp = j > 0 ? j - 1 : 0;
+                const int64_t kc = k < nz - 1 ? k : nz - 2, kp = k > 0 ? k - 1 : 0;
+                da = bdiff(j, ny, lo[1], hi[1], g->inv_e[1][j], fa[off3(i, jc, k, ny - 1, nz)],
+                           fa[off3(i, jp, k, ny - 1, nz)], fa[off3(i, 0, k, ny - 1, nz)],
+                           fa[off3(i, ny - 2, k, ny - 1, nz)]);
+                db = bdiff(k, nz, lo[2], hi[2], g->inv_e[2][k], fb[off3(i, j, kc, ny, nz - 1)],
+                           fb[off3(i, j, kp, ny, nz - 1)], fb[off3(i, j, 0, ny, nz - 1)],
+                           fb[off3(i, j, nz - 2, ny, nz - 1)]);
+              } else if (c == 1) {
+                const int64_t ic = i < nx - 1 ? i : nx - 2, ip = i > 0 ? i - 1 : 0;
+                const int64_t kc = k < nz - 1 ? k : nz - 2, kp = k > 0 ? k - 1 : 0;
+                da = bdiff(i, nx, lo[0], hi[0], g->inv_e[0][i], fa[off3(ic, j, k, ny - 1, nz)],
+                           fa[off3(ip, j, k, ny - 1, nz)], fa[off3(0, j, k, ny - 1, nz)],
+                           fa[off3(nx - 2, j, k, ny - 1, nz)]);
+                db = bdiff(k, nz, lo[2], hi[2], g->inv_e[2][k], fb[off3(i, j, kc, ny - 1, nz - 1)],
+                           fb[off3(i, j, kp, ny - 1, nz - 1)], fb[off3(i, j, 0, ny - 1, nz - 1)],
+                           fb[off3(i, j, nz - 2, ny - 1, nz - 1)]);
+              } else {
+                const int64_t ic = i < nx - 1 ? i : nx - 2, ip = i > 0 ? i - 1 : 0;
+                const int64_t jc = j < ny - 1 ? j : ny - 2, jp = j > 0 ? j - 1 : 0;
+                da = bdiff(i, nx, lo[0], hi[0], g->inv_e[0][i], fa[off3(ic, j, k, ny, nz - 1)],
+                           fa[off3(ip, j, k, ny, nz - 1)], fa[off3(0, j, k, ny, nz - 1)],
+                           fa[off3(nx - 2, j, k, ny, nz - 1)]);
+                db = bdiff(j, ny, lo[1], hi[1], g->inv_e[1][j], fb[off3(i, jc, k, ny - 1, nz - 1)],
+                           fb[off3(i, jp, k, ny - 1, nz - 1)], fb[off3(i, 0, k, ny - 1, nz - 1)],
+                           fb[off3(i, ny - 2, k, ny - 1, nz - 1)]);
+              }
+              real_t ta = da, tb = db;
+              if (cp) {
+                ta = da * g->kinv_e[ax_a][ca] + fpsi[2 * c][l];
+                tb = db * g->kinv_e[ax_b][cb] + fpsi[2 * c + 1][l];
+              }
+              const real_t curl = sa * (ta - tb);
+              g_curl[c][l] += lam * curl;
+              g_decay[c][l] += lam * fe[c][l];
+            }
+          }
+          /* ADE: subtraction E -= dt*J'*inv_eps, then the advance from E^n */
+          if (ad) {
+            const real_t lj = -(real_t)ad->dt * ad->inv_eps[c][l] * lam;
+            for (int p = 0; p < ad->count[c]; ++p) {
+              const real_t d = (real_t)ad->decay[c][p], r = (real_t)ad->restoring[c][p];
+              const real_t gdr = ad->drive[c][p][l];
+              real_t* apol = ad->apol[c][p];
+              real_t* acur = ad->acur[c][p];
+              if (ad->kind[c][p] == 0) {
+                /* J' = (P' - P)/dt feeds the subtraction; J^n itself is never read */
+                const real_t ljt = acur[l] + lj;
+                const real_t lp = apol[l] + ljt * (real_t)(1.0 / ad->dt);
+                apol[l] = d * lp - ljt * (real_t)(1.0 / ad->dt);
+                acur[l] = 0;
+                out += gdr * lp;
+              } else if (ad->kind[c][p] == 1) {
+                const real_t lc = acur[l] + lj;
+                acur[l] = d * lc;
+                out += gdr * lc;
+              } else {
+                const real_t lpol = apol[l];
+                const real_t lnc = acur[l] + lj + (real_t)ad->dt * lpol;
+                acur[l] = d * lnc;
+                apol[l] = lpol - r * lnc;
+                out += gdr * lnc;
+              }
+            }
+          }
+          tmp[c][l] = out;
+        }
+  }
+  /* ---- H side: H update transposed; scatter into the E adjoint ---- */
+  real_t *tx = tmp[0], *ty = tmp[1], *tz = tmp[2];
+  /* Hx (nx, ny-1, nz-1): a = dEz/dy (+), b = dEy/dz (-) */
+  for (int64_t i = 0; i < nx; ++i)
+    for (int64_t j = 0; j < ny - 1; ++j)
+      for (int64_t k = 0; k < nz - 1; ++k) {
+        const int64_t l = off3(i, j, k, ny - 1, nz - 1);
+        const real_t lam = hx[l];
+        const real_t nu = -g->curl[3][l] * lam;
+        const real_t wa = term_t(g, cp, 0, 1, j, 1.0f, nu, &apsi[6][l]) * g->inv_h[1][j];
+        const real_t wb = term_t(g, cp, 0, 2, k, -1.0f, nu, &apsi[7][l]) * g->inv_h[2][k];
+        tz[off3(i, j + 1, k, ny, nz - 1)] += wa;
+        tz[off3(i, j, k, ny, nz - 1)] -= wa;
+        ty[off3(i, j, k + 1, ny - 1, nz)] += wb;
+        ty[off3(i, j, k, ny - 1, nz)] -= wb;
+        hx[l] = g->decay[3][l] * lam;
+      }
+  /* Hy (nx-1, ny, nz-1): a = dEz/dx (-), b = dEx/dz (+) */
+  for (int64_t i = 0; i < nx - 1; ++i)
+    for (int64_t j = 0; j < ny; ++j)
+      for (int64_t k = 0; k < nz - 1; ++k) {
+        const int64_t l = off3(i, j, k, ny, nz - 1);
+        const real_t lam = hy[l];
+        const real_t nu = -g->curl[4][l] * lam;
+        const real_t wa = term_t(g, cp, 0, 0, i, -1.0f, nu, &apsi[8][l]) * g->inv_h[0][i];
+        const real_t wb = term_t(g, cp, 0, 2, k, 1.0f, nu, &apsi[9][l]) * g->inv_h[2][k];
+        tz[off3(i + 1, j, k, ny, nz - 1)] += wa;
+        tz[off3(i, j, k, ny, nz - 1)] -= wa;
+        tx[off3(i, j, k + 1, ny, nz)] += wb;
+        tx[off3(i, j, k, ny, nz)] -= wb;
+        hy[l] = g->decay[4][l] * lam;
+      }
+  /* Hz (nx-1, ny-1, nz): a = dEy/dx (+), b = dEx/dy (-) */
+  for (int64_t i = 0; i < nx - 1; ++i)
+    for (int64_t j = 0; j < ny - 1; ++j)
+      for (int64_t k = 0; k < nz; ++k) {
+        const int64_t l = off3(i, j, k, ny - 1, nz);
+        const real_t lam = hz[l];
+        const real_t nu = -g->curl[5][l] * lam;
+        const real_t wa = term_t(g, cp, 0, 0, i, 1.0f, nu, &apsi[10][l]) * g->inv_h[0][i];
+        const real_t wb = term_t(g, cp, 0, 1, j, -1.0f, nu, &apsi[11][l]) * g->inv_h[1][j];
+        ty[off3(i + 1, j, k, ny - 1, nz)] += wa;
+        ty[off3(i, j, k, ny - 1, nz)] -= wa;
+        tx[off3(i, j + 1, k, ny, nz)] += wb;
+        tx[off3(i, j, k, ny, nz)] -= wb;
+        hz[l] = g->decay[5][l] * lam;
+      }
+  for (int c = 0; c < 3; ++c) {
+    const int64_t n = (int64_t)(c == 0 ? nx - 1 : nx) * (c == 1 ? ny - 1 : ny) * (c == 2 ? nz - 1 : nz);
+    memcpy(ae[c], tmp[c], (size_t)n * sizeof(real_t));
+  }
+}

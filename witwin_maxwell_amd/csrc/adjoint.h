@@ -1,0 +1,392 @@
+// Reverse (adjoint) step kernels: the exact transpose of one forward step
+// (H update, ADE advance, E update, ADE subtraction), in gather form so that no
+// two threads write the same sample (no atomics on the field adjoints).
+//
+// Reference: fdtd/cuda/kernels/adjoint.cu:149-420 (standard reverse phases),
+// :1224-1452 (CPML psi adjoints), :898-979 (Debye/Drude/Lorentz currents),
+// :1737-1870 (seed injection); phase order fdtd/adjoint/native.py:48-727.
+//
+// One reverse step n, with lambda holding the adjoint of the state after step n:
+//   1. E seeds of step n           lambda_E += sum_f gRe w_cos[n,f] + gIm w_sin[n,f]
+//   2. reverse_h (P1), per H sample:
+//        finish step n+1's H transpose (lambda_H <- decay_H lambda_H and the
+//        H-side psi adjoints, deferred so P2 of step n+1 could read them), then
+//        lambda_H += D_b^T q(lambda_E, lambda_psi_E)      (E update transposed)
+//   3. H seeds of step n
+//   4. coefficient gradients over the design window (optional)
+//   5. reverse_e (P2), per E sample: boundary mask, decay, E-side psi adjoints,
+//        ADE subtraction/advance transposes, lambda_E += D_f^T r(lambda_H, ...)
+//        (H update transposed)
+// Finishing the H transpose of step n+1 inside P1 of step n is exact because an
+// all-zero adjoint finishes to zero, so the first reverse step needs no flag.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "maxwell_fdtd.h"
+#include "terms.h"
+#include "yee.h"
+
+namespace mw {
+
+struct RevArgs {
+  mw_grid g;
+  mw_coeffs cf;
+  mw_cpml cp;   // CPML profiles of the forward run; psi[] are the adjoint psi slabs
+  mw_fields a;  // adjoint fields
+  mw_ade ad;    // forward pole constants; polarization/current are adjoint states
+  int32_t cpml;
+  int32_t ade;
+};
+
+__device__ __forceinline__ int axis_size(const mw_grid& g, int axis) {
+  return axis == 0 ? g.nx : (axis == 1 ? g.ny : g.nz);
+}
+
+__device__ __forceinline__ void e_shape(const mw_grid& g, int c, int& sx, int& sy, int& sz) {
+  sx = c == 0 ? g.nx - 1 : g.nx;
+  sy = c == 1 ? g.ny - 1 : g.ny;
+  sz = c == 2 ? g.nz - 1 : g.nz;
+}
+__device__ __forceinline__ void h_shape(const mw_grid& g, int h, int& sx, int& sy, int& sz) {
+  sx = h == 0 ? g.nx : g.nx - 1;
+  sy = h == 1 ? g.ny : g.ny - 1;
+  sz = h == 2 ? g.nz : g.nz - 1;
+}
+
+// Derivative terms of the E update (kernels/electric.cu:463-553):
+//   Ex: a = dHz/dy, b = dHy/dz, curl = a - b
+//   Ey: a = dHz/dx, b = dHx/dz, curl = b - a
+//   Ez: a = dHy/dx, b = dHx/dy, curl = a - b
+__device__ __forceinline__ int e_axis(int c, int t) { return t == 0 ? (c == 0 ? 1 : 0) : (c == 2 ? 1 : 2); }
+__device__ __forceinline__ float e_sign(int c, int t) { return ((c == 1) != (t == 1)) ? -1.0f : 1.0f; }
+// Derivative terms of the H update (kernels/magnetic.cu:62-117):
+//   Hx: a = dEz/dy, b = dEy/dz, curl = a - b
+//   Hy: a = dEz/dx, b = dEx/dz, curl = b - a
+//   Hz: a = dEy/dx, b = dEx/dy, curl = a - b
+__device__ __forceinline__ int h_axis(int h, int t) { return t == 0 ? (h == 0 ? 1 : 0) : (h == 2 ? 1 : 2); }
+__device__ __forceinline__ float h_sign(int h, int t) { return ((h == 1) != (t == 1)) ? -1.0f : 1.0f; }
+
+__device__ __forceinline__ int pick(int axis, int i, int j, int k) { return axis == 0 ? i : (axis == 1 ? j : k); }
+
+__device__ __forceinline__ float* psi_at(const mw_cpml& cp, int q, int axis, int coord, int i, int j, int k, int sy,
+                                         int sz) {
+  const int local = slab_local(coord, cp.psi_low[q], cp.psi_high_start[q], cp.psi_high_len[q]);
+  if (local < 0) return nullptr;
+  return cp.psi[q] + slab_offset(axis, i, j, k, local, sy, sz, cp.psi_low[q] + cp.psi_high_len[q]);
+}
+
+// Is E sample (c; i,j,k) held by its boundary mode (PEC -> 0, none/PML face ->
+// unchanged)?  Returns 0 active, 1 PEC, 2 inactive (kernels/electric.cu:23-51).
+__device__ __forceinline__ int e_mask(const mw_grid& g, int c, int i, int j, int k) {
+  int m = 0;
+  for (int t = 0; t < 2; ++t) {
+    const int ax = e_axis(c, t);
+    const int x = pick(ax, i, j, k), n = axis_size(g, ax), lo = g.bc_lo[ax], hi = g.bc_hi[ax];
+    if ((x == 0 && lo == MW_BC_PEC) || (x == n - 1 && hi == MW_BC_PEC)) return 1;
+    if ((x == 0 && (lo == MW_BC_NONE || lo == MW_BC_PML)) || (x == n - 1 && (hi == MW_BC_NONE || hi == MW_BC_PML)))
+      m = 2;
+  }
+  return m;
+}
+
+// Adjoint weight of the backward difference of E sample (c; i,j,k), term t,
+// times its spacing reciprocal: q = (s mu kinv + c_e (lambda_psi + s mu)) inv_e
+// with mu = curl_E lambda_E; zero for held samples.
+template <bool CPML>
+__device__ __forceinline__ float q_e(const RevArgs& a, int c, int t, int i, int j, int k) {
+  if (e_mask(a.g, c, i, j, k)) return 0.0f;
+  int sx, sy, sz;
+  e_shape(a.g, c, sx, sy, sz);
+  const uint32_t o = ((uint32_t)i * sy + j) * sz + k;
+  const float mu = coef(a.cf.curl[c], a.cf.curl_value[c], o) * a.a.e[c][o];
+  const int ax = e_axis(c, t);
+  const int x = pick(ax, i, j, k);
+  const float s = e_sign(c, t);
+  float w = s * mu;
+  if (CPML) {
+    const float* p = psi_at(a.cp, 2 * c + t, ax, x, i, j, k, sy, sz);
+    const float lp = p ? *p : 0.0f;
+    w = s * mu * a.cp.inv_kappa_e[ax][x] + a.cp.c_e[ax][x] * (lp + s * mu);
+  }
+  return w * a.g.inv_e[ax][x];
+}
+
+// Same for the forward difference of H sample (h; i,j,k), term t, with
+// nu = -curl_H lambda_H (the H-side psi adjoint read before it is finished).
+template <bool CPML>
+__device__ __forceinline__ float r_h(const RevArgs& a, int h, int t, int i, int j, int k) {
+  int sx, sy, sz;
+  h_shape(a.g, h, sx, sy, sz);
+  const uint32_t o = ((uint32_t)i * sy + j) * sz + k;
+  const float nu = -coef(a.cf.curl[3 + h], a.cf.curl_value[3 + h], o) * a.a.h[h][o];
+  const int ax = h_axis(h, t);
+  const int x = pick(ax, i, j, k);
+  const float s = h_sign(h, t);
+  float w = s * nu;
+  if (CPML) {
+    const float* p = psi_at(a.cp, 6 + 2 * h + t, ax, x, i, j, k, sy, sz);
+    const float lp = p ? *p : 0.0f;
+    w = s * nu * a.cp.inv_kappa_h[ax][x] + a.cp.c_h[ax][x] * (lp + s * nu);
+  }
+  return w * a.g.inv_h[ax][x];
+}
+
+// Sum over the E samples whose backward difference (with the face rules of
+// electric.cu:23-87) reads H at position m of this axis line.
+template <bool CPML>
+__device__ __forceinline__ float gather_e(const RevArgs& a, int c, int t, int i, int j, int k) {
+  const int ax = e_axis(c, t);
+  const int n = axis_size(a.g, ax), lo = a.g.bc_lo[ax], hi = a.g.bc_hi[ax];
+  const int m = pick(ax, i, j, k);
+  auto q = [&](int x) {
+    return q_e<CPML>(a, c, t, ax == 0 ? x : i, ax == 1 ? x : j, ax == 2 ? x : k);
+  };
+  float s = 0.0f;
+  if (m > 0) {
+    s += q(m);
+  } else {
+    if (lo == MW_BC_PERIODIC) s += q(0);
+    else if (lo == MW_BC_PMC) s += 2.0f * q(0);
+    if (hi == MW_BC_PERIODIC) s += q(n - 1);
+  }
+  if (m + 1 < n - 1) {
+    s -= q(m + 1);
+  } else {
+    if (hi == MW_BC_PERIODIC) s -= q(n - 1);
+    else if (hi == MW_BC_PMC) s -= 2.0f * q(n - 1);
+    if (lo == MW_BC_PERIODIC) s -= q(0);
+  }
+  return s;
+}
+
+// P1: lambda_H of every H sample at (i, j, k).
+template <bool CPML>
+__global__ __launch_bounds__(256) void reverse_h_kernel(RevArgs a) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  const int i = blockIdx.z;
+  const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+  if (k >= nz || j >= ny || i >= nx) return;
+  for (int h = 0; h < 3; ++h) {
+    int sx, sy, sz;
+    h_shape(a.g, h, sx, sy, sz);
+    if (i >= sx || j >= sy || k >= sz) continue;
+    const uint32_t o = ((uint32_t)i * sy + j) * sz + k;
+    const float lam = a.a.h[h][o];
+    // finish the previous reverse step's H transpose
+    if (CPML) {
+      const float nu = -coef(a.cf.curl[3 + h], a.cf.curl_value[3 + h], o) * lam;
+      for (int t = 0; t < 2; ++t) {
+        const int ax = h_axis(h, t);
+        const int x = pick(ax, i, j, k);
+        float* p = psi_at(a.cp, 6 + 2 * h + t, ax, x, i, j, k, sy, sz);
+        if (p) *p = a.cp.b_h[ax][x] * (*p + h_sign(h, t) * nu);
+      }
+    }
+    float v = coef(a.cf.decay[3 + h], a.cf.decay_value[3 + h], o) * lam;
+    // E samples whose curl read this H sample
+    if (h == 0) v += gather_e<CPML>(a, 2, 1, i, j, k) + gather_e<CPML>(a, 1, 1, i, j, k);       // Ez b, Ey b
+    else if (h == 1) v += gather_e<CPML>(a, 0, 1, i, j, k) + gather_e<CPML>(a, 2, 0, i, j, k);  // Ex b, Ez a
+    else v += gather_e<CPML>(a, 0, 0, i, j, k) + gather_e<CPML>(a, 1, 0, i, j, k);              // Ex a, Ey a
+    a.a.h[h][o] = v;
+  }
+}
+
+// P2: lambda_E of every E sample at (i, j, k).
+template <bool CPML, bool ADE>
+__global__ __launch_bounds__(256) void reverse_e_kernel(RevArgs a) {
+  const int k = blockIdx.x * 64 + threadIdx.x;
+  const int j = blockIdx.y * 4 + threadIdx.y;
+  const int i = blockIdx.z;
+  const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+  if (k >= nz || j >= ny || i >= nx) return;
+  for (int c = 0; c < 3; ++c) {
+    int sx, sy, sz;
+    e_shape(a.g, c, sx, sy, sz);
+    if (i >= sx || j >= sy || k >= sz) continue;
+    const uint32_t o = ((uint32_t)i * sy + j) * sz + k;
+    const float lam = a.a.e[c][o];
+    const int mask = e_mask(a.g, c, i, j, k);
+    float out;
+    if (mask == 1) {
+      out = 0.0f;
+    } else if (mask == 2) {
+      out = lam;
+    } else {
+      out = coef(a.cf.decay[c], a.cf.decay_value[c], o) * lam;
+      if (CPML) {
+        const float mu = coef(a.cf.curl[c], a.cf.curl_value[c], o) * lam;
+        for (int t = 0; t < 2; ++t) {
+          const int ax = e_axis(c, t);
+          const int x = pick(ax, i, j, k);
+          float* p = psi_at(a.cp, 2 * c + t, ax, x, i, j, k, sy, sz);
+          if (p) *p = a.cp.b_e[ax][x] * (*p + e_sign(c, t) * mu);
+        }
+      }
+    }
+    if (ADE) {
+      // E -= dt J' inv_eps, then the pole advances from E^n (kernels/dispersive.cu:18-83)
+      const int np = a.ad.count[c];
+      if (np > 0) {
+        const float dt = a.ad.dt;
+        const float lj = -dt * a.ad.inv_eps[c][o] * lam;
+        for (int p = 0; p < np; ++p) {
+          const mw_pole& pl = a.ad.poles[c][p];
+          const float drive = pl.drive[o];
+          if (pl.kind == 0) {  // Debye: P' = d P + g E, J' = (P' - P)/dt; J^n unread
+            const float ljt = pl.current[o] + lj;
+            const float lp = pl.polarization[o] + ljt * (1.0f / dt);
+            pl.polarization[o] = pl.decay * lp - ljt * (1.0f / dt);
+            pl.current[o] = 0.0f;
+            out += drive * lp;
+          } else if (pl.kind == 1) {  // Drude: J' = d J + g E
+            const float lc = pl.current[o] + lj;
+            pl.current[o] = pl.decay * lc;
+            out += drive * lc;
+          } else {  // Lorentz: J' = d J - r P + g E, P' = P + dt J'
+            const float lpol = pl.polarization[o];
+            const float lnc = pl.current[o] + lj + dt * lpol;
+            pl.current[o] = pl.decay * lnc;
+            pl.polarization[o] = lpol - pl.restoring * lnc;
+            out += drive * lnc;
+          }
+        }
+      }
+    }
+    // H samples whose forward difference read this E sample: +r(m = x-1) - r(m = x)
+    auto fwd = [&](int h, int t) {
+      const int ax = h_axis(h, t);
+      const int x = pick(ax, i, j, k), n = axis_size(a.g, ax);
+      float s = 0.0f;
+      if (x >= 1) s += r_h<CPML>(a, h, t, ax == 0 ? x - 1 : i, ax == 1 ? x - 1 : j, ax == 2 ? x - 1 : k);
+      if (x <= n - 2) s -= r_h<CPML>(a, h, t, i, j, k);
+      return s;
+    };
+    if (c == 0) out += fwd(1, 1) + fwd(2, 1);       // Hy b (dEx/dz), Hz b (dEx/dy)
+    else if (c == 1) out += fwd(0, 1) + fwd(2, 0);  // Hx b (dEy/dz), Hz a (dEy/dx)
+    else out += fwd(0, 0) + fwd(1, 0);              // Hx a (dEz/dy), Hy a (dEz/dx)
+    a.a.e[c][o] = out;
+  }
+}
+
+// --------------------------------------------------------------- seeds
+// Transposes of the observer accumulations (fdtd/adjoint/seeds.py:922-990,
+// kernels/adjoint.cu:1737-1858).  Several observers may name one sample, so the
+// plane and point seeds add atomically (vector-memory float atomics).
+struct SeedPlaneSet {
+  int32_t count;
+  int32_t pad;
+  mw_plane_obs p[MW_MAX_PLANES];  // re/im = gradients of the sums
+};
+
+__global__ __launch_bounds__(256) void seed_planes_kernel(SeedPlaneSet set, FieldSet fs, int nf, const float* e_cos,
+                                                          const float* e_sin, const float* h_cos, const float* h_sin,
+                                                          int family) {
+  const int qi = blockIdx.y;
+  if (qi >= set.count) return;
+  const mw_plane_obs& q = set.p[qi];
+  const int c = q.component;
+  if ((c < 3) != (family == 0)) return;
+  const int sx = fs.shape[c][0], sy = fs.shape[c][1], sz = fs.shape[c][2];
+  int na, nb;
+  if (q.axis == 0) { na = sy; nb = sz; } else if (q.axis == 1) { na = sx; nb = sz; } else { na = sx; nb = sy; }
+  const int64_t n = (int64_t)na * nb;
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n) return;
+  const int ia = (int)(l / nb), ib = (int)(l % nb);
+  uint32_t o;
+  if (q.axis == 0) o = ((uint32_t)q.index * sy + ia) * sz + ib;
+  else if (q.axis == 1) o = ((uint32_t)ia * sy + q.index) * sz + ib;
+  else o = ((uint32_t)ia * sy + ib) * sz + q.index;
+  const float* wc = c < 3 ? e_cos : h_cos;
+  const float* ws = c < 3 ? e_sin : h_sin;
+  float v = 0.0f;
+  for (int f = 0; f < nf; ++f) v += q.re[f * n + l] * wc[f] + q.im[f * n + l] * ws[f];
+  atomicAdd(fs.ptr[c] + o, v);
+}
+
+__global__ __launch_bounds__(64) void seed_points_kernel(FieldSet fs, int count, const int32_t* comp,
+                                                         const int32_t* lin, int nf, const float* e_cos,
+                                                         const float* e_sin, const float* h_cos, const float* h_sin,
+                                                         const float* gre, const float* gim, int family) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= count) return;
+  const int c = comp[p];
+  if ((c < 3) != (family == 0)) return;
+  const float* wc = c < 3 ? e_cos : h_cos;
+  const float* ws = c < 3 ? e_sin : h_sin;
+  float v = 0.0f;
+  for (int f = 0; f < nf; ++f) v += gre[f * count + p] * wc[f] + gim[f * count + p] * ws[f];
+  atomicAdd(fs.ptr[c] + lin[p], v);
+}
+
+__global__ __launch_bounds__(256) void seed_dense_kernel(float* __restrict__ lam, int64_t n, int nf,
+                                                         const float* __restrict__ wc, const float* __restrict__ ws,
+                                                         const float* __restrict__ gre,
+                                                         const float* __restrict__ gim) {
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n) return;
+  float v = 0.0f;
+  for (int f = 0; f < nf; ++f) v += gre[f * n + l] * wc[f] + gim[f * n + l] * ws[f];
+  lam[l] += v;
+}
+
+// ------------------------------------------------------- design window
+// The window is the node box [lo, hi); E^n is captured on it and H^{n+1/2} on
+// [lo-1, hi) (every H sample the E curl inside the window reads).
+struct WindowArgs {
+  int32_t lo[3], hi[3];
+};
+
+__global__ __launch_bounds__(256) void window_capture_kernel(WindowArgs w, mw_grid g, const float* e0, const float* e1,
+                                                             const float* e2, float* dst, int halo) {
+  const int wx = w.hi[0] - w.lo[0] + halo, wy = w.hi[1] - w.lo[1] + halo, wz = w.hi[2] - w.lo[2] + halo;
+  const int64_t per = (int64_t)wx * wy * wz;
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (l >= per) return;
+  const int wk = (int)(l % wz), wj = (int)((l / wz) % wy), wi = (int)(l / ((int64_t)wz * wy));
+  const int i = w.lo[0] - halo + wi, j = w.lo[1] - halo + wj, k = w.lo[2] - halo + wk;
+  int sx, sy, sz;
+  const float* src;
+  if (halo) { h_shape(g, c, sx, sy, sz); } else { e_shape(g, c, sx, sy, sz); }
+  src = c == 0 ? e0 : (c == 1 ? e1 : e2);
+  dst[c * per + l] = src[((uint32_t)i * sy + j) * sz + k];
+}
+
+// dL/dcurl_E += lambda_E * curl^n and dL/ddecay_E += lambda_E * E^n on the
+// window (every window sample is an updated sample outside the CPML slabs).
+__global__ __launch_bounds__(256) void window_gradient_kernel(WindowArgs w, mw_grid g, mw_fields lam,
+                                                              const float* __restrict__ e_win,
+                                                              const float* __restrict__ h_win, float* g_decay0,
+                                                              float* g_decay1, float* g_decay2, float* g_curl0,
+                                                              float* g_curl1, float* g_curl2) {
+  const int wx = w.hi[0] - w.lo[0], wy = w.hi[1] - w.lo[1], wz = w.hi[2] - w.lo[2];
+  const int64_t per = (int64_t)wx * wy * wz;
+  const int hy = wy + 1, hz = wz + 1;
+  const int64_t hper = (int64_t)(wx + 1) * hy * hz;
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int c = blockIdx.y;
+  if (l >= per) return;
+  const int wk = (int)(l % wz), wj = (int)((l / wz) % wy), wi = (int)(l / ((int64_t)wz * wy));
+  const int i = w.lo[0] + wi, j = w.lo[1] + wj, k = w.lo[2] + wk;
+  // H window index of global (i', j', k') is (i'-lo+1, ...)
+  auto H = [&](int h, int di, int dj, int dk) {
+    return h_win[h * hper + ((int64_t)(wi + 1 + di) * hy + (wj + 1 + dj)) * hz + (wk + 1 + dk)];
+  };
+  float curl;
+  if (c == 0) curl = (H(2, 0, 0, 0) - H(2, 0, -1, 0)) * g.inv_e[1][j] - (H(1, 0, 0, 0) - H(1, 0, 0, -1)) * g.inv_e[2][k];
+  else if (c == 1) curl = (H(0, 0, 0, 0) - H(0, 0, 0, -1)) * g.inv_e[2][k] - (H(2, 0, 0, 0) - H(2, -1, 0, 0)) * g.inv_e[0][i];
+  else curl = (H(1, 0, 0, 0) - H(1, -1, 0, 0)) * g.inv_e[0][i] - (H(0, 0, 0, 0) - H(0, 0, -1, 0)) * g.inv_e[1][j];
+  int sx, sy, sz;
+  e_shape(g, c, sx, sy, sz);
+  const float lv = lam.e[c][((uint32_t)i * sy + j) * sz + k];
+  float* gd = c == 0 ? g_decay0 : (c == 1 ? g_decay1 : g_decay2);
+  float* gc = c == 0 ? g_curl0 : (c == 1 ? g_curl1 : g_curl2);
+  gd[l] += lv * e_win[c * per + l];
+  gc[l] += lv * curl;
+}
+
+}  // namespace mw

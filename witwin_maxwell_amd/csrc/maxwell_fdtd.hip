@@ -8,6 +8,7 @@
 #include <string>
 
 #include "maxwell_fdtd.h"
+#include "adjoint.h"
 #include "fused.h"
 #include "terms.h"
 #include "yee.h"
@@ -355,6 +356,40 @@ bool has_periodic(const mw_grid* g) {
   return false;
 }
 
+// state of step n (fused mode double-buffers E/H by step parity)
+const mw_fields& state_fields(const mw_program* p, int n) { return (p->fused && (n & 1)) ? p->fields_alt : p->fields; }
+
+mw::WindowArgs window_args(const mw_window* w) {
+  mw::WindowArgs a;
+  for (int x = 0; x < 3; ++x) { a.lo[x] = w->lo[x]; a.hi[x] = w->hi[x]; }
+  return a;
+}
+
+int64_t window_cells(const mw_window* w, int halo) {
+  return (int64_t)(w->hi[0] - w->lo[0] + halo) * (w->hi[1] - w->lo[1] + halo) * (w->hi[2] - w->lo[2] + halo);
+}
+
+int check_window(const mw_program* p, const mw_window* w) {
+  const mw_grid* g = &p->grid;
+  const int n[3] = {g->nx, g->ny, g->nz};
+  for (int x = 0; x < 3; ++x) {
+    if (w->lo[x] < 1 || w->hi[x] > n[x] - 1 || w->hi[x] <= w->lo[x])
+      return fail(MW_ERR_ARGUMENT, "window axis %d: [%d, %d) must lie in [1, %d)", x, w->lo[x], w->hi[x], n[x] - 1);
+  }
+  if (p->cpml.enabled) {
+    // E-side psi q=0..5 differentiate along axes y z x z x y
+    static const int axis_of[6] = {1, 2, 0, 2, 0, 1};
+    for (int q = 0; q < 6; ++q) {
+      const int ax = axis_of[q];
+      const int low = p->cpml.psi_low[q], hs = p->cpml.psi_high_start[q], hl = p->cpml.psi_high_len[q];
+      if (w->lo[ax] < low || (hl > 0 && w->hi[ax] > hs))
+        return fail(MW_ERR_ARGUMENT, "window axis %d [%d, %d) overlaps the CPML layers", ax, w->lo[ax], w->hi[ax]);
+    }
+  }
+  if (w->capacity <= 0 || !w->e_store || !w->h_store) return fail(MW_ERR_ARGUMENT, "window stores missing");
+  return MW_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -364,9 +399,10 @@ const char* mw_last_error(void) { return g_error.c_str(); }
 int32_t mw_abi_version(void) { return 1; }
 
 int32_t mw_abi_struct_sizes(int64_t* out, int32_t capacity) {
-  const int64_t sizes[] = {(int64_t)sizeof(mw_grid), (int64_t)sizeof(mw_coeffs), (int64_t)sizeof(mw_cpml),
-                           (int64_t)sizeof(mw_fields), (int64_t)sizeof(mw_pole), (int64_t)sizeof(mw_ade),
-                           (int64_t)sizeof(mw_term), (int64_t)sizeof(mw_plane_obs), (int64_t)sizeof(mw_program)};
+  const int64_t sizes[] = {(int64_t)sizeof(mw_grid),     (int64_t)sizeof(mw_coeffs),   (int64_t)sizeof(mw_cpml),
+                           (int64_t)sizeof(mw_fields),   (int64_t)sizeof(mw_pole),     (int64_t)sizeof(mw_ade),
+                           (int64_t)sizeof(mw_term),     (int64_t)sizeof(mw_plane_obs), (int64_t)sizeof(mw_program),
+                           (int64_t)sizeof(mw_window),   (int64_t)sizeof(mw_adjoint)};
   const int32_t n = (int32_t)(sizeof(sizes) / sizeof(sizes[0]));
   for (int32_t i = 0; i < n && i < capacity; ++i) out[i] = sizes[i];
   return n;
@@ -539,6 +575,172 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
                          p->obs_h_cos + off, p->obs_h_sin + off, p->point_re, p->point_im);
       if ((rc = check_launch("accumulate_points"))) return rc;
     }
+  }
+  return MW_OK;
+}
+
+int mw_capture_steps(const mw_program* p, int32_t first, int32_t count, const mw_window* w, void* stream) {
+  if (!p || !w) return fail(MW_ERR_ARGUMENT, "program or window is NULL");
+  int rc;
+  const mw_grid* g = &p->grid;
+  if ((rc = check_window(p, w))) return rc;
+  if (count < 0 || count > w->capacity)
+    return fail(MW_ERR_ARGUMENT, "capture of %d steps exceeds the window capacity %d", count, w->capacity);
+  // replay without observers: the forward sums were taken by the first pass
+  mw_program q = *p;
+  q.n_planes = 0;
+  q.n_points = 0;
+  q.nf_dft = 0;
+  hipStream_t s = (hipStream_t)stream;
+  const mw::WindowArgs wa = window_args(w);
+  const int64_t we = window_cells(w, 0), wh = window_cells(w, 1);
+  auto capture = [&](const mw_fields& f, bool h, float* dst) {
+    const float* const* src = h ? (const float* const*)f.h : (const float* const*)f.e;
+    hipLaunchKernelGGL(mw::window_capture_kernel, dim3(grid_blocks(h ? wh : we), 3), dim3(256), 0, s, wa, *g, src[0],
+                       src[1], src[2], dst, h ? 1 : 0);
+    return check_launch("window_capture");
+  };
+  if ((rc = capture(state_fields(p, first), false, w->e_store))) return rc;
+  for (int n = first; n < first + count; ++n) {
+    if ((rc = mw_run_steps(&q, n, 1, stream))) return rc;
+    const mw_fields& cur = state_fields(p, n + 1);
+    if ((rc = capture(cur, false, w->e_store + (int64_t)(n - first + 1) * 3 * we))) return rc;
+    if ((rc = capture(cur, true, w->h_store + (int64_t)(n - first) * 3 * wh))) return rc;
+  }
+  return MW_OK;
+}
+
+int mw_reverse_steps(const mw_program* p, const mw_adjoint* adj, const mw_window* w, int32_t first, int32_t count,
+                     void* stream) {
+  if (!p || !adj) return fail(MW_ERR_ARGUMENT, "program or adjoint is NULL");
+  int rc;
+  const mw_grid* g = &p->grid;
+  if ((rc = check_grid(g)) || (rc = check_fields(&adj->fields, true, true))) return rc;
+  if (first < 0 || count < 0 || first + count > p->total_steps)
+    return fail(MW_ERR_ARGUMENT, "steps [%d, %d) outside the program's %d steps", first, first + count, p->total_steps);
+  if (w) {
+    if ((rc = check_window(p, w))) return rc;
+    if (count > w->capacity) return fail(MW_ERR_ARGUMENT, "%d steps exceed the window capacity %d", count, w->capacity);
+    for (int c = 0; c < 3; ++c)
+      if (!w->g_decay[c] || !w->g_curl[c]) return fail(MW_ERR_ARGUMENT, "window gradient %d is NULL", c);
+  }
+  mw::RevArgs a{};
+  a.g = *g;
+  a.cf = p->coeffs;
+  a.cp = p->cpml;
+  a.a = adj->fields;
+  a.cpml = p->cpml.enabled ? 1 : 0;
+  if (a.cpml) {
+    for (int q = 0; q < 12; ++q) {
+      a.cp.psi[q] = adj->psi[q];
+      if (p->cpml.psi_low[q] + p->cpml.psi_high_len[q] > 0 && !adj->psi[q])
+        return fail(MW_ERR_ARGUMENT, "adjoint psi %d is NULL", q);
+    }
+  }
+  a.ade = p->has_ade ? 1 : 0;
+  if (a.ade) {
+    if ((rc = check_ade(g, &p->ade))) return rc;
+    a.ad = p->ade;
+    for (int c = 0; c < 3; ++c)
+      for (int q = 0; q < p->ade.count[c]; ++q) {
+        mw_pole& pl = a.ad.poles[c][q];
+        pl.polarization = adj->polarization[c][q];
+        pl.current = adj->current[c][q];
+        if (!pl.current || (pl.kind != 1 && !pl.polarization))
+          return fail(MW_ERR_ARGUMENT, "adjoint pole (%d, %d) has a NULL state array", c, q);
+      }
+  }
+  // observer seeds
+  if (p->n_planes < 0 || p->n_planes > MW_MAX_PLANES) return fail(MW_ERR_ARGUMENT, "%d planes (max %d)", p->n_planes, MW_MAX_PLANES);
+  mw::SeedPlaneSet planes{};
+  int64_t plane_cells = 0;
+  for (int q = 0; q < p->n_planes; ++q) {
+    if (!adj->g_plane_re[q] && !adj->g_plane_im[q]) continue;
+    if (!adj->g_plane_re[q] || !adj->g_plane_im[q])
+      return fail(MW_ERR_ARGUMENT, "plane %d: give both seed gradients or neither", q);
+    mw_plane_obs o = p->planes[q];
+    if (o.component < 0 || o.component > 5 || o.axis < 0 || o.axis > 2)
+      return fail(MW_ERR_ARGUMENT, "plane %d descriptor invalid", q);
+    {
+      int shp[3];
+      component_shape(g, o.component, shp);
+      if (o.index < 0 || o.index >= shp[o.axis]) return fail(MW_ERR_ARGUMENT, "plane %d index %d outside axis", q, o.index);
+    }
+    if (p->nf_obs <= 0 || !p->obs_e_cos || !p->obs_e_sin || !p->obs_h_cos || !p->obs_h_sin)
+      return fail(MW_ERR_ARGUMENT, "observer weight tables missing");
+    o.re = (float*)adj->g_plane_re[q];
+    o.im = (float*)adj->g_plane_im[q];
+    int shp[3];
+    component_shape(g, o.component, shp);
+    const int64_t n = o.axis == 0 ? (int64_t)shp[1] * shp[2] : (o.axis == 1 ? (int64_t)shp[0] * shp[2] : (int64_t)shp[0] * shp[1]);
+    plane_cells = std::max(plane_cells, n);
+    planes.p[planes.count++] = o;
+  }
+  const bool point_seeds = p->n_points > 0 && adj->g_point_re && adj->g_point_im;
+  if (point_seeds && (!p->point_component || !p->point_linear || p->nf_obs <= 0 || !p->obs_e_cos))
+    return fail(MW_ERR_ARGUMENT, "point observer arrays missing");
+  if (p->n_points > 0 && (!adj->g_point_re != !adj->g_point_im))
+    return fail(MW_ERR_ARGUMENT, "points: give both seed gradients or neither");
+  bool dft_seeds = false;
+  for (int c = 0; c < 3 && p->nf_dft > 0; ++c) {
+    if (!adj->g_dft_re[c] != !adj->g_dft_im[c]) return fail(MW_ERR_ARGUMENT, "DFT %d: give both seed gradients or neither", c);
+    dft_seeds = dft_seeds || adj->g_dft_re[c];
+  }
+  hipStream_t s = (hipStream_t)stream;
+  const mw::FieldSet fs = field_set(g, &adj->fields);
+  auto seeds = [&](int n, int family) {
+    const int64_t off = (int64_t)n * p->nf_obs;
+    if (planes.count > 0) {
+      hipLaunchKernelGGL(mw::seed_planes_kernel, dim3(grid_blocks(plane_cells), planes.count), dim3(256), 0, s, planes,
+                         fs, p->nf_obs, p->obs_e_cos + off, p->obs_e_sin + off, p->obs_h_cos + off, p->obs_h_sin + off,
+                         family);
+      if (int r = check_launch("seed_planes")) return r;
+    }
+    if (point_seeds) {
+      hipLaunchKernelGGL(mw::seed_points_kernel, dim3((p->n_points + 63) / 64), dim3(64), 0, s, fs, p->n_points,
+                         p->point_component, p->point_linear, p->nf_obs, p->obs_e_cos + off, p->obs_e_sin + off,
+                         p->obs_h_cos + off, p->obs_h_sin + off, adj->g_point_re, adj->g_point_im, family);
+      if (int r = check_launch("seed_points")) return r;
+    }
+    if (family == 0 && dft_seeds) {
+      for (int c = 0; c < 3; ++c) {
+        if (!adj->g_dft_re[c]) continue;
+        int shp[3];
+        component_shape(g, c, shp);
+        const int64_t cnt = (int64_t)shp[0] * shp[1] * shp[2];
+        hipLaunchKernelGGL(mw::seed_dense_kernel, dim3(grid_blocks(cnt)), dim3(256), 0, s, adj->fields.e[c], cnt,
+                           p->nf_dft, p->dft_cos + (int64_t)n * p->nf_dft, p->dft_sin + (int64_t)n * p->nf_dft,
+                           adj->g_dft_re[c], adj->g_dft_im[c]);
+      }
+      if (int r = check_launch("seed_dft")) return r;
+    }
+    return MW_OK;
+  };
+  const dim3 block(64, 4, 1);
+  const dim3 grid((g->nz + 63) / 64, (g->ny + 3) / 4, g->nx);
+  const mw::WindowArgs wa = w ? window_args(w) : mw::WindowArgs{};
+  const int64_t we = w ? window_cells(w, 0) : 0, wh = w ? window_cells(w, 1) : 0;
+  for (int n = first + count - 1; n >= first; --n) {
+    if ((rc = seeds(n, 0))) return rc;
+    if (a.cpml) hipLaunchKernelGGL(mw::reverse_h_kernel<true>, grid, block, 0, s, a);
+    else hipLaunchKernelGGL(mw::reverse_h_kernel<false>, grid, block, 0, s, a);
+    if ((rc = check_launch("reverse_h"))) return rc;
+    if ((rc = seeds(n, 1))) return rc;
+    if (w) {
+      const int64_t si = n - first;
+      hipLaunchKernelGGL(mw::window_gradient_kernel, dim3(grid_blocks(we), 3), dim3(256), 0, s, wa, *g, adj->fields,
+                         w->e_store + si * 3 * we, w->h_store + si * 3 * wh, w->g_decay[0], w->g_decay[1],
+                         w->g_decay[2], w->g_curl[0], w->g_curl[1], w->g_curl[2]);
+      if ((rc = check_launch("window_gradient"))) return rc;
+    }
+    if (a.cpml) {
+      if (a.ade) hipLaunchKernelGGL((mw::reverse_e_kernel<true, true>), grid, block, 0, s, a);
+      else hipLaunchKernelGGL((mw::reverse_e_kernel<true, false>), grid, block, 0, s, a);
+    } else {
+      if (a.ade) hipLaunchKernelGGL((mw::reverse_e_kernel<false, true>), grid, block, 0, s, a);
+      else hipLaunchKernelGGL((mw::reverse_e_kernel<false, false>), grid, block, 0, s, a);
+    }
+    if ((rc = check_launch("reverse_e"))) return rc;
   }
   return MW_OK;
 }

@@ -169,19 +169,26 @@ def node_permittivity(prepared, model: NodeMaterialModel, region_densities=None)
     return eps
 
 
+def electric_coefficients(eps_rel_edge, sigma_edge, dt: float):
+    """Semi-implicit lossy E coefficients (materials.py:942-970): with
+    h = sigma dt / (2 eps), decay = (1-h)/(1+h) and curl = (dt/eps)/(1+h); sigma = 0
+    gives exactly (1, dt/eps).  Works on numpy arrays and on float64 torch
+    tensors (the adjoint chains coefficient gradients through it)."""
+    if isinstance(eps_rel_edge, torch.Tensor):
+        sigma_edge = torch.as_tensor(sigma_edge, dtype=eps_rel_edge.dtype, device=eps_rel_edge.device)
+    eps = eps_rel_edge * EPSILON_0
+    half = 0.5 * sigma_edge * dt / eps
+    return (1.0 - half) / (1.0 + half), (dt / eps) / (1.0 + half)
+
+
 def update_coefficients(prepared, model: NodeMaterialModel, eps_node: np.ndarray, dt: float):
     """Per-component (decay, curl) in float64 (materials.py:942-1008, :1577-1594)."""
     coef = {}
     eps_edges = {}
     for comp in E_COMPONENTS:
-        eps = average_node_to_component(eps_node, comp) * EPSILON_0
-        sig = average_node_to_component(model.sigma_e, comp)
-        eps_edges[comp] = eps
-        if np.any(model.sigma_e != 0):
-            half = 0.5 * sig * dt / eps
-            coef[comp] = ((1.0 - half) / (1.0 + half), (dt / eps) / (1.0 + half))
-        else:
-            coef[comp] = (np.ones_like(eps), dt / eps)
+        eps_rel = average_node_to_component(eps_node, comp)
+        eps_edges[comp] = eps_rel * EPSILON_0
+        coef[comp] = electric_coefficients(eps_rel, average_node_to_component(model.sigma_e, comp), dt)
     for comp in H_COMPONENTS:
         mu = average_node_to_magnetic_component(model.mu_r, comp) * MU_0
         coef[comp] = (np.ones_like(mu), dt / mu)

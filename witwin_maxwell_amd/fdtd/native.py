@@ -87,11 +87,23 @@ class MwProgram(ctypes.Structure):
                 ("h_terms_dev", vp), ("e_terms_dev", vp)]
 
 
-STRUCTS = (MwGrid, MwCoeffs, MwCpml, MwFields, MwPole, MwAde, MwTerm, MwPlaneObs, MwProgram)
+class MwWindow(ctypes.Structure):
+    _fields_ = [("lo", ctypes.c_int32 * 3), ("hi", ctypes.c_int32 * 3), ("capacity", ctypes.c_int32),
+                ("pad", ctypes.c_int32), ("e_store", vp), ("h_store", vp), ("g_decay", vp * 3), ("g_curl", vp * 3)]
+
+
+class MwAdjoint(ctypes.Structure):
+    _fields_ = [("fields", MwFields), ("psi", vp * 12), ("polarization", (vp * MW_MAX_POLES) * 3),
+                ("current", (vp * MW_MAX_POLES) * 3), ("g_point_re", vp), ("g_point_im", vp),
+                ("g_plane_re", vp * MW_MAX_PLANES), ("g_plane_im", vp * MW_MAX_PLANES),
+                ("g_dft_re", vp * 3), ("g_dft_im", vp * 3)]
+
+
+STRUCTS = (MwGrid, MwCoeffs, MwCpml, MwFields, MwPole, MwAde, MwTerm, MwPlaneObs, MwProgram, MwWindow, MwAdjoint)
 
 EXPORTS = ("mw_update_magnetic", "mw_update_electric", "mw_add_source_patch", "mw_add_time_shifted_patch",
-           "mw_accumulate_dft", "mw_accumulate_plane", "mw_accumulate_points", "mw_run_steps", "mw_last_error",
-           "mw_abi_version", "mw_abi_struct_sizes")
+           "mw_accumulate_dft", "mw_accumulate_plane", "mw_accumulate_points", "mw_run_steps", "mw_capture_steps",
+           "mw_reverse_steps", "mw_last_error", "mw_abi_version", "mw_abi_struct_sizes")
 
 _LIB = None
 
@@ -122,6 +134,8 @@ def load_library(path=None):
     lib.mw_accumulate_plane.argtypes = [vp, i32, i32, i32, i32, i32, i32, vp, vp, vp, vp, vp]
     lib.mw_accumulate_points.argtypes = [vp, vp, i32, i32, vp, vp, vp, vp, vp]
     lib.mw_run_steps.argtypes = [P(MwProgram), i32, i32, vp]
+    lib.mw_capture_steps.argtypes = [P(MwProgram), i32, i32, P(MwWindow), vp]
+    lib.mw_reverse_steps.argtypes = [P(MwProgram), P(MwAdjoint), P(MwWindow), i32, i32, vp]
     lib.mw_last_error.restype = ctypes.c_char_p
     lib.mw_abi_struct_sizes.argtypes = [P(ctypes.c_int64), i32]
     lib.mw_abi_struct_sizes.restype = i32

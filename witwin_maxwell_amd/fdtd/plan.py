@@ -106,6 +106,9 @@ def compile_plan(scene, *, frequencies=(), time_steps=None, dt=None, courant=Non
     bc_lo = tuple(BOUNDARY_CODE[prepared.boundary.face_kind(a, "low")] for a in "xyz")
     bc_hi = tuple(BOUNDARY_CODE[prepared.boundary.face_kind(a, "high")] for a in "xyz")
 
+    for region in scene.material_regions:
+        if tuple(region.mu_bounds) != (1.0, 1.0):
+            raise NotImplementedError("MaterialRegion.mu_bounds other than (1, 1) is outside this FDTD runtime")
     model = _mat.NodeMaterialModel(prepared)
     if node_eps_override is not None:
         eps_node = np.asarray(node_eps_override, dtype=np.float64)

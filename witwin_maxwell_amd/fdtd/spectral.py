@@ -173,11 +173,32 @@ def _exact_widths(prepared, axis, points):
     return widths[idx]
 
 
+def _is_torch(x):
+    return type(x).__module__.startswith("torch")
+
+
+def _complex(re, im):
+    if _is_torch(re):
+        import torch
+        return torch.complex(re.double(), im.double())
+    return re + 1j * im
+
+
+def _scale_like(scale, ref):
+    if _is_torch(ref):
+        import torch
+        return torch.as_tensor(scale, dtype=torch.float64, device=ref.device)
+    return scale
+
+
 def assemble_monitor_results(layout: ObserverLayout, point_re, point_im, plane_re, plane_im, window_sum):
     """Monitor payloads from accumulated sums.
 
     ``point_re/im``: (F, n_points) arrays; ``plane_re/im``: list of (F, a, b)
-    arrays per plane; ``window_sum``: (F,) window normalisation."""
+    arrays per plane; ``window_sum``: (F,) window normalisation.  The sums may be
+    numpy arrays or torch tensors; with tensors every payload (and the flux) is a
+    complex128 / float64 tensor that keeps the autograd graph of the sums, which
+    is how monitor losses reach the adjoint."""
     prepared = layout.prepared
     scale = np.where(window_sum > 0, 2.0 / np.where(window_sum > 0, window_sum, 1.0), 0.0)
     results = {}
@@ -189,12 +210,13 @@ def assemble_monitor_results(layout: ObserverLayout, point_re, point_im, plane_r
         for comp, info in rec["components"].items():
             if "point" in info:
                 p = info["point"]
-                data = (point_re[fidx, p] + 1j * point_im[fidx, p]) * scale[fidx]
+                data = _complex(point_re[fidx, p], point_im[fidx, p]) * _scale_like(scale[fidx], point_re)
                 comps[comp] = data[0] if single else data
             else:
                 acc = None
                 for plane, weight, _ in info["samples"]:
-                    val = (plane_re[plane][fidx] + 1j * plane_im[plane][fidx]) * scale[fidx][:, None, None]
+                    sc = _scale_like(scale[fidx][:, None, None], plane_re[plane])
+                    val = _complex(plane_re[plane][fidx], plane_im[plane][fidx]) * sc
                     acc = weight * val if acc is None else acc + weight * val
                 comps[comp] = {"data": acc[0] if single else acc, "coords": info["coords"]}
         entry = {"kind": "point" if isinstance(m, PointMonitor) else "plane", "fields": m.fields,
@@ -275,7 +297,13 @@ def _crop_to_physical(prepared, axis, aligned):
     ia, ib = masks
     if ia.size == 0 or ib.size == 0:
         raise ValueError("FluxMonitor plane has no samples inside the physical domain.")
-    fields = {c: np.take(np.take(v, ia, axis=-2), ib, axis=-1) for c, v in aligned["fields"].items()}
+    def take(v):
+        if _is_torch(v):
+            import torch
+            return v.index_select(-2, torch.as_tensor(ia, device=v.device)).index_select(
+                -1, torch.as_tensor(ib, device=v.device))
+        return np.take(np.take(v, ia, axis=-2), ib, axis=-1)
+    fields = {c: take(v) for c, v in aligned["fields"].items()}
     return {a_name: np.asarray(aligned[a_name])[ia], b_name: np.asarray(aligned[b_name])[ib], "fields": fields}
 
 
@@ -284,17 +312,22 @@ def plane_flux(entry, single):
     ai = "xyz".index(axis)
     a_name, b_name = _COORD_NAMES[axis]
     weights = entry["cell_widths"][a_name][:, None] * entry["cell_widths"][b_name][None, :]
-    shape = np.asarray(entry[next(c for c in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz") if c in entry)]).shape
-    e = np.zeros(shape + (3,), dtype=np.complex128)
-    h = np.zeros(shape + (3,), dtype=np.complex128)
-    for c in ("Ex", "Ey", "Ez"):
-        if c in entry:
-            e[..., "xyz".index(c[1])] = entry[c]
-    for c in ("Hx", "Hy", "Hz"):
-        if c in entry:
-            h[..., "xyz".index(c[1])] = entry[c]
     direction = 1.0 if entry.get("normal_direction", "+") == "+" else -1.0
-    s = 0.5 * np.real(np.cross(e, np.conj(h), axis=-1)[..., ai]) * direction
+    # normal component of E x conj(H): (E_b H_c* - E_c H_b*) with (ai, b, c) cyclic
+    b, c = "xyz"[(ai + 1) % 3], "xyz"[(ai + 2) % 3]
+    def comp(name):
+        return entry.get(name)
+    def prod(e, h):
+        if e is None or h is None:
+            return 0.0
+        return e * (h.conj() if _is_torch(h) else np.conj(h))
+    cross = prod(comp("E" + b), comp("H" + c)) - prod(comp("E" + c), comp("H" + b))
+    if _is_torch(cross):
+        import torch
+        w = torch.as_tensor(weights, dtype=torch.float64, device=cross.device)
+        flux = (0.5 * direction * cross.real * w).sum(dim=(-2, -1))
+        return flux[0] if single and flux.ndim else flux
+    s = 0.5 * np.real(cross) * direction
     flux = np.sum(s * weights, axis=(-2, -1))
     return float(flux) if single else flux
 
