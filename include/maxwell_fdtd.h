@@ -155,6 +155,7 @@ int mw_accumulate_points(const float* field, const int32_t* linear_index, int32_
 
 #define MW_TERM_ADDITIVE 0 /* value = signal_table[step] * patch            */
 #define MW_TERM_SHIFTED 1  /* value = patch * s(step*dt + t_offset - delay) */
+#define MW_TERM_LINE 2     /* value = patch * line[line_base + line_step * p_axis] (TFSF face) */
 
 typedef struct mw_term {
   int32_t kind;
@@ -170,7 +171,27 @@ typedef struct mw_term {
   float* values;              /* SHIFTED, single-pass mode: device buffer of
                                  patch-size floats the step fills with
                                  patch * s(t - delay) before the fused kernel */
+  const float* line;          /* LINE: the auxiliary line's E (H-side terms) or H
+                                 (E-side terms); p_axis is the patch-local
+                                 coordinate along line_axis (line_axis < 0: one
+                                 sample for the whole patch) */
+  int32_t line_axis, line_base, line_step, line_len;
 } mw_term;
+
+/* TFSF auxiliary 1-D line (reference fdtd/excitation/spatial.py:311-469,
+ * kernels/sources.cu:803-839): advanced once per step, its H before the field
+ * update and its E (hard source e[0] = source[step], e[n_e-1] = 0) after it. */
+typedef struct mw_aux_line {
+  int32_t n_e; /* E samples; 0 = no line */
+  int32_t pad;
+  float* e;    /* n_e */
+  float* h;    /* n_e - 1 */
+  const float* decay_e;
+  const float* curl_e;
+  const float* decay_h;
+  const float* curl_h;
+  const float* source; /* total_steps values */
+} mw_aux_line;
 
 typedef struct mw_plane_obs {
   int32_t component, axis, index, pad;
@@ -218,6 +239,7 @@ typedef struct mw_program {
   float* psi_h_alt[6];
   const mw_term* h_terms_dev;
   const mw_term* e_terms_dev;
+  mw_aux_line aux;
 } mw_program;
 
 /* Advance `count` steps starting at step index `first` (t = n*dt).  Replaces the

@@ -153,6 +153,12 @@ class OracleRun:
         self.e_terms = [(t, arr(t["patch"]), np.ascontiguousarray(t["delay"], dtype=np.float64)) for t in plan.e_terms]
         self.h_terms = [(t, arr(t["patch"]), np.ascontiguousarray(t["delay"], dtype=np.float64)) for t in plan.h_terms]
 
+        # TFSF auxiliary line (float64 restatement, witwin_maxwell_amd.fdtd.tfsf)
+        self.aux = plan.aux
+        if self.aux is not None:
+            self.aux_e = np.zeros(self.aux["n_e"])
+            self.aux_h = np.zeros(self.aux["n_e"] - 1)
+
         self.obs = plan.observers
         if self.obs is not None:
             nf = len(self.obs.frequencies)
@@ -184,6 +190,23 @@ class OracleRun:
                                     *t["offset"], time, st["kind_code"], st["frequency"], st["fwidth"],
                                     st["amplitude"], st["phase"], st["delay"])
 
+    def _line_terms(self, terms, line):
+        """field[patch] += patch * line[base + step * p_axis] (TFSF faces)."""
+        for t in terms:
+            f = self.fields[t["field"]]
+            patch = t["patch"]
+            o = t["offset"]
+            box = tuple(slice(o[a], o[a] + patch.shape[a]) for a in range(3))
+            if t["line_axis"] < 0:
+                samples = line[t["line_base"]]
+            else:
+                ax = t["line_axis"]
+                idx = t["line_base"] + t["line_step"] * np.arange(patch.shape[ax])
+                shape = [1, 1, 1]
+                shape[ax] = patch.shape[ax]
+                samples = line[idx].reshape(shape)
+            f[box] += (patch * samples).astype(f.dtype)
+
     def step(self):
         plan, lib, real = self.plan, self.lib, self.lib._real
         n = self.step_index
@@ -191,12 +214,20 @@ class OracleRun:
         e_before = {c: self.fields[c].copy() for c in _COMP[:3]} if self.trajectory is not None else None
         lib.or_update_h(ctypes.byref(self.grid), self._eptr, self._hptr, self._psiptr)
         self._shifted(self.h_terms, t)
+        if self.aux is not None:
+            from witwin_maxwell_amd.fdtd.tfsf import advance_line_h
+            self._line_terms(plan.line_h_terms, self.aux_e)
+            advance_line_h(self.aux, self.aux_e, self.aux_h)
         for c in ("Ex", "Ey", "Ez"):
             e = self.fields[c]
             for p in self.ade[c]["poles"]:
                 lib.or_ade_advance(p["kind"], e.size, _ptr(e, real), _ptr(p["drive"], real), p["decay"],
                                    p["restoring"], plan.dt, _ptr(p["pol"], real), _ptr(p["cur"], real))
         lib.or_update_e(ctypes.byref(self.grid), self._eptr, self._hptr, self._psiptr)
+        if self.aux is not None:
+            from witwin_maxwell_amd.fdtd.tfsf import advance_line_e
+            self._line_terms(plan.line_e_terms, self.aux_h)
+            advance_line_e(self.aux, self.aux_e, self.aux_h, n)
         self._shifted(self.e_terms, t + 0.5 * plan.dt)
         for ti, (comp, off, patch) in enumerate(self.point_terms):
             f = self.fields[comp]

@@ -190,3 +190,28 @@ def test_lean_interior_kernel(case):
     assert plan.shape[2] >= 63 + 2 * 8 + 4
     _check_fields(ref, dev)
     _check_monitors(ref, dev)
+
+
+@pytest.mark.parametrize("mode,fused", [("slab", True), ("slab", False), ("box", True), ("box", False)])
+def test_tfsf_plane_wave(mode, fused):
+    """TFSF faces driven by the auxiliary line: slab (dispersive spheres, CPML
+    on every face) and box (-y incidence into a dielectric scene)."""
+    print(f"case: TFSF {mode}, fused={fused}", flush=True)
+    if mode == "slab":
+        scene, kw = configs.dispersive_spheres(cells=(20, 22, 70), pml=6, fill=0.25, radius=(3, 6), seed=4,
+                                               injection="tfsf", pulse_delay=3e-15)
+        steps = 1400  # ~0.07 cells/step at the resolved-plasma dt: the pulse crosses the monitor by ~1100
+    else:
+        dx = configs.DX
+        scene = mw.Scene(domain=mw.Domain(((-10 * dx, 10 * dx), (-16 * dx, 16 * dx), (-9 * dx, 9 * dx))),
+                         grid=mw.GridSpec.uniform(dx), boundary=mw.BoundarySpec.pml(6), device="cuda")
+        scene.add_structure(mw.Structure(mw.Box((0, 0, 0), (6 * dx, 5 * dx, 4 * dx)), mw.Material(eps_r=4.0)))
+        scene.add_source(mw.PlaneWave((0, -1, 0), (0, 0, 1), source_time=mw.GaussianPulse(configs.F0, configs.F0 / 4),
+                                      injection=mw.TFSF(((-7 * dx, 7 * dx), (-11 * dx, 11 * dx), (-6 * dx, 6 * dx)))))
+        scene.add_monitor(mw.FluxMonitor("sf_flux", axis="y", position=-13 * dx, frequencies=(configs.F0,)))
+        kw = {"frequencies": (configs.F0,), "courant": 0.95}
+        steps = 400
+    plan, ref, dev = _run_both(scene, kw, steps, fused)
+    assert plan.aux is not None and (plan.line_e_terms and plan.line_h_terms)
+    _check_fields(ref, dev)
+    _check_monitors(ref, dev)

@@ -10,7 +10,7 @@ from .geometry import Box, Sphere, Structure
 from .media import DebyePole, DrudePole, LorentzPole, Material
 from .monitors import FluxMonitor, PlaneMonitor, PointMonitor
 from .scene import BoundarySpec, Domain, GridSpec, MaterialRegion, Scene, SceneModule, prepare_scene
-from .sources import CW, TFSF, GaussianPulse, PlaneWave, PointDipole
+from .sources import CW, TFSF, GaussianPulse, PlaneWave, PointDipole, RickerWavelet
 
 __all__ = [
     "C_0", "EPSILON_0", "ETA_0", "MU_0",
@@ -18,7 +18,7 @@ __all__ = [
     "Material", "DebyePole", "DrudePole", "LorentzPole",
     "PointMonitor", "PlaneMonitor", "FluxMonitor",
     "Scene", "SceneModule", "MaterialRegion", "Domain", "GridSpec", "BoundarySpec", "prepare_scene",
-    "GaussianPulse", "CW", "PointDipole", "PlaneWave", "TFSF",
+    "GaussianPulse", "CW", "RickerWavelet", "PointDipole", "PlaneWave", "TFSF",
 ]
 
 

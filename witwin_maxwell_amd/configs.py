@@ -67,10 +67,17 @@ def dispersive_spheres(cells=(384, 384, 256), pml=12, fill=0.30, steps=4000, see
     rng = np.random.default_rng(seed)
     scene = Scene(domain=_domain(cells), grid=GridSpec.uniform(DX), boundary=BoundarySpec.pml(pml), device=device)
     half = np.array(cells) * 0.5 * DX
+    # a TFSF slab needs vacuum on its two faces (4 cells inside the physical z
+    # span): every structure stays within |z| <= half_z - 6 cells
+    zlim = half[2] - 6 * DX if injection == "tfsf" else np.inf
     # background: eps in [1,4] slabs of random boxes
     for _ in range(12):
         size = rng.integers(8, 64, size=3) * DX
         centre = rng.uniform(-half, half)
+        z0, z1 = max(centre[2] - 0.5 * size[2], -zlim), min(centre[2] + 0.5 * size[2], zlim)
+        if z1 <= z0:
+            continue
+        centre[2], size[2] = 0.5 * (z0 + z1), z1 - z0
         scene.add_structure(Structure(Box(tuple(centre), tuple(size)), Material(eps_r=float(rng.uniform(1, 4)))))
     two_pi = 2.0 * np.pi
     metal = Material(eps_r=2.0, drude_poles=(DrudePole(1.2e16 / two_pi, 1e14 / two_pi),),
@@ -81,9 +88,11 @@ def dispersive_spheres(cells=(384, 384, 256), pml=12, fill=0.30, steps=4000, see
     while filled < fill * volume:
         r = rng.uniform(radius[0], radius[1])
         centre = rng.uniform(-half, half)
+        centre[2] = float(np.clip(centre[2], -zlim + r * DX, zlim - r * DX)) if np.isfinite(zlim) else centre[2]
         scene.add_structure(Structure(Sphere(tuple(centre), r * DX), metal))
         filled += 4.0 / 3.0 * np.pi * r ** 3 * 0.85  # overlaps and clipping lose some volume
-    inj = TFSF.slab("z") if injection == "tfsf" else "soft"
+    # slab faces 4 cells inside the physical z span
+    inj = TFSF.slab(axis="z", bounds=(-half[2] + 4 * DX, half[2] - 4 * DX)) if injection == "tfsf" else "soft"
     scene.add_source(PlaneWave((0, 0, 1), (1, 0, 0), source_time=GaussianPulse(F0, F0 / 4, delay=pulse_delay),
                                injection=inj))
     freqs = tuple(np.linspace(0.7 * F0, 1.3 * F0, 16))

@@ -78,7 +78,14 @@ __device__ __forceinline__ float term_value(const mw_term* __restrict__ terms, i
     const int pi = i - d.offset[0], pj = j - d.offset[1], pk = k - d.offset[2];
     if (pi < 0 || pj < 0 || pk < 0 || pi >= d.shape[0] || pj >= d.shape[1] || pk >= d.shape[2]) continue;
     const int64_t p = ((int64_t)pi * d.shape[1] + pj) * d.shape[2] + pk;
-    acc += d.kind == MW_TERM_ADDITIVE ? d.signal_table[step] * d.patch[p] : d.values[p];
+    if (d.kind == MW_TERM_ADDITIVE) {
+      acc += d.signal_table[step] * d.patch[p];
+    } else if (d.kind == MW_TERM_LINE) {
+      const int pa = d.line_axis == 0 ? pi : (d.line_axis == 1 ? pj : pk);
+      acc += d.patch[p] * d.line[d.line_axis < 0 ? d.line_base : d.line_base + d.line_step * pa];
+    } else {
+      acc += d.values[p];
+    }
   }
   return acc;
 }

@@ -176,7 +176,16 @@ int check_terms(const mw_grid* g, const mw_term* t, int n, const char* which, in
     if (!x.patch) return fail(MW_ERR_ARGUMENT, "%s[%d]: patch is NULL", which, i);
     if (x.kind == MW_TERM_ADDITIVE && !x.signal_table) return fail(MW_ERR_ARGUMENT, "%s[%d]: signal table is NULL", which, i);
     if (x.kind == MW_TERM_SHIFTED && !x.delay) return fail(MW_ERR_ARGUMENT, "%s[%d]: delay is NULL", which, i);
-    if (x.kind != MW_TERM_ADDITIVE && x.kind != MW_TERM_SHIFTED) return fail(MW_ERR_ARGUMENT, "%s[%d]: bad kind", which, i);
+    if (x.kind == MW_TERM_LINE) {
+      if (!x.line || x.line_len <= 0) return fail(MW_ERR_ARGUMENT, "%s[%d]: line is NULL", which, i);
+      if (x.line_axis > 2) return fail(MW_ERR_ARGUMENT, "%s[%d]: bad line axis %d", which, i, x.line_axis);
+      const int last = x.line_axis < 0 ? x.line_base : x.line_base + x.line_step * (x.shape[x.line_axis] - 1);
+      if (x.line_base < 0 || x.line_base >= x.line_len || last < 0 || last >= x.line_len)
+        return fail(MW_ERR_ARGUMENT, "%s[%d]: line samples [%d, %d] outside the line of %d", which, i, x.line_base,
+                    last, x.line_len);
+    }
+    if (x.kind != MW_TERM_ADDITIVE && x.kind != MW_TERM_SHIFTED && x.kind != MW_TERM_LINE)
+      return fail(MW_ERR_ARGUMENT, "%s[%d]: bad kind", which, i);
   }
   (void)total_steps;
   return MW_OK;
@@ -537,9 +546,17 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
         if (t[i].kind == MW_TERM_SHIFTED && !t[i].values) return fail(MW_ERR_ARGUMENT, "term %d has no value buffer", i);
     }
   }
+  const mw_aux_line& aux = p->aux;
+  if (aux.n_e < 0 || aux.n_e == 1 ||
+      (aux.n_e > 0 && (!aux.e || !aux.h || !aux.decay_e || !aux.curl_e || !aux.decay_h || !aux.curl_h || !aux.source)))
+    return fail(MW_ERR_ARGUMENT, "auxiliary line arrays missing");
   hipStream_t s = (hipStream_t)stream;
   const mw_ade* ade = p->has_ade ? &p->ade : nullptr;
   for (int n = first; n < first + count; ++n) {
+    if (aux.n_e > 0) {
+      hipLaunchKernelGGL(mw::aux_line_h_kernel, dim3((aux.n_e + 254) / 256), dim3(256), 0, s, aux);
+      if ((rc = check_launch("aux_line_h"))) return rc;
+    }
     mw_fields cur = p->fields;
     if (p->fused) {
       if ((rc = launch_fused(p, n, s, &cur))) return rc;
@@ -548,6 +565,10 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
       if ((rc = launch_terms(g, &p->fields, p->h_terms, p->n_h_terms, n, p->dt, s))) return rc;
       if ((rc = launch_e(g, &p->coeffs, &p->cpml, ade, &p->fields, 7, s))) return rc;
       if ((rc = launch_terms(g, &p->fields, p->e_terms, p->n_e_terms, n, p->dt, s))) return rc;
+    }
+    if (aux.n_e > 0) {
+      hipLaunchKernelGGL(mw::aux_line_e_kernel, dim3((aux.n_e + 255) / 256), dim3(256), 0, s, aux, n);
+      if ((rc = check_launch("aux_line_e"))) return rc;
     }
     const mw::FieldSet fs = field_set(g, &cur);
     if (p->nf_dft > 0 && !p->fused) {

@@ -52,6 +52,9 @@ __global__ __launch_bounds__(256) void apply_terms_kernel(TermSet set, FieldSet 
   float value;
   if (t.kind == MW_TERM_ADDITIVE) {
     value = t.signal_table[step] * t.patch[p];
+  } else if (t.kind == MW_TERM_LINE) {
+    const int pa = t.line_axis == 0 ? px : (t.line_axis == 1 ? py : pz);
+    value = t.patch[p] * t.line[t.line_axis < 0 ? t.line_base : t.line_base + t.line_step * pa];
   } else {
     const double time = (double)step * dt + t.t_offset - t.delay[p];
     value = (float)(waveform(t.wave_kind, time, t.frequency, t.fwidth, t.amplitude, t.phase, t.delay0) *
@@ -99,6 +102,22 @@ __global__ __launch_bounds__(256) void add_shifted_kernel(float* field, int fy, 
   const int px = (int)(p / ((int64_t)pnz * pny));
   const double s = waveform(kind, time - delay[p], freq, fwidth, amp, phase, delay0);
   field[((uint32_t)(oi + px) * fy + (oj + py)) * fz + (ok + pz)] += (float)(s * (double)patch[p]);
+}
+
+// ------------------------------------------------------------ TFSF line
+// One workgroup-free pass each (the line holds a few hundred samples).
+__global__ __launch_bounds__(256) void aux_line_h_kernel(mw_aux_line a) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= a.n_e - 1) return;
+  a.h[m] = a.decay_h[m] * a.h[m] - a.curl_h[m] * (a.e[m + 1] - a.e[m]);
+}
+
+__global__ __launch_bounds__(256) void aux_line_e_kernel(mw_aux_line a, int step) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= a.n_e) return;
+  if (m == a.n_e - 1) { a.e[m] = 0.0f; return; }
+  if (m == 0) { a.e[0] = a.source[step]; return; }
+  a.e[m] = a.decay_e[m] * a.e[m] - a.curl_e[m] * (a.h[m] - a.h[m - 1]);
 }
 
 // ------------------------------------------------------------ spectral sums

@@ -51,7 +51,8 @@ def snapshot(run: DeviceRun) -> dict:
     return {"fields": {c: t.clone() for c, t in run.fields.items()},
             "psi": {name: run.psi_current(name).clone() for name in run.psi},
             "ade": {c: [{k: (v.clone() if v is not None else None) for k, v in s.items()} for s in states]
-                    for c, states in run.ade_state.items()}}
+                    for c, states in run.ade_state.items()},
+            "aux": None if run.aux_e is None else (run.aux_e.clone(), run.aux_h.clone())}
 
 
 def restore(run: DeviceRun, snap: dict | None, step: int) -> None:
@@ -68,6 +69,13 @@ def restore(run: DeviceRun, snap: dict | None, step: int) -> None:
             for k, v in s.items():
                 if v is not None:
                     v.copy_(snap["ade"][c][si][k]) if snap is not None else v.zero_()
+    if run.aux_e is not None:
+        if snap is not None:
+            run.aux_e.copy_(snap["aux"][0])
+            run.aux_h.copy_(snap["aux"][1])
+        else:
+            run.aux_e.zero_()
+            run.aux_h.zero_()
 
 
 # ------------------------------------------------------------------ window
@@ -99,7 +107,7 @@ def check_window(plan: FDTDPlan, lo, hi):
         for pole in plan.ade[c]["poles"]:
             if np.any(np.asarray(pole["drive"])[box] != 0):
                 raise NotImplementedError("gradients through dispersive poles inside a design window")
-    for t in plan.point_terms + plan.e_terms + plan.h_terms:
+    for t in plan.point_terms + plan.e_terms + plan.h_terms + plan.line_e_terms + plan.line_h_terms:
         off, shp = t["offset"], t["patch"].shape
         if all(off[a] < hi[a] + 1 and off[a] + shp[a] > lo[a] - 1 for a in range(3)):
             raise NotImplementedError("a source patch overlaps the design window (source patches are held fixed)")

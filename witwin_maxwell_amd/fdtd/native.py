@@ -65,7 +65,13 @@ class MwTerm(ctypes.Structure):
                 ("wave_kind", ctypes.c_int32), ("pad", ctypes.c_int32),
                 ("t_offset", ctypes.c_double), ("frequency", ctypes.c_double), ("fwidth", ctypes.c_double),
                 ("amplitude", ctypes.c_double), ("phase", ctypes.c_double), ("delay0", ctypes.c_double),
-                ("values", vp)]
+                ("values", vp), ("line", vp), ("line_axis", ctypes.c_int32), ("line_base", ctypes.c_int32),
+                ("line_step", ctypes.c_int32), ("line_len", ctypes.c_int32)]
+
+
+class MwAuxLine(ctypes.Structure):
+    _fields_ = [("n_e", ctypes.c_int32), ("pad", ctypes.c_int32), ("e", vp), ("h", vp), ("decay_e", vp),
+                ("curl_e", vp), ("decay_h", vp), ("curl_h", vp), ("source", vp)]
 
 
 class MwPlaneObs(ctypes.Structure):
@@ -84,7 +90,7 @@ class MwProgram(ctypes.Structure):
                 ("point_linear", vp), ("point_re", vp), ("point_im", vp), ("nf_dft", ctypes.c_int32),
                 ("pad3", ctypes.c_int32), ("dft_cos", vp), ("dft_sin", vp), ("dft_re", vp * 3), ("dft_im", vp * 3),
                 ("fused", ctypes.c_int32), ("pad4", ctypes.c_int32), ("fields_alt", MwFields), ("psi_h_alt", vp * 6),
-                ("h_terms_dev", vp), ("e_terms_dev", vp)]
+                ("h_terms_dev", vp), ("e_terms_dev", vp), ("aux", MwAuxLine)]
 
 
 class MwWindow(ctypes.Structure):

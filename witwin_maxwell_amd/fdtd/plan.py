@@ -17,7 +17,7 @@ import numpy as np
 
 from ..constants import C_0
 from ..scene import PreparedScene, prepare_scene
-from ..sources import PlaneWave, PointDipole
+from ..sources import TFSF, PlaneWave, PointDipole, cw_signal, gaussian_pulse_signal, ricker_wavelet_signal
 from . import cpml as _cpml
 from . import excitation as _exc
 from . import materials as _mat
@@ -25,6 +25,16 @@ from . import spectral as _spec
 
 COMPONENTS = ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz")
 BOUNDARY_CODE = {"none": 0, "pml": 1, "periodic": 2, "pec": 4, "pmc": 5}
+
+
+def signal_value(st, t):
+    """s(t) of a compiled source time (sources.py:497-521)."""
+    if st["kind"] == "cw":
+        return cw_signal(t, frequency=st["frequency"], amplitude=st["amplitude"], phase=st["phase"])
+    if st["kind"] == "ricker_wavelet":
+        return ricker_wavelet_signal(t, frequency=st["frequency"], amplitude=st["amplitude"], delay=st["delay"])
+    return gaussian_pulse_signal(t, frequency=st["frequency"], fwidth=st["fwidth"], amplitude=st["amplitude"],
+                                 phase=st["phase"], delay=st["delay"])
 
 
 def auto_dt(dx, dy, dz, frequency, steps_per_cycle=30):
@@ -62,6 +72,9 @@ class FDTDPlan:
     dft_window_sum: np.ndarray | None
     node_eps: np.ndarray
     material_model: object = None
+    line_e_terms: list = field(default_factory=list)   # TFSF E faces sampling the auxiliary line's H
+    line_h_terms: list = field(default_factory=list)   # TFSF H faces sampling the auxiliary line's E
+    aux: dict | None = None                            # TFSF auxiliary 1-D line
     meta: dict = field(default_factory=dict)
 
     def field_shape(self, comp):
@@ -127,28 +140,25 @@ def compile_plan(scene, *, frequencies=(), time_steps=None, dt=None, courant=Non
             layout[name] = _cpml.psi_layout(prepared, name, _mat.component_shape(comp, shape))
 
     point_terms, e_terms, h_terms = [], [], []
+    line_e, line_h, aux = [], [], None
     for src, st in zip(sources, sts):
         if isinstance(src, PointDipole):
             point_terms.extend(_exc.point_dipole_terms(prepared, src, eps_edges, dt, st))
         elif isinstance(src, PlaneWave):
-            if src.injection != "soft":
-                from .tfsf import tfsf_terms
-                et, ht = tfsf_terms(prepared, src, coef, dt, st)
+            if isinstance(src.injection, TFSF):
+                if aux is not None:
+                    raise ValueError("FDTD currently supports at most one TFSF source per scene.")
+                from .tfsf import tfsf_setup
+                values = np.array([signal_value(st, n * dt) for n in range(steps)])
+                line_e, line_h, aux = tfsf_setup(prepared, src, coef, dt, st, values, eps_node, model)
             else:
                 et, ht = _exc.soft_plane_wave_terms(prepared, src, coef, dt, st)
-            e_terms.extend(et)
-            h_terms.extend(ht)
-    from ..sources import gaussian_pulse_signal, cw_signal
+                e_terms.extend(et)
+                h_terms.extend(ht)
     signals = np.zeros((len(point_terms), steps))
     for ti, term in enumerate(point_terms):
-        st = term["st"]
         for n in range(steps):
-            t = n * dt
-            if st["kind"] == "cw":
-                signals[ti, n] = cw_signal(t, frequency=st["frequency"], amplitude=st["amplitude"], phase=st["phase"])
-            else:
-                signals[ti, n] = gaussian_pulse_signal(t, frequency=st["frequency"], fwidth=st["fwidth"],
-                                                       amplitude=st["amplitude"], phase=st["phase"], delay=st["delay"])
+            signals[ti, n] = signal_value(term["st"], n * dt)
 
     win = "none" if pulsed else str(window)
     observers = None
@@ -176,4 +186,5 @@ def compile_plan(scene, *, frequencies=(), time_steps=None, dt=None, courant=Non
                     h_terms=h_terms, signals=signals, observers=observers, obs_weights=obs_weights,
                     obs_window_sum=obs_sum, dft_frequencies=dft_freqs, dft_weights=dft_weights,
                     dft_window_sum=dft_sum, node_eps=eps_node, material_model=model,
+                    line_e_terms=line_e, line_h_terms=line_h, aux=aux,
                     meta={"window": win, "pulsed": pulsed})

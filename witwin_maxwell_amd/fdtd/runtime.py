@@ -141,9 +141,30 @@ class DeviceRun:
         prog.dt = plan.dt
         prog.total_steps = plan.time_steps
 
+        # ---- TFSF auxiliary line
+        self.aux_e = self.aux_h = None
+        if plan.aux is not None:
+            aux = plan.aux
+            n_e = int(aux["n_e"])
+            self.aux_e = torch.zeros(n_e, dtype=torch.float32, device=dev)
+            self.aux_h = torch.zeros(n_e - 1, dtype=torch.float32, device=dev)
+            line = prog.aux
+            line.n_e = n_e
+            line.e = nat.ptr(self.aux_e)
+            line.h = nat.ptr(self.aux_h)
+            line.decay_e = self._dev(aux["decay_e"])
+            line.curl_e = self._dev(aux["curl_e"])
+            line.decay_h = self._dev(aux["decay_h"])
+            line.curl_h = self._dev(aux["curl_h"])
+            line.source = self._dev(aux["source"])
+
         # ---- source terms
         cidx = {c: i for i, c in enumerate(COMPONENTS)}
         e_terms, h_terms = [], []
+        for t in plan.line_h_terms:
+            h_terms.append(self._term_line(cidx[t["field"]], t, self.aux_e))
+        for t in plan.line_e_terms:
+            e_terms.append(self._term_line(cidx[t["field"]], t, self.aux_h))
         for ti, t in enumerate(plan.point_terms):
             e_terms.append(self._term_additive(cidx[t["field"]], t, plan.signals[ti]))
         for t in plan.e_terms:
@@ -259,6 +280,16 @@ class DeviceRun:
         term.signal_table = self._dev(signal)
         return term
 
+    def _term_line(self, comp, t, line):
+        term = self._term_common(comp, t)
+        term.kind = 2
+        term.line = nat.ptr(line)
+        term.line_axis = int(t["line_axis"])
+        term.line_base = int(t["line_base"])
+        term.line_step = int(t["line_step"])
+        term.line_len = int(line.numel())
+        return term
+
     def _term_shifted(self, comp, t, t_offset):
         term = self._term_common(comp, t)
         term.kind = 1
@@ -300,6 +331,9 @@ class DeviceRun:
                     s["polarization"].zero_()
         for t in self.plane_re + self.plane_im + list(self.dft_re.values()) + list(self.dft_im.values()):
             t.zero_()
+        if self.aux_e is not None:
+            self.aux_e.zero_()
+            self.aux_h.zero_()
         if self.point_re is not None:
             self.point_re.zero_()
             self.point_im.zero_()

@@ -96,6 +96,45 @@ class CW:
         return self.frequency
 
 
+def ricker_wavelet_signal(t, *, frequency, amplitude, delay):
+    """A (1 - 2 a^2) exp(-a^2), a = pi f (t - delay) (waveforms.py:56-61)."""
+    a = math.pi * frequency * (float(t) - float(delay))
+    return amplitude * (1.0 - 2.0 * a * a) * math.exp(-a * a)
+
+
+@dataclass(frozen=True)
+class RickerWavelet:
+    """Mexican-hat wavelet; default delay 4/(pi f), characteristic frequency 3f
+    (sources.py:254-283)."""
+    frequency: float
+    amplitude: float = 1.0
+    delay: float | None = None
+    kind: str = "ricker_wavelet"
+
+    def __post_init__(self):
+        f = float(self.frequency)
+        if f <= 0:
+            raise ValueError("RickerWavelet frequency must be > 0")
+        object.__setattr__(self, "frequency", f)
+        object.__setattr__(self, "amplitude", float(self.amplitude))
+        object.__setattr__(self, "delay", 4.0 / (math.pi * f) if self.delay is None else float(self.delay))
+
+    @property
+    def phase(self):
+        return 0.0
+
+    @property
+    def fwidth(self):
+        return 0.0
+
+    def evaluate(self, t: float) -> float:
+        return ricker_wavelet_signal(t, frequency=self.frequency, amplitude=self.amplitude, delay=self.delay)
+
+    @property
+    def characteristic_frequency(self) -> float:
+        return 3.0 * self.frequency
+
+
 def source_time_dict(st) -> dict:
     """Compiled source-time record, the fields the device waveform needs."""
     return {"kind": st.kind, "kind_code": KIND_CODE[st.kind], "frequency": float(st.frequency),
@@ -147,17 +186,47 @@ class PointDipole:
         object.__setattr__(self, "kind", "point_dipole")
 
 
-@dataclass(frozen=True)
 class TFSF:
-    """Total-field/scattered-field slab injection: the incident wave is confined to
-    the region between ``bounds`` along ``axis`` (fdtd/excitation/tfsf_*.py)."""
+    """Total-field/scattered-field injection region (sources.py:388-429).
 
-    axis: str = "z"
-    bounds: tuple | None = None
+    ``TFSF(bounds)`` is a box ((x0, x1), (y0, y1), (z0, z1)); ``TFSF.slab(axis=...,
+    bounds=(s0, s1))`` keeps only the two faces normal to ``axis`` and spans the
+    whole non-PML transverse extent."""
+
+    kind = "tfsf"
+
+    def __init__(self, bounds):
+        if len(bounds) != 3:
+            raise ValueError("bounds must contain x, y, z ranges.")
+        norm = []
+        for ab in bounds:
+            if len(ab) != 2:
+                raise ValueError("Each TFSF axis bound must contain exactly two values.")
+            a, b = float(ab[0]), float(ab[1])
+            if b <= a:
+                raise ValueError("Each TFSF axis bound must have end > start.")
+            norm.append((a, b))
+        self.bounds = tuple(norm)
+        self.mode = "box"
+        self.axis = None
+        self.axis_bounds = None
 
     @classmethod
-    def slab(cls, axis="z", bounds=None):
-        return cls(axis=str(axis).lower(), bounds=None if bounds is None else (float(bounds[0]), float(bounds[1])))
+    def slab(cls, *, axis, bounds):
+        axis = str(axis).lower()
+        if axis not in ("x", "y", "z"):
+            raise ValueError("TFSF slab axis must be 'x', 'y', or 'z'.")
+        if len(bounds) != 2:
+            raise ValueError("TFSF slab bounds must contain exactly two values.")
+        a, b = float(bounds[0]), float(bounds[1])
+        if b <= a:
+            raise ValueError("TFSF slab bounds must have end > start.")
+        src = cls.__new__(cls)
+        src.bounds = None
+        src.mode = "slab"
+        src.axis = axis
+        src.axis_bounds = (a, b)
+        return src
 
 
 def _normalize(v):
