@@ -299,6 +299,23 @@ int mw_capture_steps(const mw_program* program, int32_t first, int32_t count, co
 int mw_reverse_steps(const mw_program* program, const mw_adjoint* adjoint, const mw_window* window, int32_t first,
                      int32_t count, void* stream);
 
+/* ---- scene batches ----
+ * Many independent scenes (uneven sizes, single-pass programs) advance in lockstep:
+ * every launch of a step covers the tiles of all scenes, and the step index lives
+ * in a device counter, so a run of `graph_steps` steps is captured once as a
+ * hipGraph and replayed.  The handle owns its small device tables, a private
+ * stream (default priority) and the graph; the scenes' buffers stay the caller's
+ * and must outlive the handle.  `xchunk` > 0 forces the x-planes per tile
+ * (0 = automatic).  No counterpart in the reference, which runs one scene per
+ * solver (its multi-scene ensemble, execution/ensemble.py:166-180, threads whole
+ * solvers across GPUs). */
+typedef struct mw_batch mw_batch;
+int mw_batch_create(const mw_program* const* programs, int32_t count, int32_t xchunk, mw_batch** out);
+/* Advance steps [first, first+steps) of every scene; graph_steps <= 1 launches
+ * directly.  Ordered after prior work on `stream` and before later work on it. */
+int mw_batch_run(mw_batch* batch, int32_t first, int32_t steps, int32_t graph_steps, void* stream);
+int mw_batch_destroy(mw_batch* batch);
+
 const char* mw_last_error(void);
 int32_t mw_abi_version(void);
 /* sizeof of mw_grid, mw_coeffs, mw_cpml, mw_fields, mw_pole, mw_ade, mw_term,

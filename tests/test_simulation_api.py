@@ -1,0 +1,75 @@
+"""Front-end semantics of Simulation.fdtd / Result that need no GPU
+(simulation.py:49-63, :122-200, fdtd/solver.py:108-131, result.py:1593-1745)."""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import witwin_maxwell_amd as mw
+from witwin_maxwell_amd.constants import C_0
+from witwin_maxwell_amd.simulation import Result, _normalize_frequencies, calculate_required_steps
+
+
+def test_frequency_normalisation_and_errors():
+    assert _normalize_frequencies(frequency=2e14) == (2e14,)
+    assert _normalize_frequencies(frequencies=[1e14, 2e14]) == (1e14, 2e14)
+    assert _normalize_frequencies(frequencies=3e14) == (3e14,)
+    with pytest.raises(ValueError):
+        _normalize_frequencies(frequency=1e14, frequencies=(1e14,))
+    with pytest.raises(ValueError):
+        _normalize_frequencies()
+    with pytest.raises(ValueError):
+        _normalize_frequencies(frequencies=())
+
+
+def test_config_validation():
+    with pytest.raises(TypeError):
+        mw.FDTDConfig(run_time=5)
+    with pytest.raises(ValueError):
+        mw.SpectralSampler(window="kaiser")
+    with pytest.raises(ValueError):
+        mw.FDTDConfig(shutoff=-1.0)
+    assert mw.SpectralSampler(window="HANN").window == "hanning"
+    t = mw.TimeConfig.auto(steady_cycles=7, transient_cycles=3)
+    assert t.time_steps is None and t.steady_cycles == 7
+
+
+def test_required_steps_formula():
+    f, dt = 2e14, 1e-17
+    period = 1 / f
+    # transient = max(15 periods, 5 round trips, settling), + 20 periods
+    got = calculate_required_steps(f, dt, num_cycles=20, transient_cycles=3, domain_size=1e-6,
+                                   source_time={"settling_time": 2e-13})
+    expect = int((max(15 * period, 2 * 1e-6 / C_0 * 5, 2e-13) + 20 * period) / dt)
+    assert got == expect
+    assert calculate_required_steps(f, dt) == int((15 * period + 20 * period) / dt)
+
+
+def test_gaussian_settling_time():
+    from witwin_maxwell_amd.sources import source_time_dict
+    st = mw.GaussianPulse(2e14, 5e13)
+    sigma = 1 / (2 * math.pi * 5e13)
+    d = source_time_dict(st)
+    assert math.isclose(d["settling_time"], d["delay"] + 6 * sigma)
+    assert source_time_dict(mw.CW(2e14))["settling_time"] == 0.0
+    r = mw.RickerWavelet(2e14)
+    assert math.isclose(source_time_dict(r)["settling_time"], r.delay + 4 / 2e14)
+
+
+def test_result_accessors():
+    freqs = (1e14, 2e14)
+    ex = torch.arange(2 * 3 * 4 * 5, dtype=torch.float32).reshape(2, 3, 4, 5)
+    mon = {"kind": "point", "frequency": 1e14, "frequencies": freqs, "components": {"Ez": torch.tensor([1 + 1j, 2 + 0j])},
+           "Ez": torch.tensor([1 + 1j, 2 + 0j]), "data": torch.tensor([1 + 1j, 2 + 0j])}
+    res = Result(method="fdtd", scene=None, prepared_scene=None, frequency=freqs[0], frequencies=freqs,
+                 fields={"Ex": ex}, monitors={"p": mon}, solver_stats={})
+    assert torch.equal(res.tensor("ex", freq_index=1), ex[1])
+    assert torch.equal(res.field("E", frequency=2e14)["EX"], ex[1])
+    assert res.monitor("p", freq_index=1)["data"] == 2 + 0j
+    assert res.monitor("p", frequency=1e14)["frequencies"] == (1e14,)
+    with pytest.raises(KeyError):
+        res.monitor("nope")
+    with pytest.raises(ValueError):
+        res.tensor("Ex", frequency=1.5e14)

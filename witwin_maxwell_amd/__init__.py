@@ -25,10 +25,8 @@ __all__ = [
 def __getattr__(name):
     # Simulation/Result pull in the native runtime; import them lazily so that the
     # scene API stays usable for compile-only work.
-    if name in ("Simulation", "FDTDConfig", "TimeConfig", "SpectralSampler"):
+    if name in ("Simulation", "FDTDConfig", "TimeConfig", "SpectralSampler", "Result", "run",
+                "calculate_required_steps"):
         from . import simulation
         return getattr(simulation, name)
-    if name == "Result":
-        from .result import Result
-        return Result
     raise AttributeError(name)

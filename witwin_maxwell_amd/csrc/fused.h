@@ -114,7 +114,7 @@ struct ERow {
 template <bool CPML>
 __device__ __forceinline__ void h_update(const FusedArgs& a, int i, int r, int k, bool lane63, const ERow& e_r,
                                          const ERow& e_r1, float ey_n, float ez_n, bool store, bool terms, float& hx,
-                                         float& hy, float& hz) {
+                                         float& hy, float& hz, int step) {
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   // shuffles first, while every lane of the wavefront is active
   float ey_k1 = lane_down(e_r.ey), ex_k1 = lane_down(e_r.ex);
@@ -141,7 +141,7 @@ __device__ __forceinline__ void h_update(const FusedArgs& a, int i, int r, int k
       curl = da - db;
     }
     hx = a.in.h[0][o] * coef(a.cf.decay[3], a.cf.decay_value[3], o) - coef(a.cf.curl[3], a.cf.curl_value[3], o) * curl;
-    if (terms) hx += term_value(a.h_terms, a.n_h_terms, 3, i, r, k, a.step, a.dt);
+    if (terms) hx += term_value(a.h_terms, a.n_h_terms, 3, i, r, k, step, a.dt);
     if (store) a.out.h[0][o] = hx;
   }
   if (iv) {
@@ -163,7 +163,7 @@ __device__ __forceinline__ void h_update(const FusedArgs& a, int i, int r, int k
         curl = db - da;
       }
       hy = a.in.h[1][o] * coef(a.cf.decay[4], a.cf.decay_value[4], o) - coef(a.cf.curl[4], a.cf.curl_value[4], o) * curl;
-      if (terms) hy += term_value(a.h_terms, a.n_h_terms, 4, i, r, k, a.step, a.dt);
+      if (terms) hy += term_value(a.h_terms, a.n_h_terms, 4, i, r, k, step, a.dt);
       if (store) a.out.h[1][o] = hy;
     }
     if (jv) {
@@ -183,7 +183,7 @@ __device__ __forceinline__ void h_update(const FusedArgs& a, int i, int r, int k
         curl = da - db;
       }
       hz = a.in.h[2][o] * coef(a.cf.decay[5], a.cf.decay_value[5], o) - coef(a.cf.curl[5], a.cf.curl_value[5], o) * curl;
-      if (terms) hz += term_value(a.h_terms, a.n_h_terms, 5, i, r, k, a.step, a.dt);
+      if (terms) hz += term_value(a.h_terms, a.n_h_terms, 5, i, r, k, step, a.dt);
       if (store) a.out.h[2][o] = hz;
     }
   }
@@ -212,7 +212,8 @@ __device__ __forceinline__ float lean_h(const FusedArgs& a, int c, uint32_t o, f
 template <int COEF, bool ADE, bool DFT>
 __device__ __forceinline__ void lean_plane(const FusedArgs& a, int i, int j, int k, bool store, const ERow& em,
                                            const ERow& ec, const ERow& ep, const ERow& nm, const ERow& nc,
-                                           float& hy_prev, float& hz_prev, bool h_terms, bool e_terms) {
+                                           float& hy_prev, float& hz_prev, bool h_terms, bool e_terms, int step,
+                                           const float* __restrict__ wc_t, const float* __restrict__ ws_t) {
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   const uint32_t sHx = (uint32_t)(ny - 1) * (nz - 1), sHy = (uint32_t)ny * (nz - 1), sHz = (uint32_t)(ny - 1) * nz;
   const uint32_t sEx = (uint32_t)ny * nz, sEy = (uint32_t)(ny - 1) * nz, sEz = (uint32_t)ny * (nz - 1);
@@ -237,11 +238,11 @@ __device__ __forceinline__ void lean_plane(const FusedArgs& a, int i, int j, int
   float hy_c = lean_h<COEF>(a, 4, oyc, a.in.h[1][oyc], (exc_k1 - ec.ex) * ihz - (nc.ez - ec.ez) * ihx);
   float hz_c = lean_h<COEF>(a, 5, ozc, a.in.h[2][ozc], (nc.ey - ec.ey) * ihx - (ep.ex - ec.ex) * ihy_c);
   if (h_terms) {  // wave-uniform: this plane/row range carries H source terms
-    hx_m += term_value(a.h_terms, a.n_h_terms, 3, i, j - 1, k, a.step, a.dt);
-    hz_m += term_value(a.h_terms, a.n_h_terms, 5, i, j - 1, k, a.step, a.dt);
-    hx_c += term_value(a.h_terms, a.n_h_terms, 3, i, j, k, a.step, a.dt);
-    hy_c += term_value(a.h_terms, a.n_h_terms, 4, i, j, k, a.step, a.dt);
-    hz_c += term_value(a.h_terms, a.n_h_terms, 5, i, j, k, a.step, a.dt);
+    hx_m += term_value(a.h_terms, a.n_h_terms, 3, i, j - 1, k, step, a.dt);
+    hz_m += term_value(a.h_terms, a.n_h_terms, 5, i, j - 1, k, step, a.dt);
+    hx_c += term_value(a.h_terms, a.n_h_terms, 3, i, j, k, step, a.dt);
+    hy_c += term_value(a.h_terms, a.n_h_terms, 4, i, j, k, step, a.dt);
+    hz_c += term_value(a.h_terms, a.n_h_terms, 5, i, j, k, step, a.dt);
   }
   if (store) {
     a.out.h[0][oxc] = hx_c;
@@ -264,9 +265,9 @@ __device__ __forceinline__ void lean_plane(const FusedArgs& a, int i, int j, int
       ez -= a.ad.dt * ade_advance(a.ad, 2, oz, ec.ez) * (a.ad.count[2] ? a.ad.inv_eps[2][oz] : 0.0f);
     }
     if (e_terms) {
-      ex += term_value(a.e_terms, a.n_e_terms, 0, i, j, k, a.step, a.dt);
-      ey += term_value(a.e_terms, a.n_e_terms, 1, i, j, k, a.step, a.dt);
-      ez += term_value(a.e_terms, a.n_e_terms, 2, i, j, k, a.step, a.dt);
+      ex += term_value(a.e_terms, a.n_e_terms, 0, i, j, k, step, a.dt);
+      ey += term_value(a.e_terms, a.n_e_terms, 1, i, j, k, step, a.dt);
+      ez += term_value(a.e_terms, a.n_e_terms, 2, i, j, k, step, a.dt);
     }
     a.out.e[0][ox] = ex;
     a.out.e[1][oy] = ey;
@@ -274,7 +275,7 @@ __device__ __forceinline__ void lean_plane(const FusedArgs& a, int i, int j, int
     if (DFT) {
       const uint64_t nxs = (uint64_t)sEx * (nx - 1), nys = (uint64_t)sEy * nx, nzs = (uint64_t)sEz * nx;
       for (int f = 0; f < a.nf_dft; ++f) {
-        const float wc = a.dft_cos[f], ws = a.dft_sin[f];
+        const float wc = wc_t[f], ws = ws_t[f];
         a.dft_re[0][f * nxs + ox] += ex * wc;
         a.dft_im[0][f * nxs + ox] += ex * ws;
         a.dft_re[1][f * nys + oy] += ey * wc;
@@ -307,24 +308,23 @@ __device__ __forceinline__ float pmc_edge(bool low_end, int mode, float own, flo
   return low_end ? 2.0f * own * inv : -2.0f * prev * inv;
 }
 
+// XCD-aware tile order: the hardware hands block w to XCD w%8; (w%8, w/8) ->
+// logical is a bijection on the padded launch (a multiple of 8 blocks), so each
+// XCD walks a contiguous run of tiles (z fastest, then y, then x-chunk).
+__device__ __forceinline__ int xcd_logical_block() {
+  const int w = blockIdx.x;
+  const int per_xcd = (int)(gridDim.x / 8);
+  return (w % 8) * per_xcd + (w / 8);
+}
+
+// One tile (tz, ty, tx) of the general kernel for step `step` with that step's
+// DFT weights wc_t / ws_t.
 template <bool CPML, bool ADE, bool DFT>
-__global__ __launch_bounds__(256) void fused_step_kernel(FusedArgs a) {
+__device__ __forceinline__ void fused_tile(const FusedArgs& a, int step, const float* __restrict__ wc_t,
+                                           const float* __restrict__ ws_t, int tz, int ty, int tx) {
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   const int lane = threadIdx.x;  // 0..63
   const int wv = threadIdx.y;    // 0..3
-
-  // XCD-aware tile order: the hardware hands block w to XCD w%8; (w%8, w/8) ->
-  // logical is a bijection on the padded launch (a multiple of 8 blocks), so each
-  // XCD walks a contiguous run of tiles (z fastest, then y, then x-chunk).
-  const int nblocks = a.tiles_z * a.tiles_y * a.tiles_x;
-  const int w = blockIdx.x;
-  const int per_xcd = (int)(gridDim.x / 8);
-  const int logical = (w % 8) * per_xcd + (w / 8);
-  if (logical >= nblocks) return;
-  const int tz = logical % a.tiles_z;
-  const int ty = (logical / a.tiles_z) % a.tiles_y;
-  const int tx = logical / (a.tiles_z * a.tiles_y);
-  if (tz >= a.lz0 && tz < a.lz1 && ty >= a.ly0 && ty < a.ly1 && tx >= a.lx0 && tx < a.lx1) return;
 
   const int kc0 = a.z_off + tz * kFOwnCols - 1;  // column of lane 0
   const int k = kc0 + lane;
@@ -373,8 +373,8 @@ __global__ __launch_bounds__(256) void fused_step_kernel(FusedArgs a) {
     const bool ht = terms_touch(a.h_terms, a.n_h_terms, i, j - 1, j, kc0, kc0 + kFLanes - 1);
     const bool et = terms_touch(a.e_terms, a.n_e_terms, i, j, j, kc0, kc0 + kFLanes - 1);
     float hx_m, hy_m, hz_m, hx_c, hy_c, hz_c;
-    h_update<CPML>(a, i, j - 1, k, lane63, em, ec, nm.ey, nm.ez, false, ht, hx_m, hy_m, hz_m);
-    h_update<CPML>(a, i, j, k, lane63, ec, ep, nc.ey, nc.ez, owned && i >= i0, ht, hx_c, hy_c, hz_c);
+    h_update<CPML>(a, i, j - 1, k, lane63, em, ec, nm.ey, nm.ez, false, ht, hx_m, hy_m, hz_m, step);
+    h_update<CPML>(a, i, j, k, lane63, ec, ep, nc.ey, nc.ez, owned && i >= i0, ht, hx_c, hy_c, hz_c, step);
     (void)hy_m;
     const float hy_km = lane_up(hy_c);
     const float hx_km = lane_up(hx_c);
@@ -404,14 +404,14 @@ __global__ __launch_bounds__(256) void fused_step_kernel(FusedArgs a) {
           e_new = e_old * coef(a.cf.decay[0], a.cf.decay_value[0], o) + coef(a.cf.curl[0], a.cf.curl_value[0], o) * curl;
         }
         if (ADE) e_new -= a.ad.dt * jsum * a.ad.inv_eps[0][o];
-        if (et) e_new += term_value(a.e_terms, a.n_e_terms, 0, i, j, k, a.step, a.dt);
+        if (et) e_new += term_value(a.e_terms, a.n_e_terms, 0, i, j, k, step, a.dt);
         if (pec) e_new = 0.0f;
         a.out.e[0][o] = e_new;
         if (DFT) {
           const uint64_t n = (uint64_t)sEx * (nx - 1);
           for (int f = 0; f < a.nf_dft; ++f) {
-            a.dft_re[0][f * n + o] += e_new * a.dft_cos[f];
-            a.dft_im[0][f * n + o] += e_new * a.dft_sin[f];
+            a.dft_re[0][f * n + o] += e_new * wc_t[f];
+            a.dft_im[0][f * n + o] += e_new * ws_t[f];
           }
         }
       }
@@ -436,14 +436,14 @@ __global__ __launch_bounds__(256) void fused_step_kernel(FusedArgs a) {
           e_new = e_old * coef(a.cf.decay[1], a.cf.decay_value[1], o) + coef(a.cf.curl[1], a.cf.curl_value[1], o) * curl;
         }
         if (ADE) e_new -= a.ad.dt * jsum * a.ad.inv_eps[1][o];
-        if (et) e_new += term_value(a.e_terms, a.n_e_terms, 1, i, j, k, a.step, a.dt);
+        if (et) e_new += term_value(a.e_terms, a.n_e_terms, 1, i, j, k, step, a.dt);
         if (pec) e_new = 0.0f;
         a.out.e[1][o] = e_new;
         if (DFT) {
           const uint64_t n = (uint64_t)sEy * nx;
           for (int f = 0; f < a.nf_dft; ++f) {
-            a.dft_re[1][f * n + o] += e_new * a.dft_cos[f];
-            a.dft_im[1][f * n + o] += e_new * a.dft_sin[f];
+            a.dft_re[1][f * n + o] += e_new * wc_t[f];
+            a.dft_im[1][f * n + o] += e_new * ws_t[f];
           }
         }
       }
@@ -468,14 +468,14 @@ __global__ __launch_bounds__(256) void fused_step_kernel(FusedArgs a) {
           e_new = e_old * coef(a.cf.decay[2], a.cf.decay_value[2], o) + coef(a.cf.curl[2], a.cf.curl_value[2], o) * curl;
         }
         if (ADE) e_new -= a.ad.dt * jsum * a.ad.inv_eps[2][o];
-        if (et) e_new += term_value(a.e_terms, a.n_e_terms, 2, i, j, k, a.step, a.dt);
+        if (et) e_new += term_value(a.e_terms, a.n_e_terms, 2, i, j, k, step, a.dt);
         if (pec) e_new = 0.0f;
         a.out.e[2][o] = e_new;
         if (DFT) {
           const uint64_t n = (uint64_t)sEz * nx;
           for (int f = 0; f < a.nf_dft; ++f) {
-            a.dft_re[2][f * n + o] += e_new * a.dft_cos[f];
-            a.dft_im[2][f * n + o] += e_new * a.dft_sin[f];
+            a.dft_re[2][f * n + o] += e_new * wc_t[f];
+            a.dft_im[2][f * n + o] += e_new * ws_t[f];
           }
         }
       }
@@ -494,19 +494,11 @@ __global__ __launch_bounds__(256) void fused_step_kernel(FusedArgs a) {
 // needed and the register footprint stays small (high occupancy hides HBM
 // latency).  Source terms are applied under wave-uniform guards.
 template <bool ADE, bool DFT, int COEF>
-__global__ __launch_bounds__(256) void lean_step_kernel(FusedArgs a) {
+__device__ __forceinline__ void lean_tile(const FusedArgs& a, int step, const float* __restrict__ wc_t,
+                                          const float* __restrict__ ws_t, int tz, int ty, int tx) {
   const int ny = a.g.ny, nz = a.g.nz;
   const int lane = threadIdx.x;
   const int wv = threadIdx.y;
-  const int bz = a.lz1 - a.lz0, by = a.ly1 - a.ly0, bx = a.lx1 - a.lx0;
-  const int nblocks = bz * by * bx;
-  const int w = blockIdx.x;
-  const int per_xcd = (int)(gridDim.x / 8);
-  const int logical = (w % 8) * per_xcd + (w / 8);
-  if (logical >= nblocks) return;
-  const int tz = a.lz0 + logical % bz;
-  const int ty = a.ly0 + (logical / bz) % by;
-  const int tx = a.lx0 + logical / (bz * by);
   const int kc0 = a.z_off + tz * kFOwnCols - 1;  // column of lane 0
   const int k = kc0 + lane;
   const int j = a.y_off + ty * kFRows + wv;
@@ -539,11 +531,87 @@ __global__ __launch_bounds__(256) void lean_step_kernel(FusedArgs a) {
     const ERow nm = load3(i + 1, -1, true), nc = load3(i + 1, 0, true), np = load3(i + 1, 1, false);
     const bool ht = terms_touch(a.h_terms, a.n_h_terms, i, j - 1, j, kc0, kc0 + kFLanes - 1);
     const bool et = terms_touch(a.e_terms, a.n_e_terms, i, j, j, kc0, kc0 + kFLanes - 1);
-    lean_plane<COEF, ADE, DFT>(a, i, j, k, lane > 0 && i >= i0, em, ec, ep, nm, nc, hy_prev, hz_prev, ht, et);
+    lean_plane<COEF, ADE, DFT>(a, i, j, k, lane > 0 && i >= i0, em, ec, ep, nm, nc, hy_prev, hz_prev, ht, et, step,
+                               wc_t, ws_t);
     em = nm;
     ec = nc;
     ep = np;
   }
+}
+
+template <bool CPML, bool ADE, bool DFT>
+__global__ __launch_bounds__(256) void fused_step_kernel(FusedArgs a) {
+  const int logical = xcd_logical_block();
+  if (logical >= a.tiles_z * a.tiles_y * a.tiles_x) return;
+  const int tz = logical % a.tiles_z;
+  const int ty = (logical / a.tiles_z) % a.tiles_y;
+  const int tx = logical / (a.tiles_z * a.tiles_y);
+  if (tz >= a.lz0 && tz < a.lz1 && ty >= a.ly0 && ty < a.ly1 && tx >= a.lx0 && tx < a.lx1) return;
+  fused_tile<CPML, ADE, DFT>(a, a.step, a.dft_cos, a.dft_sin, tz, ty, tx);
+}
+
+template <bool ADE, bool DFT, int COEF>
+__global__ __launch_bounds__(256) void lean_step_kernel(FusedArgs a) {
+  const int bz = a.lz1 - a.lz0, by = a.ly1 - a.ly0, bx = a.lx1 - a.lx0;
+  const int logical = xcd_logical_block();
+  if (logical >= bz * by * bx) return;
+  lean_tile<ADE, DFT, COEF>(a, a.step, a.dft_cos, a.dft_sin, a.lz0 + logical % bz, a.ly0 + (logical / bz) % by,
+                            a.lx0 + logical / (bz * by));
+}
+
+// ---------------------------------------------------------------- batches
+// Many independent scenes advance in one launch: args[2 s + p] holds scene s's
+// arguments for steps of parity p (buffers swapped), with dft_cos/dft_sin the
+// base of its (steps, nf) weight tables; prefix[s] is the first logical tile of
+// scene s (general tiles in the general launch, lean-box tiles in the lean one).
+// The step index is read from a device counter so a captured graph of steps
+// replays without host arguments.
+struct BatchView {
+  const FusedArgs* args;
+  const int* prefix;  // nscenes + 1
+  int nscenes;
+  int total;
+  const int* counter;
+};
+
+__device__ __forceinline__ int batch_scene(const BatchView& b, int logical) {
+  int lo = 0, hi = b.nscenes - 1;  // last s with prefix[s] <= logical
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (b.prefix[mid] <= logical) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <bool CPML, bool ADE, bool DFT>
+__global__ __launch_bounds__(256) void fused_batch_kernel(BatchView b) {
+  const int logical = xcd_logical_block();
+  if (logical >= b.total) return;
+  const int s = batch_scene(b, logical);
+  const int n = *b.counter;
+  const FusedArgs& a = b.args[2 * s + (n & 1)];
+  const int t = logical - b.prefix[s];
+  const int tz = t % a.tiles_z;
+  const int ty = (t / a.tiles_z) % a.tiles_y;
+  const int tx = t / (a.tiles_z * a.tiles_y);
+  if (tz >= a.lz0 && tz < a.lz1 && ty >= a.ly0 && ty < a.ly1 && tx >= a.lx0 && tx < a.lx1) return;
+  const int64_t w = (int64_t)n * a.nf_dft;
+  fused_tile<CPML, ADE, DFT>(a, n, a.dft_cos + w, a.dft_sin + w, tz, ty, tx);
+}
+
+template <bool ADE, bool DFT, int COEF>
+__global__ __launch_bounds__(256) void lean_batch_kernel(BatchView b) {
+  const int logical = xcd_logical_block();
+  if (logical >= b.total) return;
+  const int s = batch_scene(b, logical);
+  const int n = *b.counter;
+  const FusedArgs& a = b.args[2 * s + (n & 1)];
+  const int t = logical - b.prefix[s];
+  const int bz = a.lz1 - a.lz0, by = a.ly1 - a.ly0;
+  const int64_t w = (int64_t)n * a.nf_dft;
+  lean_tile<ADE, DFT, COEF>(a, n, a.dft_cos + w, a.dft_sin + w, a.lz0 + t % bz, a.ly0 + (t / bz) % by,
+                            a.lx0 + t / (bz * by));
 }
 
 }  // namespace mw

@@ -6,9 +6,11 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "maxwell_fdtd.h"
 #include "adjoint.h"
+#include "batch.h"
 #include "fused.h"
 #include "terms.h"
 #include "yee.h"
@@ -265,11 +267,11 @@ int launch_shifted_values(const mw_term* t, int n, int step, double dt, hipStrea
   return check_launch("shifted_values");
 }
 
-int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_fields) {
-  int rc;
-  if ((rc = launch_shifted_values(p->h_terms, p->n_h_terms, n, p->dt, s))) return rc;
-  if ((rc = launch_shifted_values(p->e_terms, p->n_e_terms, n, p->dt, s))) return rc;
-  mw::FusedArgs a;
+// Arguments of the single-pass step for step n (buffers by parity, this step's
+// DFT weights) and the host tiling: lean box aligned with the PML interfaces,
+// x-chunk chosen near `target` planes (chunk_override > 0 forces it).
+int build_fused_args(const mw_program* p, int n, int chunk_override, mw::FusedArgs* out, int* nlean_out) {
+  mw::FusedArgs& a = *out;
   memset(&a, 0, sizeof(a));
   a.g = p->grid;
   a.cf = p->coeffs;
@@ -315,7 +317,9 @@ int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_field
   // the chunk near the target that leaves the fewest interior planes to the
   // general kernel (they are the ones the lean box cannot cover)
   int chunk = target;
-  {
+  if (chunk_override > 0) {
+    chunk = chunk_override;
+  } else {
     const int span = ihi - (ilo + 1);
     int best = span >= 0 ? span % target : 0;
     for (int c = std::max(8, target - 8); c <= std::min(32, target + 8) && span >= 8; ++c) {
@@ -336,6 +340,18 @@ int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_field
   a.lx0 = tx.lean0; a.lx1 = tx.lean1;
   int nlean = (a.lz1 - a.lz0) * (a.ly1 - a.ly0) * (a.lx1 - a.lx0);
   if (nlean <= 0) { nlean = 0; a.lz0 = a.lz1 = a.ly0 = a.ly1 = a.lx0 = a.lx1 = 0; }
+  *nlean_out = nlean;
+  return MW_OK;
+}
+
+int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_fields) {
+  int rc;
+  if ((rc = launch_shifted_values(p->h_terms, p->n_h_terms, n, p->dt, s))) return rc;
+  if ((rc = launch_shifted_values(p->e_terms, p->n_e_terms, n, p->dt, s))) return rc;
+  mw::FusedArgs a;
+  int nlean = 0;
+  build_fused_args(p, n, 0, &a, &nlean);
+  const bool ade = p->has_ade && (p->ade.count[0] + p->ade.count[1] + p->ade.count[2]) > 0;
   const int nblocks = a.tiles_z * a.tiles_y * a.tiles_x;
   dim3 block(mw::kFLanes, mw::kFRows, 1);
   const bool cpml = p->cpml.enabled != 0, dft = p->nf_dft > 0;
@@ -397,6 +413,123 @@ int check_window(const mw_program* p, const mw_window* w) {
   }
   if (w->capacity <= 0 || !w->e_store || !w->h_store) return fail(MW_ERR_ARGUMENT, "window stores missing");
   return MW_OK;
+}
+
+template <bool CPML, bool ADE, bool DFT>
+void launch_batch_general(dim3 grid, hipStream_t s, const mw::BatchView& v) {
+  hipLaunchKernelGGL((mw::fused_batch_kernel<CPML, ADE, DFT>), grid, dim3(mw::kFLanes, mw::kFRows, 1), 0, s, v);
+}
+
+template <bool ADE, bool DFT>
+void launch_batch_lean(dim3 grid, hipStream_t s, const mw::BatchView& v, int mode) {
+  const dim3 block(mw::kFLanes, mw::kFRows, 1);
+  if (mode == 2) hipLaunchKernelGGL((mw::lean_batch_kernel<ADE, DFT, 2>), grid, block, 0, s, v);
+  else if (mode == 1) hipLaunchKernelGGL((mw::lean_batch_kernel<ADE, DFT, 1>), grid, block, 0, s, v);
+  else hipLaunchKernelGGL((mw::lean_batch_kernel<ADE, DFT, 0>), grid, block, 0, s, v);
+}
+
+}  // namespace
+
+struct mw_batch {
+  int count = 0;
+  int total_steps = 0;
+  bool cpml = false, ade = false, dft = false;
+  int mode = 0;
+  mw::FusedArgs* d_args = nullptr;
+  int* d_prefix = nullptr;
+  int* d_lean_prefix = nullptr;
+  int total = 0, total_lean = 0;
+  int* d_counter = nullptr;
+  mw::BatchTerm* d_terms = nullptr;
+  int n_terms = 0;
+  int64_t max_term_cells = 0;
+  mw_aux_line* d_aux = nullptr;
+  int n_aux = 0, max_aux = 0;
+  mw::BatchPlane* d_planes = nullptr;
+  int n_planes = 0;
+  int64_t max_plane_cells = 0;
+  mw::BatchPoint* d_points = nullptr;
+  int n_points = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_steps = 0;
+};
+
+namespace {
+
+void batch_free(mw_batch* b) {
+  if (!b) return;
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  void* bufs[] = {b->d_args, b->d_prefix, b->d_lean_prefix, b->d_counter, b->d_terms, b->d_aux, b->d_planes,
+                  b->d_points};
+  for (void* q : bufs)
+    if (q) (void)hipFree(q);
+  if (b->ev_in) (void)hipEventDestroy(b->ev_in);
+  if (b->ev_out) (void)hipEventDestroy(b->ev_out);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+template <typename T>
+int upload(const std::vector<T>& host, T** dst, const char* what) {
+  if (host.empty()) { *dst = nullptr; return MW_OK; }
+  if (hipMalloc((void**)dst, host.size() * sizeof(T)) != hipSuccess)
+    return fail(MW_ERR_RUNTIME, "batch: cannot allocate %s", what);
+  if (hipMemcpy(*dst, host.data(), host.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(MW_ERR_RUNTIME, "batch: cannot upload %s", what);
+  return MW_OK;
+}
+
+// One step of every scene, reading the step index from the device counter.
+int batch_step(const mw_batch* b, hipStream_t s) {
+  int rc;
+  if (b->n_aux > 0) {
+    hipLaunchKernelGGL(mw::aux_line_h_batch_kernel, dim3((b->max_aux + 255) / 256, b->n_aux), dim3(256), 0, s,
+                       b->d_aux, b->n_aux);
+    if ((rc = check_launch("batch aux_line_h"))) return rc;
+  }
+  if (b->n_terms > 0) {
+    hipLaunchKernelGGL(mw::shifted_values_batch_kernel, dim3(grid_blocks(b->max_term_cells), b->n_terms), dim3(256),
+                       0, s, b->d_terms, b->n_terms, b->d_counter);
+    if ((rc = check_launch("batch shifted_values"))) return rc;
+  }
+  mw::BatchView v{b->d_args, b->d_prefix, b->count, b->total, b->d_counter};
+  const dim3 grid(((b->total + 7) / 8) * 8);
+  if (b->cpml) {
+    if (b->ade) { if (b->dft) launch_batch_general<true, true, true>(grid, s, v); else launch_batch_general<true, true, false>(grid, s, v); }
+    else { if (b->dft) launch_batch_general<true, false, true>(grid, s, v); else launch_batch_general<true, false, false>(grid, s, v); }
+  } else {
+    if (b->ade) { if (b->dft) launch_batch_general<false, true, true>(grid, s, v); else launch_batch_general<false, true, false>(grid, s, v); }
+    else { if (b->dft) launch_batch_general<false, false, true>(grid, s, v); else launch_batch_general<false, false, false>(grid, s, v); }
+  }
+  if ((rc = check_launch("batch fused_step (general)"))) return rc;
+  if (b->total_lean > 0) {
+    mw::BatchView lv{b->d_args, b->d_lean_prefix, b->count, b->total_lean, b->d_counter};
+    const dim3 lgrid(((b->total_lean + 7) / 8) * 8);
+    if (b->ade) { if (b->dft) launch_batch_lean<true, true>(lgrid, s, lv, b->mode); else launch_batch_lean<true, false>(lgrid, s, lv, b->mode); }
+    else { if (b->dft) launch_batch_lean<false, true>(lgrid, s, lv, b->mode); else launch_batch_lean<false, false>(lgrid, s, lv, b->mode); }
+    if ((rc = check_launch("batch fused_step (lean)"))) return rc;
+  }
+  if (b->n_aux > 0) {
+    hipLaunchKernelGGL(mw::aux_line_e_batch_kernel, dim3((b->max_aux + 255) / 256, b->n_aux), dim3(256), 0, s,
+                       b->d_aux, b->n_aux, b->d_counter);
+    if ((rc = check_launch("batch aux_line_e"))) return rc;
+  }
+  if (b->n_planes > 0) {
+    hipLaunchKernelGGL(mw::accumulate_planes_batch_kernel, dim3(grid_blocks(b->max_plane_cells), b->n_planes),
+                       dim3(256), 0, s, b->d_planes, b->n_planes, b->d_counter);
+    if ((rc = check_launch("batch accumulate_planes"))) return rc;
+  }
+  if (b->n_points > 0) {
+    hipLaunchKernelGGL(mw::accumulate_points_batch_kernel, dim3((b->n_points + 63) / 64), dim3(64), 0, s,
+                       b->d_points, b->n_points, b->d_counter);
+    if ((rc = check_launch("batch accumulate_points"))) return rc;
+  }
+  hipLaunchKernelGGL(mw::step_counter_kernel, dim3(1), dim3(64), 0, s, b->d_counter, 0, 0);
+  return check_launch("batch step_counter");
 }
 
 }  // namespace
@@ -763,6 +896,191 @@ int mw_reverse_steps(const mw_program* p, const mw_adjoint* adj, const mw_window
     }
     if ((rc = check_launch("reverse_e"))) return rc;
   }
+  return MW_OK;
+}
+
+int mw_batch_create(const mw_program* const* programs, int32_t count, int32_t xchunk, mw_batch** out) {
+  if (!programs || !out || count <= 0) return fail(MW_ERR_ARGUMENT, "batch: need programs, a count > 0 and an output");
+  *out = nullptr;
+  if (xchunk < 0 || xchunk > 256) return fail(MW_ERR_ARGUMENT, "batch: xchunk %d out of range", xchunk);
+  mw_batch* b = new mw_batch();
+  auto bail = [&](int rc) { batch_free(b); return rc; };
+  int rc;
+  std::vector<mw::FusedArgs> args;
+  std::vector<int> prefix(1, 0), lean_prefix(1, 0);
+  std::vector<mw::BatchTerm> terms;
+  std::vector<mw_aux_line> aux;
+  std::vector<mw::BatchPlane> planes;
+  std::vector<mw::BatchPoint> points;
+  b->count = count;
+  for (int si = 0; si < count; ++si) {
+    const mw_program* p = programs[si];
+    if (!p) return bail(fail(MW_ERR_ARGUMENT, "batch: program %d is NULL", si));
+    const mw_grid* g = &p->grid;
+    if ((rc = check_grid(g)) || (rc = check_fields(&p->fields, true, true)) ||
+        (rc = check_fields(&p->fields_alt, true, true)) || (rc = check_cpml(g, &p->cpml, true)) ||
+        (rc = check_cpml(g, &p->cpml, false)) || (rc = check_ade(g, p->has_ade ? &p->ade : nullptr)) ||
+        (rc = check_terms(g, p->h_terms, p->n_h_terms, "h_terms", p->total_steps)) ||
+        (rc = check_terms(g, p->e_terms, p->n_e_terms, "e_terms", p->total_steps)))
+      return bail(rc);
+    if (!p->fused || has_periodic(g)) return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d is not a single-pass program", si));
+    if ((p->n_h_terms > 0 && !p->h_terms_dev) || (p->n_e_terms > 0 && !p->e_terms_dev))
+      return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d has no device term descriptors", si));
+    const bool cpml = p->cpml.enabled != 0;
+    const bool ade = p->has_ade && (p->ade.count[0] + p->ade.count[1] + p->ade.count[2]) > 0;
+    const bool dft = p->nf_dft > 0;
+    const int mode = coef_mode(p->coeffs);
+    if (si == 0) {
+      b->cpml = cpml; b->ade = ade; b->dft = dft; b->mode = mode; b->total_steps = p->total_steps;
+    } else if (cpml != b->cpml || ade != b->ade || dft != b->dft) {
+      return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d differs in CPML/ADE/full-field-DFT use", si));
+    }
+    if (mode != b->mode) b->mode = 0;
+    b->total_steps = std::min(b->total_steps, p->total_steps);
+    for (int par = 0; par < 2; ++par) {
+      mw::FusedArgs a;
+      int nlean = 0;
+      build_fused_args(p, par, xchunk, &a, &nlean);
+      a.dft_cos = p->dft_cos;  // the batch kernels add step * nf
+      a.dft_sin = p->dft_sin;
+      args.push_back(a);
+      if (par == 0) {
+        prefix.push_back(prefix.back() + a.tiles_z * a.tiles_y * a.tiles_x);
+        lean_prefix.push_back(lean_prefix.back() + nlean);
+      }
+    }
+    for (int f = 0; f < 2; ++f) {
+      const mw_term* t = f ? p->e_terms : p->h_terms;
+      for (int i = 0; i < (f ? p->n_e_terms : p->n_h_terms); ++i) {
+        if (t[i].kind != MW_TERM_SHIFTED) continue;
+        if (!t[i].values) return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d term %d has no value buffer", si, i));
+        terms.push_back(mw::BatchTerm{t[i], p->dt});
+        b->max_term_cells = std::max<int64_t>(b->max_term_cells, (int64_t)t[i].shape[0] * t[i].shape[1] * t[i].shape[2]);
+      }
+    }
+    if (p->aux.n_e > 0) {
+      const mw_aux_line& l = p->aux;
+      if (!l.e || !l.h || !l.decay_e || !l.curl_e || !l.decay_h || !l.curl_h || !l.source)
+        return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d auxiliary line arrays missing", si));
+      aux.push_back(l);
+      b->max_aux = std::max(b->max_aux, l.n_e);
+    }
+    if (p->n_planes < 0 || p->n_planes > MW_MAX_PLANES) return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d plane count", si));
+    if ((p->n_planes > 0 || p->n_points > 0) &&
+        (p->nf_obs <= 0 || !p->obs_e_cos || !p->obs_e_sin || !p->obs_h_cos || !p->obs_h_sin))
+      return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d observer weight tables missing", si));
+    auto comp_ptr = [](const mw_fields& f, int c) -> const float* { return c < 3 ? f.e[c] : f.h[c - 3]; };
+    for (int q = 0; q < p->n_planes; ++q) {
+      const mw_plane_obs& o = p->planes[q];
+      if (o.component < 0 || o.component > 5 || o.axis < 0 || o.axis > 2 || !o.re || !o.im)
+        return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d plane %d descriptor invalid", si, q));
+      mw::BatchPlane bp{};
+      bp.component = o.component; bp.axis = o.axis; bp.index = o.index; bp.nf = p->nf_obs;
+      component_shape(g, o.component, bp.shape);
+      if (o.index < 0 || o.index >= bp.shape[o.axis])
+        return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d plane %d index outside axis", si, q));
+      bp.field[0] = comp_ptr(p->fields_alt, o.component);  // state after an even step
+      bp.field[1] = comp_ptr(p->fields, o.component);
+      bp.re = o.re; bp.im = o.im;
+      bp.w_cos = o.component < 3 ? p->obs_e_cos : p->obs_h_cos;
+      bp.w_sin = o.component < 3 ? p->obs_e_sin : p->obs_h_sin;
+      const int* sh = bp.shape;
+      const int64_t cells = o.axis == 0 ? (int64_t)sh[1] * sh[2] : (o.axis == 1 ? (int64_t)sh[0] * sh[2] : (int64_t)sh[0] * sh[1]);
+      b->max_plane_cells = std::max(b->max_plane_cells, cells);
+      planes.push_back(bp);
+    }
+    if (p->n_points > 0) {
+      if (!p->point_component || !p->point_linear || !p->point_re || !p->point_im)
+        return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d point observer arrays missing", si));
+      std::vector<int32_t> comp(p->n_points), lin(p->n_points);
+      if (hipMemcpy(comp.data(), p->point_component, p->n_points * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(lin.data(), p->point_linear, p->n_points * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+        return bail(fail(MW_ERR_RUNTIME, "batch: cannot read scene %d point tables", si));
+      for (int q = 0; q < p->n_points; ++q) {
+        if (comp[q] < 0 || comp[q] > 5) return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d point %d component", si, q));
+        int sh[3];
+        component_shape(g, comp[q], sh);
+        if (lin[q] < 0 || (int64_t)lin[q] >= (int64_t)sh[0] * sh[1] * sh[2])
+          return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d point %d index outside the field", si, q));
+        mw::BatchPoint bq{};
+        bq.linear = lin[q]; bq.nf = p->nf_obs; bq.stride = p->n_points;
+        bq.field[0] = comp_ptr(p->fields_alt, comp[q]);
+        bq.field[1] = comp_ptr(p->fields, comp[q]);
+        bq.re = p->point_re + q; bq.im = p->point_im + q;
+        bq.w_cos = comp[q] < 3 ? p->obs_e_cos : p->obs_h_cos;
+        bq.w_sin = comp[q] < 3 ? p->obs_e_sin : p->obs_h_sin;
+        points.push_back(bq);
+      }
+    }
+  }
+  b->total = prefix.back();
+  b->total_lean = lean_prefix.back();
+  b->n_terms = (int)terms.size();
+  b->n_aux = (int)aux.size();
+  b->n_planes = (int)planes.size();
+  b->n_points = (int)points.size();
+  if (b->n_terms > 65535 || b->n_aux > 65535 || b->n_planes > 65535)
+    return bail(fail(MW_ERR_ARGUMENT, "batch: too many terms, lines or planes for one launch"));
+  if ((rc = upload(args, &b->d_args, "arguments")) || (rc = upload(prefix, &b->d_prefix, "tile table")) ||
+      (rc = upload(lean_prefix, &b->d_lean_prefix, "lean tile table")) || (rc = upload(terms, &b->d_terms, "terms")) ||
+      (rc = upload(aux, &b->d_aux, "auxiliary lines")) || (rc = upload(planes, &b->d_planes, "planes")) ||
+      (rc = upload(points, &b->d_points, "points")))
+    return bail(rc);
+  if (hipMalloc((void**)&b->d_counter, sizeof(int)) != hipSuccess) return bail(fail(MW_ERR_RUNTIME, "batch: counter"));
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) != hipSuccess)
+    return bail(fail(MW_ERR_RUNTIME, "batch: cannot create its stream/events"));
+  *out = b;
+  return MW_OK;
+}
+
+int mw_batch_run(mw_batch* b, int32_t first, int32_t steps, int32_t graph_steps, void* stream) {
+  if (!b) return fail(MW_ERR_ARGUMENT, "batch is NULL");
+  if (first < 0 || steps < 0 || first + steps > b->total_steps)
+    return fail(MW_ERR_ARGUMENT, "batch: steps [%d, %d) outside the %d steps every scene holds", first, first + steps,
+                b->total_steps);
+  if (graph_steps < 0 || graph_steps > 4096) return fail(MW_ERR_ARGUMENT, "batch: graph_steps %d out of range", graph_steps);
+  int rc;
+  hipStream_t caller = (hipStream_t)stream;
+  if (hipEventRecord(b->ev_in, caller) != hipSuccess || hipStreamWaitEvent(b->stream, b->ev_in, 0) != hipSuccess)
+    return fail(MW_ERR_RUNTIME, "batch: cannot order after the caller's stream");
+  hipLaunchKernelGGL(mw::step_counter_kernel, dim3(1), dim3(64), 0, b->stream, b->d_counter, 1, first);
+  if ((rc = check_launch("batch counter"))) return rc;
+  int done = 0;
+  if (graph_steps > 1 && steps >= graph_steps) {
+    if (!b->exec || b->graph_steps != graph_steps) {
+      if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+      if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+      if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeRelaxed) != hipSuccess)
+        return fail(MW_ERR_RUNTIME, "batch: cannot begin graph capture");
+      int crc = MW_OK;
+      for (int i = 0; i < graph_steps && crc == MW_OK; ++i) crc = batch_step(b, b->stream);
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture(b->stream, &graph);
+      if (crc != MW_OK) { if (graph) (void)hipGraphDestroy(graph); return crc; }
+      if (e != hipSuccess || !graph) return fail(MW_ERR_RUNTIME, "batch: graph capture failed: %s", hipGetErrorString(e));
+      if (hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+        (void)hipGraphDestroy(graph);
+        return fail(MW_ERR_RUNTIME, "batch: cannot instantiate the step graph");
+      }
+      b->graph = graph;
+      b->graph_steps = graph_steps;
+    }
+    for (; done + graph_steps <= steps; done += graph_steps)
+      if (hipGraphLaunch(b->exec, b->stream) != hipSuccess) return fail(MW_ERR_LAUNCH, "batch: graph launch failed");
+  }
+  for (; done < steps; ++done)
+    if ((rc = batch_step(b, b->stream))) return rc;
+  if (hipEventRecord(b->ev_out, b->stream) != hipSuccess || hipStreamWaitEvent(caller, b->ev_out, 0) != hipSuccess)
+    return fail(MW_ERR_RUNTIME, "batch: cannot order the caller's stream after the batch");
+  return MW_OK;
+}
+
+int mw_batch_destroy(mw_batch* b) {
+  if (!b) return MW_OK;
+  (void)hipStreamSynchronize(b->stream);
+  batch_free(b);
   return MW_OK;
 }
 

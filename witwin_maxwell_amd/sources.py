@@ -140,7 +140,18 @@ def source_time_dict(st) -> dict:
     return {"kind": st.kind, "kind_code": KIND_CODE[st.kind], "frequency": float(st.frequency),
             "fwidth": float(getattr(st, "fwidth", 0.0)), "amplitude": float(st.amplitude),
             "phase": float(getattr(st, "phase", 0.0)), "delay": float(st.delay),
-            "characteristic_frequency": float(st.characteristic_frequency)}
+            "characteristic_frequency": float(st.characteristic_frequency),
+            "settling_time": float(settling_time(st))}
+
+
+def settling_time(st) -> float:
+    """Time after which the waveform has settled (sources.py:205, :249, :284):
+    CW 0, Gaussian delay + 6 sigma_t, Ricker delay + 4/f."""
+    if st.kind == "gaussian_pulse":
+        return float(st.delay) + 6.0 * gaussian_sigma_t(st.fwidth)
+    if st.kind == "ricker_wavelet":
+        return float(st.delay) + 4.0 / max(float(st.frequency), 1e-30)
+    return 0.0
 
 
 def polarization_vector(polarization):
