@@ -316,6 +316,18 @@ int mw_batch_create(const mw_program* const* programs, int32_t count, int32_t xc
 int mw_batch_run(mw_batch* batch, int32_t first, int32_t steps, int32_t graph_steps, void* stream);
 int mw_batch_destroy(mw_batch* batch);
 
+/* Adjoint of a scene batch: one adjoint state and design window per scene
+ * (the single-scene mw_reverse_steps / mw_capture_steps, batched: every launch
+ * covers all scenes).  `forward` must be the batch the scenes advanced in; its
+ * replay inside a capture accumulates no observers.  Full-field DFT seeds are not
+ * batched.  Same reference counterparts as mw_reverse_steps / mw_capture_steps. */
+typedef struct mw_adjoint_batch mw_adjoint_batch;
+int mw_adjoint_batch_create(mw_batch* forward, const mw_program* const* programs, const mw_adjoint* const* adjoints,
+                            const mw_window* const* windows, int32_t count, mw_adjoint_batch** out);
+int mw_adjoint_batch_capture(mw_adjoint_batch* batch, int32_t first, int32_t count, void* stream);
+int mw_adjoint_batch_reverse(mw_adjoint_batch* batch, int32_t first, int32_t count, void* stream);
+int mw_adjoint_batch_destroy(mw_adjoint_batch* batch);
+
 const char* mw_last_error(void);
 int32_t mw_abi_version(void);
 /* sizeof of mw_grid, mw_coeffs, mw_cpml, mw_fields, mw_pole, mw_ade, mw_term,

@@ -41,13 +41,14 @@ def _scene(boundary, cells, structures=(), sources=(), monitors=(), regions=()):
     return scene
 
 
-def _design_case(bname, seed=0):
+def _design_case(bname, seed=0, cells=(14, 12, 16)):
     rng = np.random.default_rng(seed)
-    cells = (14, 12, 16)
     density = torch.as_tensor(rng.uniform(0.1, 0.9, size=(4, 3, 3)), dtype=torch.float64)
     two_pi = 2 * np.pi
     structures = [mw.Structure(mw.Box((3 * D, 3 * D, 11 * D), (3 * D, 3 * D, 3 * D)), mw.Material(eps_r=2.2))]
-    if bname == "cpml_ade":
+    if bname == "plain_cpml":
+        boundary = mw.BoundarySpec.pml(3)
+    elif bname == "cpml_ade":
         boundary = mw.BoundarySpec.pml(3)
         structures.append(mw.Structure(mw.Box((10 * D, 8 * D, 4 * D), (3 * D, 3 * D, 3 * D)),
                                        mw.Material(eps_r=2.0, drude_poles=(mw.DrudePole(2e15 / two_pi, 1e14 / two_pi),),

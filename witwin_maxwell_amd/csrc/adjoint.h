@@ -163,10 +163,7 @@ __device__ __forceinline__ float gather_e(const RevArgs& a, int c, int t, int i,
 
 // P1: lambda_H of every H sample at (i, j, k).
 template <bool CPML>
-__global__ __launch_bounds__(256) void reverse_h_kernel(RevArgs a) {
-  const int k = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * 4 + threadIdx.y;
-  const int i = blockIdx.z;
+__device__ __forceinline__ void reverse_h_at(const RevArgs& a, int i, int j, int k) {
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   if (k >= nz || j >= ny || i >= nx) return;
   for (int h = 0; h < 3; ++h) {
@@ -196,10 +193,7 @@ __global__ __launch_bounds__(256) void reverse_h_kernel(RevArgs a) {
 
 // P2: lambda_E of every E sample at (i, j, k).
 template <bool CPML, bool ADE>
-__global__ __launch_bounds__(256) void reverse_e_kernel(RevArgs a) {
-  const int k = blockIdx.x * 64 + threadIdx.x;
-  const int j = blockIdx.y * 4 + threadIdx.y;
-  const int i = blockIdx.z;
+__device__ __forceinline__ void reverse_e_at(const RevArgs& a, int i, int j, int k) {
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   if (k >= nz || j >= ny || i >= nx) return;
   for (int c = 0; c < 3; ++c) {
@@ -269,6 +263,52 @@ __global__ __launch_bounds__(256) void reverse_e_kernel(RevArgs a) {
     else out += fwd(0, 0) + fwd(1, 0);              // Hx a (dEz/dy), Hy a (dEz/dx)
     a.a.e[c][o] = out;
   }
+}
+
+template <bool CPML>
+__global__ __launch_bounds__(256) void reverse_h_kernel(RevArgs a) {
+  reverse_h_at<CPML>(a, blockIdx.z, blockIdx.y * 4 + threadIdx.y, blockIdx.x * 64 + threadIdx.x);
+}
+
+template <bool CPML, bool ADE>
+__global__ __launch_bounds__(256) void reverse_e_kernel(RevArgs a) {
+  reverse_e_at<CPML, ADE>(a, blockIdx.z, blockIdx.y * 4 + threadIdx.y, blockIdx.x * 64 + threadIdx.x);
+}
+
+// Batched reverse phases: block b of the launch belongs to the scene s with
+// prefix[s] <= b < prefix[s+1]; scene s's blocks tile (z: 64, y: 4, x: 1).
+__device__ __forceinline__ int block_scene(const int* prefix, int count, int b) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (prefix[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <bool CPML>
+__global__ __launch_bounds__(256) void reverse_h_batch_kernel(const RevArgs* __restrict__ table, const int* prefix,
+                                                              int count, int total) {
+  const int b = blockIdx.x;
+  if (b >= total) return;
+  const int s = block_scene(prefix, count, b);
+  const RevArgs& a = table[s];
+  const int t = b - prefix[s];
+  const int bz = (a.g.nz + 63) / 64, by = (a.g.ny + 3) / 4;
+  reverse_h_at<CPML>(a, t / (bz * by), (t / bz) % by * 4 + threadIdx.y, t % bz * 64 + threadIdx.x);
+}
+
+template <bool CPML, bool ADE>
+__global__ __launch_bounds__(256) void reverse_e_batch_kernel(const RevArgs* __restrict__ table, const int* prefix,
+                                                              int count, int total) {
+  const int b = blockIdx.x;
+  if (b >= total) return;
+  const int s = block_scene(prefix, count, b);
+  const RevArgs& a = table[s];
+  const int t = b - prefix[s];
+  const int bz = (a.g.nz + 63) / 64, by = (a.g.ny + 3) / 4;
+  reverse_e_at<CPML, ADE>(a, t / (bz * by), (t / bz) % by * 4 + threadIdx.y, t % bz * 64 + threadIdx.x);
 }
 
 // --------------------------------------------------------------- seeds
@@ -387,6 +427,137 @@ __global__ __launch_bounds__(256) void window_gradient_kernel(WindowArgs w, mw_g
   float* gc = c == 0 ? g_curl0 : (c == 1 ? g_curl1 : g_curl2);
   gd[l] += lv * e_win[c * per + l];
   gc[l] += lv * curl;
+}
+
+// ------------------------------------------------------ batched adjoint
+// Seeds of one observer plane / point / full-field DFT component of one scene:
+// lam += sum_f g_re[f] w_cos[n, f] + g_im[f] w_sin[n, f].
+struct SeedPlane {
+  int32_t axis, index, nf, family;  // family 0 = E, 1 = H
+  int32_t shape[3];
+  int32_t pad;
+  float* lam;
+  const float* gre;
+  const float* gim;
+  const float* w_cos;  // (steps, nf)
+  const float* w_sin;
+};
+
+__global__ __launch_bounds__(256) void seed_planes_batch_kernel(const SeedPlane* __restrict__ planes, int count,
+                                                                int step, int family) {
+  const int qi = blockIdx.y;
+  if (qi >= count) return;
+  const SeedPlane& q = planes[qi];
+  if (q.family != family) return;
+  const int sx = q.shape[0], sy = q.shape[1], sz = q.shape[2];
+  int na, nb;
+  if (q.axis == 0) { na = sy; nb = sz; } else if (q.axis == 1) { na = sx; nb = sz; } else { na = sx; nb = sy; }
+  const int64_t cells = (int64_t)na * nb;
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= cells) return;
+  const int ia = (int)(l / nb), ib = (int)(l % nb);
+  uint32_t o;
+  if (q.axis == 0) o = ((uint32_t)q.index * sy + ia) * sz + ib;
+  else if (q.axis == 1) o = ((uint32_t)ia * sy + q.index) * sz + ib;
+  else o = ((uint32_t)ia * sy + ib) * sz + q.index;
+  const float* wc = q.w_cos + (int64_t)step * q.nf;
+  const float* ws = q.w_sin + (int64_t)step * q.nf;
+  float v = 0.0f;
+  for (int f = 0; f < q.nf; ++f) v += q.gre[f * cells + l] * wc[f] + q.gim[f * cells + l] * ws[f];
+  atomicAdd(q.lam + o, v);
+}
+
+struct SeedPoint {
+  int32_t linear, nf, stride, family;
+  float* lam;
+  const float* gre;  // offset to this point's column of (nf, stride)
+  const float* gim;
+  const float* w_cos;
+  const float* w_sin;
+};
+
+__global__ __launch_bounds__(64) void seed_points_batch_kernel(const SeedPoint* __restrict__ points, int count,
+                                                               int step, int family) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= count) return;
+  const SeedPoint& q = points[p];
+  if (q.family != family) return;
+  const float* wc = q.w_cos + (int64_t)step * q.nf;
+  const float* ws = q.w_sin + (int64_t)step * q.nf;
+  float v = 0.0f;
+  for (int f = 0; f < q.nf; ++f) v += q.gre[(int64_t)f * q.stride] * wc[f] + q.gim[(int64_t)f * q.stride] * ws[f];
+  atomicAdd(q.lam + q.linear, v);
+}
+
+// Design window of one scene for the batched capture / gradient kernels.
+struct BatchWindow {
+  WindowArgs w;
+  mw_grid g;
+  const float* e_state[2][3];  // E buffers holding a state of even / odd index
+  const float* h_state[2][3];
+  float* lam_e[3];
+  float* e_store;
+  float* h_store;
+  float* g_decay[3];
+  float* g_curl[3];
+};
+
+// Capture E of state m (slot m - first) and, for m > first, H of the step that
+// produced it (slot m - first - 1).  blockIdx.y = 3 scene + component.
+__global__ __launch_bounds__(256) void window_capture_batch_kernel(const BatchWindow* __restrict__ wins, int count,
+                                                                   int m, int first) {
+  const int s = blockIdx.y / 3, c = blockIdx.y % 3;
+  if (s >= count) return;
+  const BatchWindow& b = wins[s];
+  const int wx = b.w.hi[0] - b.w.lo[0], wy = b.w.hi[1] - b.w.lo[1], wz = b.w.hi[2] - b.w.lo[2];
+  const int64_t per = (int64_t)wx * wy * wz, hper = (int64_t)(wx + 1) * (wy + 1) * (wz + 1);
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int par = m & 1;
+  if (l < per) {
+    const int wk = (int)(l % wz), wj = (int)((l / wz) % wy), wi = (int)(l / ((int64_t)wz * wy));
+    int sx, sy, sz;
+    e_shape(b.g, c, sx, sy, sz);
+    b.e_store[(int64_t)(m - first) * 3 * per + c * per + l] =
+        b.e_state[par][c][((uint32_t)(b.w.lo[0] + wi) * sy + (b.w.lo[1] + wj)) * sz + (b.w.lo[2] + wk)];
+  }
+  if (m > first && l < hper) {
+    const int hy = wy + 1, hz = wz + 1;
+    const int wk = (int)(l % hz), wj = (int)((l / hz) % hy), wi = (int)(l / ((int64_t)hz * hy));
+    int sx, sy, sz;
+    h_shape(b.g, c, sx, sy, sz);
+    b.h_store[(int64_t)(m - first - 1) * 3 * hper + c * hper + l] =
+        b.h_state[par][c][((uint32_t)(b.w.lo[0] - 1 + wi) * sy + (b.w.lo[1] - 1 + wj)) * sz + (b.w.lo[2] - 1 + wk)];
+  }
+}
+
+__global__ __launch_bounds__(256) void window_gradient_batch_kernel(const BatchWindow* __restrict__ wins, int count,
+                                                                    int slot) {
+  const int s = blockIdx.y / 3, c = blockIdx.y % 3;
+  if (s >= count) return;
+  const BatchWindow& b = wins[s];
+  const WindowArgs& w = b.w;
+  const int wx = w.hi[0] - w.lo[0], wy = w.hi[1] - w.lo[1], wz = w.hi[2] - w.lo[2];
+  const int64_t per = (int64_t)wx * wy * wz;
+  const int hy = wy + 1, hz = wz + 1;
+  const int64_t hper = (int64_t)(wx + 1) * hy * hz;
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= per) return;
+  const int wk = (int)(l % wz), wj = (int)((l / wz) % wy), wi = (int)(l / ((int64_t)wz * wy));
+  const int i = w.lo[0] + wi, j = w.lo[1] + wj, k = w.lo[2] + wk;
+  const float* hw = b.h_store + (int64_t)slot * 3 * hper;
+  auto H = [&](int h, int di, int dj, int dk) {
+    return hw[h * hper + ((int64_t)(wi + 1 + di) * hy + (wj + 1 + dj)) * hz + (wk + 1 + dk)];
+  };
+  const mw_grid& g = b.g;
+  float curl;
+  if (c == 0) curl = (H(2, 0, 0, 0) - H(2, 0, -1, 0)) * g.inv_e[1][j] - (H(1, 0, 0, 0) - H(1, 0, 0, -1)) * g.inv_e[2][k];
+  else if (c == 1) curl = (H(0, 0, 0, 0) - H(0, 0, 0, -1)) * g.inv_e[2][k] - (H(2, 0, 0, 0) - H(2, -1, 0, 0)) * g.inv_e[0][i];
+  else curl = (H(1, 0, 0, 0) - H(1, -1, 0, 0)) * g.inv_e[0][i] - (H(0, 0, 0, 0) - H(0, 0, -1, 0)) * g.inv_e[1][j];
+  int sx, sy, sz;
+  e_shape(g, c, sx, sy, sz);
+  const float lv = b.lam_e[c][((uint32_t)i * sy + j) * sz + k];
+  b.g_decay[c][l] += lv * b.e_store[(int64_t)slot * 3 * per + c * per + l];
+  b.g_curl[c][l] += lv * curl;
 }
 
 }  // namespace mw

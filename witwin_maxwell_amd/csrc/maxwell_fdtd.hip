@@ -483,8 +483,9 @@ int upload(const std::vector<T>& host, T** dst, const char* what) {
   return MW_OK;
 }
 
-// One step of every scene, reading the step index from the device counter.
-int batch_step(const mw_batch* b, hipStream_t s) {
+// One step of every scene, reading the step index from the device counter
+// (observers off for the adjoint's replay: the forward sums are already taken).
+int batch_step(const mw_batch* b, hipStream_t s, bool observe = true) {
   int rc;
   if (b->n_aux > 0) {
     hipLaunchKernelGGL(mw::aux_line_h_batch_kernel, dim3((b->max_aux + 255) / 256, b->n_aux), dim3(256), 0, s,
@@ -518,12 +519,12 @@ int batch_step(const mw_batch* b, hipStream_t s) {
                        b->d_aux, b->n_aux, b->d_counter);
     if ((rc = check_launch("batch aux_line_e"))) return rc;
   }
-  if (b->n_planes > 0) {
+  if (observe && b->n_planes > 0) {
     hipLaunchKernelGGL(mw::accumulate_planes_batch_kernel, dim3(grid_blocks(b->max_plane_cells), b->n_planes),
                        dim3(256), 0, s, b->d_planes, b->n_planes, b->d_counter);
     if ((rc = check_launch("batch accumulate_planes"))) return rc;
   }
-  if (b->n_points > 0) {
+  if (observe && b->n_points > 0) {
     hipLaunchKernelGGL(mw::accumulate_points_batch_kernel, dim3((b->n_points + 63) / 64), dim3(64), 0, s,
                        b->d_points, b->n_points, b->d_counter);
     if ((rc = check_launch("batch accumulate_points"))) return rc;
@@ -1074,6 +1075,235 @@ int mw_batch_run(mw_batch* b, int32_t first, int32_t steps, int32_t graph_steps,
     if ((rc = batch_step(b, b->stream))) return rc;
   if (hipEventRecord(b->ev_out, b->stream) != hipSuccess || hipStreamWaitEvent(caller, b->ev_out, 0) != hipSuccess)
     return fail(MW_ERR_RUNTIME, "batch: cannot order the caller's stream after the batch");
+  return MW_OK;
+}
+
+}  // extern "C"
+
+struct mw_adjoint_batch {
+  mw_batch* fwd = nullptr;
+  int count = 0;
+  bool cpml = false, ade = false;
+  mw::RevArgs* d_rev = nullptr;
+  int* d_prefix = nullptr;
+  int total = 0;
+  mw::SeedPlane* d_planes = nullptr;
+  int n_planes = 0;
+  int64_t max_plane_cells = 0;
+  mw::SeedPoint* d_points = nullptr;
+  int n_points = 0;
+  mw::BatchWindow* d_windows = nullptr;
+  int64_t max_window = 0, max_hwindow = 0;
+  int capacity = 0;
+};
+
+namespace {
+void adjoint_batch_free(mw_adjoint_batch* b) {
+  if (!b) return;
+  void* bufs[] = {b->d_rev, b->d_prefix, b->d_planes, b->d_points, b->d_windows};
+  for (void* q : bufs)
+    if (q) (void)hipFree(q);
+  delete b;
+}
+}  // namespace
+
+extern "C" {
+
+int mw_adjoint_batch_create(mw_batch* fwd, const mw_program* const* programs, const mw_adjoint* const* adjoints,
+                            const mw_window* const* windows, int32_t count, mw_adjoint_batch** out) {
+  if (!fwd || !programs || !adjoints || !windows || !out || count != fwd->count)
+    return fail(MW_ERR_ARGUMENT, "adjoint batch: need the forward batch and one program/adjoint/window per scene");
+  *out = nullptr;
+  mw_adjoint_batch* b = new mw_adjoint_batch();
+  auto bail = [&](int rc) { adjoint_batch_free(b); return rc; };
+  b->fwd = fwd;
+  b->count = count;
+  std::vector<mw::RevArgs> rev;
+  std::vector<int> prefix(1, 0);
+  std::vector<mw::SeedPlane> planes;
+  std::vector<mw::SeedPoint> points;
+  std::vector<mw::BatchWindow> wins;
+  int rc;
+  for (int si = 0; si < count; ++si) {
+    const mw_program* p = programs[si];
+    const mw_adjoint* adj = adjoints[si];
+    const mw_window* w = windows[si];
+    if (!p || !adj || !w) return bail(fail(MW_ERR_ARGUMENT, "adjoint batch: scene %d has a NULL input", si));
+    const mw_grid* g = &p->grid;
+    if ((rc = check_grid(g)) || (rc = check_fields(&adj->fields, true, true)) || (rc = check_window(p, w))) return bail(rc);
+    if (p->nf_dft > 0) return bail(fail(MW_ERR_ARGUMENT, "adjoint batch: full-field DFT seeds are not batched (scene %d)", si));
+    for (int c = 0; c < 3; ++c)
+      if (!w->g_decay[c] || !w->g_curl[c]) return bail(fail(MW_ERR_ARGUMENT, "adjoint batch: scene %d window gradients", si));
+    mw::RevArgs a{};
+    a.g = *g;
+    a.cf = p->coeffs;
+    a.cp = p->cpml;
+    a.a = adj->fields;
+    a.cpml = p->cpml.enabled ? 1 : 0;
+    if (a.cpml)
+      for (int q = 0; q < 12; ++q) {
+        a.cp.psi[q] = adj->psi[q];
+        if (p->cpml.psi_low[q] + p->cpml.psi_high_len[q] > 0 && !adj->psi[q])
+          return bail(fail(MW_ERR_ARGUMENT, "adjoint batch: scene %d adjoint psi %d is NULL", si, q));
+      }
+    a.ade = p->has_ade ? 1 : 0;
+    if (a.ade) {
+      if ((rc = check_ade(g, &p->ade))) return bail(rc);
+      a.ad = p->ade;
+      for (int c = 0; c < 3; ++c)
+        for (int q = 0; q < p->ade.count[c]; ++q) {
+          mw_pole& pl = a.ad.poles[c][q];
+          pl.polarization = adj->polarization[c][q];
+          pl.current = adj->current[c][q];
+          if (!pl.current || (pl.kind != 1 && !pl.polarization))
+            return bail(fail(MW_ERR_ARGUMENT, "adjoint batch: scene %d pole (%d, %d) state is NULL", si, c, q));
+        }
+    }
+    if (si == 0) { b->cpml = a.cpml; b->ade = a.ade; }
+    else if ((bool)a.cpml != b->cpml || (bool)a.ade != b->ade)
+      return bail(fail(MW_ERR_ARGUMENT, "adjoint batch: scene %d differs in CPML/ADE use", si));
+    rev.push_back(a);
+    prefix.push_back(prefix.back() + ((g->nz + 63) / 64) * ((g->ny + 3) / 4) * g->nx);
+    // seeds
+    if (p->n_planes < 0 || p->n_planes > MW_MAX_PLANES) return bail(fail(MW_ERR_ARGUMENT, "adjoint batch: plane count"));
+    for (int q = 0; q < p->n_planes; ++q) {
+      if (!adj->g_plane_re[q] && !adj->g_plane_im[q]) continue;
+      if (!adj->g_plane_re[q] || !adj->g_plane_im[q])
+        return bail(fail(MW_ERR_ARGUMENT, "adjoint batch: scene %d plane %d: give both seed gradients or neither", si, q));
+      const mw_plane_obs& o = p->planes[q];
+      if (o.component < 0 || o.component > 5 || o.axis < 0 || o.axis > 2)
+        return bail(fail(MW_ERR_ARGUMENT, "adjoint batch: scene %d plane %d descriptor invalid", si, q));
+      mw::SeedPlane sp{};
+      sp.axis = o.axis; sp.index = o.index; sp.nf = p->nf_obs; sp.family = o.component < 3 ? 0 : 1;
+      component_shape(g, o.component, sp.shape);
+      if (o.index < 0 || o.index >= sp.shape[o.axis]) return bail(fail(MW_ERR_ARGUMENT, "adjoint batch: plane index"));
+      sp.lam = o.component < 3 ? adj->fields.e[o.component] : adj->fields.h[o.component - 3];
+      sp.gre = adj->g_plane_re[q]; sp.gim = adj->g_plane_im[q];
+      sp.w_cos = o.component < 3 ? p->obs_e_cos : p->obs_h_cos;
+      sp.w_sin = o.component < 3 ? p->obs_e_sin : p->obs_h_sin;
+      if (!sp.w_cos || !sp.w_sin) return bail(fail(MW_ERR_ARGUMENT, "adjoint batch: observer weight tables missing"));
+      const int* sh = sp.shape;
+      b->max_plane_cells = std::max(b->max_plane_cells, o.axis == 0 ? (int64_t)sh[1] * sh[2]
+                                                        : (o.axis == 1 ? (int64_t)sh[0] * sh[2] : (int64_t)sh[0] * sh[1]));
+      planes.push_back(sp);
+    }
+    if (p->n_points > 0 && adj->g_point_re && adj->g_point_im) {
+      std::vector<int32_t> comp(p->n_points), lin(p->n_points);
+      if (hipMemcpy(comp.data(), p->point_component, p->n_points * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
+          hipMemcpy(lin.data(), p->point_linear, p->n_points * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+        return bail(fail(MW_ERR_RUNTIME, "adjoint batch: cannot read scene %d point tables", si));
+      for (int q = 0; q < p->n_points; ++q) {
+        if (comp[q] < 0 || comp[q] > 5) return bail(fail(MW_ERR_ARGUMENT, "adjoint batch: point component"));
+        int sh[3];
+        component_shape(g, comp[q], sh);
+        if (lin[q] < 0 || (int64_t)lin[q] >= (int64_t)sh[0] * sh[1] * sh[2])
+          return bail(fail(MW_ERR_ARGUMENT, "adjoint batch: point index outside the field"));
+        mw::SeedPoint sp{};
+        sp.linear = lin[q]; sp.nf = p->nf_obs; sp.stride = p->n_points; sp.family = comp[q] < 3 ? 0 : 1;
+        sp.lam = comp[q] < 3 ? adj->fields.e[comp[q]] : adj->fields.h[comp[q] - 3];
+        sp.gre = adj->g_point_re + q; sp.gim = adj->g_point_im + q;
+        sp.w_cos = comp[q] < 3 ? p->obs_e_cos : p->obs_h_cos;
+        sp.w_sin = comp[q] < 3 ? p->obs_e_sin : p->obs_h_sin;
+        points.push_back(sp);
+      }
+    }
+    mw::BatchWindow bw{};
+    bw.w = window_args(w);
+    bw.g = *g;
+    for (int c = 0; c < 3; ++c) {
+      bw.e_state[0][c] = p->fields.e[c];
+      bw.e_state[1][c] = p->fused ? p->fields_alt.e[c] : p->fields.e[c];
+      bw.h_state[0][c] = p->fields.h[c];
+      bw.h_state[1][c] = p->fused ? p->fields_alt.h[c] : p->fields.h[c];
+      bw.lam_e[c] = adj->fields.e[c];
+      bw.g_decay[c] = w->g_decay[c];
+      bw.g_curl[c] = w->g_curl[c];
+    }
+    bw.e_store = w->e_store;
+    bw.h_store = w->h_store;
+    b->capacity = si == 0 ? w->capacity : std::min(b->capacity, w->capacity);
+    b->max_window = std::max(b->max_window, window_cells(w, 0));
+    b->max_hwindow = std::max(b->max_hwindow, window_cells(w, 1));
+    wins.push_back(bw);
+  }
+  b->total = prefix.back();
+  b->n_planes = (int)planes.size();
+  b->n_points = (int)points.size();
+  if ((rc = upload(rev, &b->d_rev, "reverse arguments")) || (rc = upload(prefix, &b->d_prefix, "block table")) ||
+      (rc = upload(planes, &b->d_planes, "seed planes")) || (rc = upload(points, &b->d_points, "seed points")) ||
+      (rc = upload(wins, &b->d_windows, "windows")))
+    return bail(rc);
+  *out = b;
+  return MW_OK;
+}
+
+int mw_adjoint_batch_capture(mw_adjoint_batch* b, int32_t first, int32_t count, void* stream) {
+  if (!b) return fail(MW_ERR_ARGUMENT, "adjoint batch is NULL");
+  mw_batch* f = b->fwd;
+  if (first < 0 || count < 0 || first + count > f->total_steps || count > b->capacity)
+    return fail(MW_ERR_ARGUMENT, "adjoint batch: capture [%d, %d) outside the steps or the window capacity %d", first,
+                first + count, b->capacity);
+  hipStream_t s = (hipStream_t)stream;
+  int rc;
+  hipLaunchKernelGGL(mw::step_counter_kernel, dim3(1), dim3(64), 0, s, f->d_counter, 1, first);
+  if ((rc = check_launch("adjoint batch counter"))) return rc;
+  const int64_t cells = std::max(b->max_window, b->max_hwindow);
+  const dim3 cgrid(grid_blocks(cells), 3 * b->count);
+  hipLaunchKernelGGL(mw::window_capture_batch_kernel, cgrid, dim3(256), 0, s, b->d_windows, b->count, first, first);
+  if ((rc = check_launch("batch window capture"))) return rc;
+  for (int m = first + 1; m <= first + count; ++m) {
+    if ((rc = batch_step(f, s, false))) return rc;
+    hipLaunchKernelGGL(mw::window_capture_batch_kernel, cgrid, dim3(256), 0, s, b->d_windows, b->count, m, first);
+    if ((rc = check_launch("batch window capture"))) return rc;
+  }
+  return MW_OK;
+}
+
+int mw_adjoint_batch_reverse(mw_adjoint_batch* b, int32_t first, int32_t count, void* stream) {
+  if (!b) return fail(MW_ERR_ARGUMENT, "adjoint batch is NULL");
+  if (first < 0 || count < 0 || first + count > b->fwd->total_steps || count > b->capacity)
+    return fail(MW_ERR_ARGUMENT, "adjoint batch: reverse [%d, %d) outside the steps or the window capacity", first,
+                first + count);
+  hipStream_t s = (hipStream_t)stream;
+  int rc;
+  const dim3 block(64, 4, 1);
+  const dim3 grid(b->total);
+  auto seeds = [&](int n, int family) {
+    if (b->n_planes > 0) {
+      hipLaunchKernelGGL(mw::seed_planes_batch_kernel, dim3(grid_blocks(b->max_plane_cells), b->n_planes), dim3(256),
+                         0, s, b->d_planes, b->n_planes, n, family);
+      if (int r = check_launch("batch seed planes")) return r;
+    }
+    if (b->n_points > 0) {
+      hipLaunchKernelGGL(mw::seed_points_batch_kernel, dim3((b->n_points + 63) / 64), dim3(64), 0, s, b->d_points,
+                         b->n_points, n, family);
+      if (int r = check_launch("batch seed points")) return r;
+    }
+    return MW_OK;
+  };
+  for (int n = first + count - 1; n >= first; --n) {
+    if ((rc = seeds(n, 0))) return rc;
+    if (b->cpml) hipLaunchKernelGGL(mw::reverse_h_batch_kernel<true>, grid, block, 0, s, b->d_rev, b->d_prefix, b->count, b->total);
+    else hipLaunchKernelGGL(mw::reverse_h_batch_kernel<false>, grid, block, 0, s, b->d_rev, b->d_prefix, b->count, b->total);
+    if ((rc = check_launch("batch reverse_h"))) return rc;
+    if ((rc = seeds(n, 1))) return rc;
+    hipLaunchKernelGGL(mw::window_gradient_batch_kernel, dim3(grid_blocks(b->max_window), 3 * b->count), dim3(256), 0, s,
+                       b->d_windows, b->count, n - first);
+    if ((rc = check_launch("batch window gradient"))) return rc;
+    if (b->cpml) {
+      if (b->ade) hipLaunchKernelGGL((mw::reverse_e_batch_kernel<true, true>), grid, block, 0, s, b->d_rev, b->d_prefix, b->count, b->total);
+      else hipLaunchKernelGGL((mw::reverse_e_batch_kernel<true, false>), grid, block, 0, s, b->d_rev, b->d_prefix, b->count, b->total);
+    } else {
+      if (b->ade) hipLaunchKernelGGL((mw::reverse_e_batch_kernel<false, true>), grid, block, 0, s, b->d_rev, b->d_prefix, b->count, b->total);
+      else hipLaunchKernelGGL((mw::reverse_e_batch_kernel<false, false>), grid, block, 0, s, b->d_rev, b->d_prefix, b->count, b->total);
+    }
+    if ((rc = check_launch("batch reverse_e"))) return rc;
+  }
+  return MW_OK;
+}
+
+int mw_adjoint_batch_destroy(mw_adjoint_batch* b) {
+  adjoint_batch_free(b);
   return MW_OK;
 }
 

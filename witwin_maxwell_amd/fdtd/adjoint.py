@@ -229,6 +229,17 @@ class DeviceAdjoint:
         nat.check(rc, self.lib)
 
 
+def run_outputs(run: DeviceRun):
+    """The observer sums of a run, in the order the autograd boundary returns them:
+    point re/im, plane re (each), plane im (each), full-field DFT re, im."""
+    out = []
+    if run.point_re is not None:
+        out += [run.point_re, run.point_im]
+    out += list(run.plane_re) + list(run.plane_im)
+    out += [run.dft_re[c] for c in E_COMPS if c in run.dft_re] + [run.dft_im[c] for c in E_COMPS if c in run.dft_im]
+    return out
+
+
 class GradientSession:
     """Checkpointed forward + reverse sweep for one plan with a design window."""
 
@@ -240,13 +251,7 @@ class GradientSession:
         self.checkpoints = {}
 
     def outputs(self):
-        run = self.run
-        out = []
-        if run.point_re is not None:
-            out += [run.point_re, run.point_im]
-        out += list(run.plane_re) + list(run.plane_im)
-        out += [run.dft_re[c] for c in E_COMPS if c in run.dft_re] + [run.dft_im[c] for c in E_COMPS if c in run.dft_im]
-        return out
+        return run_outputs(self.run)
 
     def forward(self):
         run, T, K = self.run, self.plan.time_steps, self.stride

@@ -137,3 +137,154 @@ class BatchRun:
             self.xchunk = int(chunk)
             self._create()
         self.graph_steps = int(graph_steps)
+
+
+# ---------------------------------------------------------------- gradients
+class BatchGradientSession:
+    """Checkpointed forward + reverse sweep of many scenes at once: the forward
+    advances through ``BatchRun``; backward restores each scene's segment
+    snapshot, replays the segment for all scenes with the design-window capture
+    and walks it backwards with the batched reverse kernels."""
+
+    def __init__(self, plans, device="cuda", windows=None, stride=None, xchunk=0, graph_steps=1):
+        from .adjoint import default_stride, design_window
+        self.plans = list(plans)
+        T = {p.time_steps for p in self.plans}
+        if len(T) != 1:
+            raise ValueError("batched scenes must run the same number of steps")
+        self.T = T.pop()
+        self.runs = [DeviceRun(p, device, fused=True) for p in self.plans]
+        self.batch = BatchRun(self.runs, xchunk=xchunk)
+        self.batch.graph_steps = int(graph_steps)
+        self.windows = list(windows) if windows is not None else [design_window(p) for p in self.plans]
+        self.stride = int(stride or default_stride(self.T))
+        self.checkpoints = []
+
+    def outputs(self, si):
+        from .adjoint import run_outputs
+        return run_outputs(self.runs[si])
+
+    def forward(self):
+        from .adjoint import snapshot
+        self.batch.reset()
+        self.checkpoints = []
+        K = self.stride
+        for s0 in range(0, self.T, K):
+            self.checkpoints.append([snapshot(r) for r in self.runs] if s0 > 0 else None)
+            self.batch.run(min(K, self.T - s0))
+        return [self.outputs(i) for i in range(len(self.runs))]
+
+    def backward(self, grads_per_scene):
+        """grads_per_scene[i] follows outputs(i); returns per scene the window
+        (g_decay[3], g_curl[3]) as float64 tensors."""
+        from .adjoint import DeviceAdjoint, restore
+        K, T = self.stride, self.T
+        adjs, scales = [], []
+        for run, win, grads in zip(self.runs, self.windows, grads_per_scene):
+            grads = list(grads)
+            point = (grads.pop(0), grads.pop(0)) if run.point_re is not None else None
+            n = len(run.plane_re)
+            planes = list(zip(grads[:n], grads[n:2 * n]))
+            peak = 0.0
+            for t in ([*point] if point else []) + [x for pair in planes for x in pair]:
+                peak = max(peak, float(t.detach().abs().max()) if t.numel() else 0.0)
+            scale = peak if peak > 0 else 1.0
+            sc = lambda t: t.detach().double() / scale
+            adj = DeviceAdjoint(run, win, capacity=K)
+            adj.zero()
+            adj.set_seeds(point=(sc(point[0]), sc(point[1])) if point else None,
+                          planes=[(sc(r), sc(i)) for r, i in planes])
+            adjs.append(adj)
+            scales.append(scale)
+        lib = self.batch.lib
+        count = len(self.runs)
+        progs = (ctypes.POINTER(nat.MwProgram) * count)(*[ctypes.pointer(r.program) for r in self.runs])
+        adjp = (ctypes.POINTER(nat.MwAdjoint) * count)(*[ctypes.pointer(a.adjoint) for a in adjs])
+        winp = (ctypes.POINTER(nat.MwWindow) * count)(*[ctypes.pointer(a.window) for a in adjs])
+        handle = ctypes.c_void_p()
+        with torch.cuda.device(self.runs[0].device):
+            nat.check(lib.mw_adjoint_batch_create(self.batch._handle, progs, adjp, winp, count, ctypes.byref(handle)),
+                      lib)
+            try:
+                starts = list(range(0, T, K))
+                for ci, s0 in reversed(list(enumerate(starts))):
+                    cnt = min(K, T - s0)
+                    snaps = self.checkpoints[ci]
+                    for ri, run in enumerate(self.runs):
+                        restore(run, None if snaps is None else snaps[ri], s0)
+                    stream = nat.current_stream()
+                    nat.check(lib.mw_adjoint_batch_capture(handle, s0, cnt, stream), lib)
+                    nat.check(lib.mw_adjoint_batch_reverse(handle, s0, cnt, stream), lib)
+            finally:
+                lib.mw_adjoint_batch_destroy(handle)
+        for run in self.runs:
+            run.step_index = T
+        self.adjoints = adjs
+        return [([g.double() * sc_ for g in a.g_decay], [g.double() * sc_ for g in a.g_curl])
+                for a, sc_ in zip(adjs, scales)]
+
+
+class _BatchGradientFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, session, n_per_scene, *coefs):
+        ctx.session = session
+        ctx.n_out = [len(session.outputs(i)) for i in range(len(session.runs))]
+        outs = session.forward()
+        return tuple(o.detach().clone() for scene in outs for o in scene)
+
+    @staticmethod
+    def backward(ctx, *grad_outputs):
+        session = ctx.session
+        grads, pos = [], 0
+        for i, n in enumerate(ctx.n_out):
+            outs = session.outputs(i)
+            g = [grad_outputs[pos + k] if grad_outputs[pos + k] is not None else torch.zeros_like(outs[k])
+                 for k in range(n)]
+            grads.append(g)
+            pos += n
+        res = session.backward(grads)
+        flat = []
+        for g_decay, g_curl in res:
+            flat += list(g_decay) + list(g_curl)
+        return (None, None, *flat)
+
+
+def batched_differentiable_run(scenes, plan_kwargs, device="cuda", stride=None, xchunk=0, graph_steps=1):
+    """Differentiable run of many scenes at once.  ``scenes`` is a list of
+    ``Scene`` whose material regions carry the densities to differentiate;
+    returns (plans, monitors per scene, session)."""
+    from ..scene import prepare_scene
+    from . import materials as mat
+    from .adjoint import design_window, window_coefficients
+    from .plan import compile_plan
+    from .spectral import assemble_monitor_results
+    plans, coefs = [], []
+    for scene in scenes:
+        dens = [r.density for r in scene.material_regions]
+        prepared = prepare_scene(scene)
+        eps_node = mat.node_permittivity(prepared, mat.NodeMaterialModel(prepared), dens)
+        plan = compile_plan(scene, region_densities=[d.detach() for d in dens], **plan_kwargs)
+        lo, hi = design_window(plan)
+        c = window_coefficients(plan, eps_node, lo, hi)
+        coefs += [d for d, _ in c] + [cu for _, cu in c]
+        plans.append(plan)
+    session = BatchGradientSession(plans, device, stride=stride, xchunk=xchunk, graph_steps=graph_steps)
+    outs = list(_BatchGradientFunction.apply(session, len(scenes), *coefs))
+    monitors, pos = [], 0
+    for i, (plan, run) in enumerate(zip(plans, session.runs)):
+        n = len(session.outputs(i))
+        mine = outs[pos:pos + n]
+        pos += n
+        pr = pi = None
+        if run.point_re is not None:
+            pr, pi = mine.pop(0), mine.pop(0)
+        k = len(run.plane_re)
+        qr, qi = mine[:k], mine[k:2 * k]
+        if plan.observers is None:
+            monitors.append({})
+            continue
+        if pr is None:
+            pr = torch.zeros((len(plan.observers.frequencies), 0), dtype=torch.float32, device=run.device)
+            pi = pr
+        monitors.append(assemble_monitor_results(plan.observers, pr, pi, qr, qi, plan.obs_window_sum))
+    return plans, monitors, session
