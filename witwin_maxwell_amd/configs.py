@@ -122,4 +122,37 @@ def design_region(cells=(200, 200, 120), design=(64, 64, 32), pml=12, steps=3000
     return scene, {"frequencies": (F0,), "time_steps": steps, "courant": COURANT, "theta": theta}
 
 
-__all__ = ["vacuum_dipole", "dielectric_boxes", "dispersive_spheres", "design_region", "F0", "DX", "LAMBDA0"]
+def batched_scenes(count=32, smallest=(64, 64, 64), largest=(192, 192, 128), design=(32, 32, 16), pml=10,
+                   steps=1500, seed=0, device="cuda", n_boxes=6):
+    """C5: ``count`` independent scenes of seeded uneven sizes between ``smallest``
+    and ``largest`` cells, each with random dielectric boxes (eps ~ U[1,12]), a
+    sigmoid-parameterised design region, one Ez dipole below it and a
+    single-frequency flux plane above it.  Returns a list of (scene, theta) and
+    the shared run kwargs (every scene runs the same steps at Courant 0.99)."""
+    rng = np.random.default_rng(seed)
+    lo, hi = np.array(smallest), np.array(largest)
+    out = []
+    dsize = np.array(design) * DX
+    for _ in range(count):
+        cells = tuple(int(v) for v in np.round(lo + rng.uniform(0, 1) * (hi - lo)))
+        scene = Scene(domain=_domain(cells), grid=GridSpec.uniform(DX), boundary=BoundarySpec.pml(pml), device=device)
+        half = np.array(cells) * 0.5 * DX
+        for _ in range(n_boxes):
+            size = rng.integers(4, 16, size=3) * DX
+            centre = rng.uniform(-0.8 * half, 0.8 * half)
+            # keep boxes clear of the source and monitor planes (|z| > design/2 + 4 cells)
+            if abs(centre[2]) < 0.5 * dsize[2] + 8 * DX:
+                continue
+            scene.add_structure(Structure(Box(tuple(centre), tuple(size)), Material(eps_r=float(rng.uniform(1, 12)))))
+        theta = torch.as_tensor(rng.standard_normal(design), dtype=torch.float64)
+        scene.add_material_region(MaterialRegion("design", Box((0, 0, 0), tuple(dsize)), torch.sigmoid(theta),
+                                                 eps_bounds=(1.0, 12.0)))
+        zsrc = -0.5 * dsize[2] - 4 * DX
+        scene.add_source(PointDipole((0, 0, zsrc), "Ez", profile="ideal", source_time=GaussianPulse(F0, F0 / 4)))
+        scene.add_monitor(FluxMonitor("objective", axis="z", position=0.5 * dsize[2] + 4 * DX, frequencies=(F0,)))
+        out.append((scene, theta))
+    return out, {"frequencies": (F0,), "time_steps": steps, "courant": COURANT}
+
+
+__all__ = ["vacuum_dipole", "dielectric_boxes", "dispersive_spheres", "design_region", "batched_scenes", "F0", "DX",
+           "LAMBDA0"]
