@@ -55,3 +55,24 @@ def test_argument_errors_are_reported(lib):
     prog = native.MwProgram()
     rc = lib.mw_run_steps(ctypes.byref(prog), 0, 1, None)
     assert rc == -1
+
+
+def test_adjoint_and_batch_argument_errors(lib):
+    """The adjoint and batch entry points reject bad arguments before any HIP call."""
+    vp = ctypes.c_void_p
+    assert lib.mw_capture_steps(None, 0, 1, None, None) == -1
+    assert lib.mw_reverse_steps(None, None, None, 0, 1, None) == -1
+    prog = native.MwProgram()
+    adj = native.MwAdjoint()
+    assert lib.mw_reverse_steps(ctypes.byref(prog), ctypes.byref(adj), None, 0, 1, None) == -1
+    handle = vp()
+    assert lib.mw_batch_create(None, 0, 0, ctypes.byref(handle)) == -1
+    progs = (ctypes.POINTER(native.MwProgram) * 1)(ctypes.POINTER(native.MwProgram)())
+    assert lib.mw_batch_create(progs, 1, 0, ctypes.byref(handle)) == -1
+    assert "NULL" in lib.mw_last_error().decode()
+    assert lib.mw_batch_create(progs, 1, 999, ctypes.byref(handle)) == -1
+    assert lib.mw_batch_run(None, 0, 1, 1, None) == -1
+    assert lib.mw_batch_destroy(None) == 0
+    assert lib.mw_adjoint_batch_create(None, None, None, None, 1, ctypes.byref(handle)) == -1
+    assert lib.mw_adjoint_batch_capture(None, 0, 1, None) == -1
+    assert lib.mw_adjoint_batch_reverse(None, 0, 1, None) == -1
