@@ -317,6 +317,7 @@ class _FDTDGradientFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, session, *coefs):
         ctx.session = session
+        ctx.coef_meta = [(c.device, c.dtype) for c in coefs]
         outs = session.forward()
         return tuple(o.detach().clone() for o in outs)
 
@@ -326,7 +327,8 @@ class _FDTDGradientFunction(torch.autograd.Function):
         outs = session.outputs()
         grads = [g if g is not None else _zeros_for(o) for g, o in zip(grad_outputs, outs)]
         g_decay, g_curl = session.backward(grads)
-        return (None, *g_decay, *g_curl)
+        # each gradient lives where its coefficient input does (the host float64 graph)
+        return (None, *[g.to(device=d, dtype=t) for g, (d, t) in zip(list(g_decay) + list(g_curl), ctx.coef_meta)])
 
 
 def differentiable_run(scene, region_densities=None, device="cuda", stride=None, fused=None, **plan_kwargs):

@@ -228,6 +228,7 @@ class _BatchGradientFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, session, n_per_scene, *coefs):
         ctx.session = session
+        ctx.coef_meta = [(c.device, c.dtype) for c in coefs]
         ctx.n_out = [len(session.outputs(i)) for i in range(len(session.runs))]
         outs = session.forward()
         return tuple(o.detach().clone() for scene in outs for o in scene)
@@ -246,7 +247,7 @@ class _BatchGradientFunction(torch.autograd.Function):
         flat = []
         for g_decay, g_curl in res:
             flat += list(g_decay) + list(g_curl)
-        return (None, None, *flat)
+        return (None, None, *[g.to(device=d, dtype=t) for g, (d, t) in zip(flat, ctx.coef_meta)])
 
 
 def batched_differentiable_run(scenes, plan_kwargs, device="cuda", stride=None, xchunk=0, graph_steps=1):
