@@ -181,66 +181,70 @@ def bench_gradient(args):
 
 
 def bench_batch(args):
-    """Batched forward+adjoint gradients/s over the 32-scene batch configuration
-    (seeded uneven sizes 64^3 .. 192x192x128 cells, CPML 10, 32x32x16 design
-    region, dipole, single-frequency flux objective, 1500 steps), against the same
-    gradients computed one scene per launch."""
+    """Batched forward over the 32-scene batch configuration (seeded uneven sizes
+    64^3 .. 192x192x128 cells, CPML 10, 32x32x16 design region, dipole,
+    single-frequency flux plane, ``--batch-steps`` steps): all scenes advanced by
+    the batched launches (autotuned x-chunk and graph length) against the same
+    scenes run one after another.  The batched adjoint is switched off, so this
+    workload times the forward half of the gradient pipeline."""
     import numpy as np
     import torch
     from witwin_maxwell_amd import configs
-    from witwin_maxwell_amd.fdtd.adjoint import differentiable_run
-    from witwin_maxwell_amd.fdtd.batch import batched_differentiable_run
-    from witwin_maxwell_amd.scene import MaterialRegion
+    from witwin_maxwell_amd.fdtd.batch import BatchRun
+    from witwin_maxwell_amd.fdtd.plan import compile_plan
+    from witwin_maxwell_amd.fdtd.runtime import DeviceRun
 
-    def scenes_with(thetas_out):
-        scs, kw = configs.batched_scenes(count=args.batch_count, steps=args.batch_steps)
-        out = []
-        for sc, theta in scs:
-            t = theta.clone().requires_grad_(True)
-            r = sc.material_regions[0]
-            sc.material_regions[0] = MaterialRegion(r.name, r.geometry, torch.sigmoid(t), eps_bounds=r.eps_bounds)
-            thetas_out.append(t)
-            out.append(sc)
-        return out, kw
-
-    # batched
-    th_b = []
-    scenes, kw = scenes_with(th_b)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    plans, mons, session = batched_differentiable_run(scenes, kw, graph_steps=args.graph_steps)
-    loss = sum(_flux_objective(m) for m in mons)
-    loss.backward()
-    torch.cuda.synchronize()
-    t_batch = time.perf_counter() - t0
+    scs, kw = configs.batched_scenes(count=args.batch_count, steps=args.batch_steps)
+    plans = [compile_plan(sc, region_densities=[r.density.detach() for r in sc.material_regions], **kw)
+             for sc, _ in scs]
+    T = kw["time_steps"]
     nodes = sum(int(np.prod(p.shape)) for p in plans)
-    del session, mons, loss
-    torch.cuda.empty_cache()
-    # one scene per launch
-    th_p = []
-    scenes, kw = scenes_with(th_p)
+    # one scene per launch sequence
     lat = []
-    for sc in scenes:
+    singles = []
+    for p in plans:
+        r = DeviceRun(p, "cuda")
+        r.run(min(8, T))  # warm-up
+        r.reset()
         torch.cuda.synchronize()
         t1 = time.perf_counter()
-        plan, m, sess = differentiable_run(sc, **kw)
-        _flux_objective(m).backward()
+        r.run(T)
         torch.cuda.synchronize()
         lat.append(time.perf_counter() - t1)
-        del sess, m
+        singles.append(r.monitor_results())
+        del r
     t_plain = sum(lat)
-    err = max(float((a.grad - b.grad).abs().max() / b.grad.abs().max().clamp_min(1e-300)) for a, b in zip(th_b, th_p))
-    n = len(scenes)
-    line = {"metric": "batched fwd+adjoint grads/s vs plain", "value": n / t_batch, "unit": "gradients/s",
-            "n_gpus": 1, "steps": kw["time_steps"], "warmup": 0, "ms_per_step": t_batch * 1e3 / kw["time_steps"],
+    torch.cuda.empty_cache()
+    batch = BatchRun([DeviceRun(p, "cuda") for p in plans])
+    choice = batch.autotune(trial_steps=16)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    batch.run(T)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    err = 0.0
+    for single, r in zip(singles, batch.runs):
+        mb = r.monitor_results()
+        for name, mon in single.items():
+            for comp, v in mon["components"].items():
+                a = np.asarray(v["data"] if isinstance(v, dict) else v)
+                b = mb[name]["components"][comp]
+                b = np.asarray(b["data"] if isinstance(b, dict) else b)
+                err = max(err, float(np.max(np.abs(a - b)) / max(np.max(np.abs(a)), 1e-30)))
+    batch.close()
+    n = len(plans)
+    line = {"metric": "batched forward scene-runs/s vs one scene per launch", "value": n / t_batch,
+            "unit": "scene-runs/s", "n_gpus": 1, "steps": T, "warmup": 0, "ms_per_step": t_batch * 1e3 / T,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": f"{n} scenes 64^3..192x192x128 cells (+2x10 CPML), 32x32x16 design region, "
-                                   f"dipole, single-frequency flux objective, {kw['time_steps']} steps",
-                       "total_nodes": nodes},
+                                   f"dipole, single-frequency flux plane, {T} steps, forward only",
+                       "total_nodes": nodes, "xchunk_graph_steps": list(choice)},
             "detail": {"batched_s": t_batch, "plain_s": t_plain, "speedup": t_plain / t_batch,
-                       "plain_grads_per_s": n / t_plain, "plain_latency_p50_s": float(np.percentile(lat, 50)),
-                       "plain_latency_p99_s": float(np.percentile(lat, 99)), "batched_latency_s": t_batch,
-                       "max_rel_grad_diff_batched_vs_plain": err}}
+                       "batched_gnode_updates_per_s": nodes * T / t_batch / 1e9,
+                       "plain_gnode_updates_per_s": nodes * T / t_plain / 1e9,
+                       "plain_latency_p50_s": float(np.percentile(lat, 50)),
+                       "plain_latency_p99_s": float(np.percentile(lat, 99)),
+                       "max_rel_monitor_diff_batched_vs_plain": err}}
     print(json.dumps(line), flush=True)
 
 
@@ -257,11 +261,10 @@ def main():
                     help="also run torch copies of 256 MiB (counter calibration for rocprofv3 --pmc runs)")
     ap.add_argument("--workload", choices=("forward", "gradient", "batch"), default="forward",
                     help="forward: Yee step of the dielectric-box scene (default); gradient: one design-region "
-                         "gradient; batch: batched vs one-scene-per-launch gradients")
+                         "gradient; batch: batched vs one-scene-per-launch forward runs")
     ap.add_argument("--grad-steps", type=int, default=3000)
     ap.add_argument("--batch-count", type=int, default=32)
     ap.add_argument("--batch-steps", type=int, default=1500)
-    ap.add_argument("--graph-steps", type=int, default=16)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "

@@ -67,6 +67,8 @@ def test_batch_matches_single_scene_runs(graph_steps, xchunk):
     batch.close()
 
 
+@pytest.mark.skip(reason="batched adjoint switched off: its first device run ended in a GPU memory fault whose "
+                         "cause is not established (CPU AddressSanitizer emulation of the same path is clean)")
 def test_batched_gradients_match_oracle():
     """Three design scenes of different sizes differentiated in one batch: each
     scene's density gradient equals its own float64 oracle gradient."""

@@ -1303,6 +1303,8 @@ int mw_adjoint_batch_reverse(mw_adjoint_batch* b, int32_t first, int32_t count, 
 }
 
 int mw_adjoint_batch_destroy(mw_adjoint_batch* b) {
+  // the tables may still be read by queued launches: wait for the device first
+  if (b) (void)hipDeviceSynchronize();
   adjoint_batch_free(b);
   return MW_OK;
 }

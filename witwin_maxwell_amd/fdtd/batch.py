@@ -176,8 +176,15 @@ class BatchGradientSession:
 
     def backward(self, grads_per_scene):
         """grads_per_scene[i] follows outputs(i); returns per scene the window
-        (g_decay[3], g_curl[3]) as float64 tensors."""
-        from .adjoint import DeviceAdjoint, restore
+        (g_decay[3], g_curl[3]) as float64 tensors.
+
+        Switched off: its first device run ended in a GPU memory fault whose cause
+        is not established (the same path runs clean under a CPU AddressSanitizer
+        emulation of the kernels), so it raises until that cause is found."""
+        raise NotImplementedError(
+            "the batched adjoint is switched off pending an unexplained GPU memory fault; "
+            "differentiate scenes one at a time with fdtd.adjoint.differentiable_run")
+        from .adjoint import DeviceAdjoint, restore  # noqa: E402  (kept for when the path is re-enabled)
         K, T = self.stride, self.T
         adjs, scales = [], []
         for run, win, grads in zip(self.runs, self.windows, grads_per_scene):
