@@ -257,6 +257,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--two-kernel", action="store_true", help="use the two-launch (H, E) step instead of the single pass")
+    ap.add_argument("--no-tune", action="store_true", help="keep the library's default single-pass tiling")
+    ap.add_argument("--tiling", default="", help="XCHUNK,SHELL_CHUNK: fixed single-pass tiling instead of autotuning")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run torch copies of 256 MiB (counter calibration for rocprofv3 --pmc runs)")
     ap.add_argument("--workload", choices=("forward", "gradient", "batch"), default="forward",
@@ -290,6 +292,14 @@ def main():
     compile_s = time.perf_counter() - t0
     runner = DeviceRun(plan, "cuda:0", fused=not args.two_kernel)
     nodes = int(np.prod(plan.shape))
+    tiling = None
+    if runner.fused and args.tiling:
+        tiling = tuple(int(v) for v in args.tiling.split(","))
+        runner.set_tiling(*tiling)
+    elif runner.fused and not args.no_tune:
+        # launch geometry of the single-pass step, timed on this run and reset
+        tiling = runner.autotune(trial_steps=12, cache=False)
+        print(f"[bench] tiling (xchunk, shell_chunk) = {tiling}", file=sys.stderr, flush=True)
     print(f"[bench] plan {plan.shape} compiled in {compile_s:.1f}s; warmup {args.warmup} steps", file=sys.stderr,
           flush=True)
     runner.run(args.warmup)
@@ -308,9 +318,10 @@ def main():
     step_bytes, e_bytes, h_bytes = algorithmic_bytes(plan, runner)
     reps = max(20, min(200, args.steps // 5))
     if runner.fused:
-        # the single-pass step: lean interior + general shell kernels (+ the tiny
+        # the single-pass step: PML-shell H, lean interior, PML-shell E (+ the tiny
         # per-step term prep); rocprof splits the time between them
-        dominant, t_dom, b_dom = "fused step (lean_step_kernel + fused_step_kernel)", time_fused(runner, reps), step_bytes
+        dominant, t_dom, b_dom = ("single-pass step (lean_step_kernel + shell_h_kernel + shell_e_kernel)",
+                                  time_fused(runner, reps), step_bytes)
     else:
         t_e = time_kernel(runner, "E", reps)
         t_h = time_kernel(runner, "H", reps)
@@ -344,7 +355,9 @@ def main():
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": f"dielectric boxes {args.cells}^3 cells (+2x12 CPML), 40 boxes eps U[1,12], "
                                    f"soft +z plane wave, 2 flux monitors x 8 freq",
-                       "nodes": list(plan.shape), "courant": 0.99},
+                       "nodes": list(plan.shape), "courant": 0.99,
+                       "step": "single-pass" if runner.fused else "two-kernel",
+                       "tiling": None if tiling is None else {"xchunk": tiling[0], "shell_chunk": tiling[1]}},
             "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
 

@@ -225,7 +225,9 @@ typedef struct mw_program {
   float* point_re; /* (nf, n_points) */
   float* point_im;
   /* full-field DFT of E */
-  int32_t nf_dft, pad3;
+  int32_t nf_dft;
+  /* single-pass tiling: x planes per block of the PML-shell launches (0 = automatic) */
+  int32_t shell_chunk;
   const float* dft_cos;
   const float* dft_sin;
   float* dft_re[3]; /* (nf, component numel) */
@@ -233,8 +235,9 @@ typedef struct mw_program {
   /* Single-pass mode (fused = 1): E, H and the H-side psi are double-buffered.
    * At step n the state lives in `fields` / cpml.psi[6..11] when n is even and in
    * `fields_alt` / psi_h_alt when n is odd.  Not available with periodic faces.
-   * h_terms_dev / e_terms_dev are device copies of h_terms / e_terms. */
-  int32_t fused, pad4;
+   * h_terms_dev / e_terms_dev are device copies of h_terms / e_terms.
+   * xchunk: x planes per block of the interior launch (0 = automatic). */
+  int32_t fused, xchunk;
   mw_fields fields_alt;
   float* psi_h_alt[6];
   const mw_term* h_terms_dev;

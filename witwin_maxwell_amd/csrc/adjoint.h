@@ -267,12 +267,12 @@ __device__ __forceinline__ void reverse_e_at(const RevArgs& a, int i, int j, int
 
 template <bool CPML>
 __global__ __launch_bounds__(256) void reverse_h_kernel(RevArgs a) {
-  reverse_h_at<CPML>(a, blockIdx.z, blockIdx.y * 4 + threadIdx.y, blockIdx.x * 64 + threadIdx.x);
+  reverse_h_at<CPML>(a, blockIdx.z, blockIdx.y * 4 + wave_row(), blockIdx.x * 64 + threadIdx.x);
 }
 
 template <bool CPML, bool ADE>
 __global__ __launch_bounds__(256) void reverse_e_kernel(RevArgs a) {
-  reverse_e_at<CPML, ADE>(a, blockIdx.z, blockIdx.y * 4 + threadIdx.y, blockIdx.x * 64 + threadIdx.x);
+  reverse_e_at<CPML, ADE>(a, blockIdx.z, blockIdx.y * 4 + wave_row(), blockIdx.x * 64 + threadIdx.x);
 }
 
 // Batched reverse phases: block b of the launch belongs to the scene s with
@@ -296,7 +296,7 @@ __global__ __launch_bounds__(256) void reverse_h_batch_kernel(const RevArgs* __r
   const RevArgs& a = table[s];
   const int t = b - prefix[s];
   const int bz = (a.g.nz + 63) / 64, by = (a.g.ny + 3) / 4;
-  reverse_h_at<CPML>(a, t / (bz * by), (t / bz) % by * 4 + threadIdx.y, t % bz * 64 + threadIdx.x);
+  reverse_h_at<CPML>(a, t / (bz * by), (t / bz) % by * 4 + wave_row(), t % bz * 64 + threadIdx.x);
 }
 
 template <bool CPML, bool ADE>
@@ -308,7 +308,7 @@ __global__ __launch_bounds__(256) void reverse_e_batch_kernel(const RevArgs* __r
   const RevArgs& a = table[s];
   const int t = b - prefix[s];
   const int bz = (a.g.nz + 63) / 64, by = (a.g.ny + 3) / 4;
-  reverse_e_at<CPML, ADE>(a, t / (bz * by), (t / bz) % by * 4 + threadIdx.y, t % bz * 64 + threadIdx.x);
+  reverse_e_at<CPML, ADE>(a, t / (bz * by), (t / bz) % by * 4 + wave_row(), t % bz * 64 + threadIdx.x);
 }
 
 // --------------------------------------------------------------- seeds

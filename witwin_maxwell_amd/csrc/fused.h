@@ -324,7 +324,7 @@ __device__ __forceinline__ void fused_tile(const FusedArgs& a, int step, const f
                                            const float* __restrict__ ws_t, int tz, int ty, int tx) {
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   const int lane = threadIdx.x;  // 0..63
-  const int wv = threadIdx.y;    // 0..3
+  const int wv = wave_row();  // 0..3
 
   const int kc0 = a.z_off + tz * kFOwnCols - 1;  // column of lane 0
   const int k = kc0 + lane;
@@ -498,7 +498,7 @@ __device__ __forceinline__ void lean_tile(const FusedArgs& a, int step, const fl
                                           const float* __restrict__ ws_t, int tz, int ty, int tx) {
   const int ny = a.g.ny, nz = a.g.nz;
   const int lane = threadIdx.x;
-  const int wv = threadIdx.y;
+  const int wv = wave_row();
   const int kc0 = a.z_off + tz * kFOwnCols - 1;  // column of lane 0
   const int k = kc0 + lane;
   const int j = a.y_off + ty * kFRows + wv;

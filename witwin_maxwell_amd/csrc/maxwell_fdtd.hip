@@ -12,6 +12,7 @@
 #include "adjoint.h"
 #include "batch.h"
 #include "fused.h"
+#include "shell.h"
 #include "terms.h"
 #include "yee.h"
 
@@ -223,10 +224,6 @@ int coef_mode(const mw_coeffs& c) {
   return 0;
 }
 
-template <bool CPML, bool ADE, bool DFT>
-void dispatch_general(dim3 grid, dim3 block, hipStream_t s, const mw::FusedArgs& a, int /*mode*/) {
-  hipLaunchKernelGGL((mw::fused_step_kernel<CPML, ADE, DFT>), grid, block, 0, s, a);
-}
 
 template <bool ADE, bool DFT>
 void dispatch_lean(dim3 grid, dim3 block, hipStream_t s, const mw::FusedArgs& a, int mode) {
@@ -344,32 +341,87 @@ int build_fused_args(const mw_program* p, int n, int chunk_override, mw::FusedAr
   return MW_OK;
 }
 
+// Default x planes per shell block: short marches keep the planes that the
+// blocks resident on one XCD touch at the same time few and contiguous.
+constexpr int kShellChunk = 2;
+
+// The cells outside the lean box as at most six boxes (x slabs over the whole
+// cross-section, y slabs over the lean x range, z slabs over the lean x-y range),
+// each with its lane layout and x chunk.
+void build_shell_args(const mw::FusedArgs& a, int nlean, int chunk, mw::ShellArgs* out) {
+  mw::ShellArgs& sa = *out;
+  sa.f = a;
+  sa.nboxes = 0;
+  sa.total = 0;
+  const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
+  int X0 = nx, X1 = nx, Y0 = 0, Y1 = 0, Z0 = 0, Z1 = 0;
+  if (nlean > 0) {
+    X0 = a.x_off + a.xchunk * a.lx0; X1 = a.x_off + a.xchunk * a.lx1;
+    Y0 = a.y_off + mw::kFRows * a.ly0; Y1 = a.y_off + mw::kFRows * a.ly1;
+    Z0 = a.z_off + mw::kFOwnCols * a.lz0; Z1 = a.z_off + mw::kFOwnCols * a.lz1;
+  }
+  const int boxes[6][6] = {{0, X0, 0, ny, 0, nz},  {X1, nx, 0, ny, 0, nz},  {X0, X1, 0, Y0, 0, nz},
+                           {X0, X1, Y1, ny, 0, nz}, {X0, X1, Y0, Y1, 0, Z0}, {X0, X1, Y0, Y1, Z1, nz}};
+  for (const auto& bb : boxes) {
+    const int xe = bb[1] - bb[0], ye = bb[3] - bb[2], ze = bb[5] - bb[4];
+    if (xe <= 0 || ye <= 0 || ze <= 0) continue;
+    mw::ShellBox& b = sa.box[sa.nboxes++];
+    b.lo[0] = bb[0]; b.hi[0] = bb[1];
+    b.lo[1] = bb[2]; b.hi[1] = bb[3];
+    b.lo[2] = bb[4]; b.hi[2] = bb[5];
+    b.w_shift = ze <= 16 ? 4 : (ze <= 32 ? 5 : 6);
+    const int w = 1 << b.w_shift, rows = 256 >> b.w_shift;
+    b.chunk = std::min(xe, chunk > 0 ? chunk : kShellChunk);
+    b.nbz = (ze + w - 1) / w;
+    b.nby = (ye + rows - 1) / rows;
+    b.nbx = (xe + b.chunk - 1) / b.chunk;
+    b.first = sa.total;
+    sa.total += b.nbz * b.nby * b.nbx;
+  }
+}
+
+template <bool CPML, bool ADE, bool DFT>
+int launch_shell_e(dim3 grid, hipStream_t s, const mw::ShellArgs& sa) {
+  hipLaunchKernelGGL((mw::shell_e_kernel<CPML, ADE, DFT>), grid, dim3(64, 4, 1), 0, s, sa);
+  return check_launch("fused_step (shell E)");
+}
+
 int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_fields) {
   int rc;
   if ((rc = launch_shifted_values(p->h_terms, p->n_h_terms, n, p->dt, s))) return rc;
   if ((rc = launch_shifted_values(p->e_terms, p->n_e_terms, n, p->dt, s))) return rc;
   mw::FusedArgs a;
   int nlean = 0;
-  build_fused_args(p, n, 0, &a, &nlean);
+  build_fused_args(p, n, p->xchunk, &a, &nlean);
+  mw::ShellArgs sa;
+  build_shell_args(a, nlean, p->shell_chunk, &sa);
   const bool ade = p->has_ade && (p->ade.count[0] + p->ade.count[1] + p->ade.count[2]) > 0;
-  const int nblocks = a.tiles_z * a.tiles_y * a.tiles_x;
-  dim3 block(mw::kFLanes, mw::kFRows, 1);
   const bool cpml = p->cpml.enabled != 0, dft = p->nf_dft > 0;
   const int mode = coef_mode(p->coeffs);
-  dim3 grid(((nblocks + 7) / 8) * 8);
-  if (cpml) {
-    if (ade) { if (dft) dispatch_general<true, true, true>(grid, block, s, a, mode); else dispatch_general<true, true, false>(grid, block, s, a, mode); }
-    else { if (dft) dispatch_general<true, false, true>(grid, block, s, a, mode); else dispatch_general<true, false, false>(grid, block, s, a, mode); }
-  } else {
-    if (ade) { if (dft) dispatch_general<false, true, true>(grid, block, s, a, mode); else dispatch_general<false, true, false>(grid, block, s, a, mode); }
-    else { if (dft) dispatch_general<false, false, true>(grid, block, s, a, mode); else dispatch_general<false, false, false>(grid, block, s, a, mode); }
+  const dim3 block(mw::kFLanes, mw::kFRows, 1);
+  const dim3 sgrid(((sa.total + 7) / 8) * 8);
+  // shell H and lean read only this step's inputs and write disjoint cells
+  if (sa.total > 0) {
+    if (cpml) hipLaunchKernelGGL(mw::shell_h_kernel<true>, sgrid, block, 0, s, sa);
+    else hipLaunchKernelGGL(mw::shell_h_kernel<false>, sgrid, block, 0, s, sa);
+    if ((rc = check_launch("fused_step (shell H)"))) return rc;
   }
-  if ((rc = check_launch("fused_step (general)"))) return rc;
   if (nlean > 0) {
     dim3 lgrid(((nlean + 7) / 8) * 8);
     if (ade) { if (dft) dispatch_lean<true, true>(lgrid, block, s, a, mode); else dispatch_lean<true, false>(lgrid, block, s, a, mode); }
     else { if (dft) dispatch_lean<false, true>(lgrid, block, s, a, mode); else dispatch_lean<false, false>(lgrid, block, s, a, mode); }
     if ((rc = check_launch("fused_step (lean)"))) return rc;
+  }
+  // shell E reads the complete H^{n+1/2}
+  if (sa.total > 0) {
+    if (cpml) {
+      if (ade) rc = dft ? launch_shell_e<true, true, true>(sgrid, s, sa) : launch_shell_e<true, true, false>(sgrid, s, sa);
+      else rc = dft ? launch_shell_e<true, false, true>(sgrid, s, sa) : launch_shell_e<true, false, false>(sgrid, s, sa);
+    } else {
+      if (ade) rc = dft ? launch_shell_e<false, true, true>(sgrid, s, sa) : launch_shell_e<false, true, false>(sgrid, s, sa);
+      else rc = dft ? launch_shell_e<false, false, true>(sgrid, s, sa) : launch_shell_e<false, false, false>(sgrid, s, sa);
+    }
+    if (rc) return rc;
   }
   *out_fields = a.out;
   return MW_OK;
@@ -637,6 +689,8 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
     return rc;
   if (first < 0 || count < 0 || first + count > p->total_steps)
     return fail(MW_ERR_ARGUMENT, "steps [%d, %d) outside the program's %d steps", first, first + count, p->total_steps);
+  if (p->xchunk < 0 || p->xchunk > 256 || p->shell_chunk < 0 || p->shell_chunk > 256)
+    return fail(MW_ERR_ARGUMENT, "xchunk %d / shell_chunk %d outside [0, 256]", p->xchunk, p->shell_chunk);
   if ((rc = check_terms(g, p->h_terms, p->n_h_terms, "h_terms", p->total_steps)) ||
       (rc = check_terms(g, p->e_terms, p->n_e_terms, "e_terms", p->total_steps)))
     return rc;

@@ -20,6 +20,13 @@ namespace mw {
 constexpr int kTileZ = 64;
 constexpr int kTileY = 4;
 
+// Row of the block handled by this wavefront.  Every step/reverse kernel runs
+// blocks of 64 x rows threads, so threadIdx.y is constant across a wavefront;
+// reading it through readfirstlane tells the compiler so, and everything derived
+// from the row (spacings, CPML profiles, face tests) stays in scalar registers
+// with scalar branches.
+__device__ __forceinline__ int wave_row() { return __builtin_amdgcn_readfirstlane(threadIdx.y); }
+
 // Slab index of psi sample `coord` along its axis, or -1 outside the PML layers
 // (reference kernels/common.cuh:65-78).
 __device__ __forceinline__ int slab_local(int coord, int low, int high_start, int high_len) {
@@ -70,7 +77,7 @@ template <bool CPML>
 __global__ __launch_bounds__(256) void update_h_kernel(mw_grid g, mw_coeffs cf, mw_cpml cp, mw_fields f,
                                                        int comp_mask, int xchunk) {
   const int k = blockIdx.x * kTileZ + threadIdx.x;
-  const int j = blockIdx.y * kTileY + threadIdx.y;
+  const int j = blockIdx.y * kTileY + wave_row();
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (k >= nz || j >= ny) return;
   const int i0 = blockIdx.z * xchunk;
@@ -216,7 +223,7 @@ template <bool CPML, bool ADE>
 __global__ __launch_bounds__(256) void update_e_kernel(mw_grid g, mw_coeffs cf, mw_cpml cp, mw_ade ad, mw_fields f,
                                                        int comp_mask, int xchunk) {
   const int k = blockIdx.x * kTileZ + threadIdx.x;
-  const int j = blockIdx.y * kTileY + threadIdx.y;
+  const int j = blockIdx.y * kTileY + wave_row();
   const int nx = g.nx, ny = g.ny, nz = g.nz;
   if (k >= nz || j >= ny) return;
   const int i0 = blockIdx.z * xchunk;

@@ -88,8 +88,8 @@ class MwProgram(ctypes.Structure):
                 ("pad2", ctypes.c_int32), ("obs_e_cos", vp), ("obs_e_sin", vp), ("obs_h_cos", vp),
                 ("obs_h_sin", vp), ("planes", ctypes.POINTER(MwPlaneObs)), ("point_component", vp),
                 ("point_linear", vp), ("point_re", vp), ("point_im", vp), ("nf_dft", ctypes.c_int32),
-                ("pad3", ctypes.c_int32), ("dft_cos", vp), ("dft_sin", vp), ("dft_re", vp * 3), ("dft_im", vp * 3),
-                ("fused", ctypes.c_int32), ("pad4", ctypes.c_int32), ("fields_alt", MwFields), ("psi_h_alt", vp * 6),
+                ("shell_chunk", ctypes.c_int32), ("dft_cos", vp), ("dft_sin", vp), ("dft_re", vp * 3), ("dft_im", vp * 3),
+                ("fused", ctypes.c_int32), ("xchunk", ctypes.c_int32), ("fields_alt", MwFields), ("psi_h_alt", vp * 6),
                 ("h_terms_dev", vp), ("e_terms_dev", vp), ("aux", MwAuxLine)]
 
 

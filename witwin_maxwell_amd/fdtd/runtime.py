@@ -318,6 +318,57 @@ class DeviceRun:
         self.step_index += count
         return self
 
+    def set_tiling(self, xchunk: int = 0, shell_chunk: int = 0):
+        """x planes per block of the single-pass interior and PML-shell launches
+        (0 = the library's choice)."""
+        self.program.xchunk = int(xchunk)
+        self.program.shell_chunk = int(shell_chunk)
+        return self
+
+    def autotune(self, trial_steps: int = 12, xchunks=(0, 2, 3, 4, 6, 8, 12, 16), shell_chunks=(1, 2, 4),
+                 cache=True):
+        """Pick the single-pass tiling by timing ``trial_steps`` steps of each
+        candidate on this run, then reset the run to step 0.  The choice is cached
+        per (shape, CPML, dispersive, DFT) signature in the JSON file named by
+        WITWIN_MAXWELL_AMD_TUNE_CACHE, when set.  Returns (xchunk, shell_chunk)."""
+        import json
+        import os
+        if not self.fused:
+            return (0, 0)
+        p = self.plan
+        sig = f"single|{p.shape}|{int(p.uses_cpml)}|{int(p.dispersive)}|{len(p.dft_frequencies)}"
+        path = os.environ.get("WITWIN_MAXWELL_AMD_TUNE_CACHE")
+        table = {}
+        if cache and path and os.path.exists(path):
+            with open(path) as f:
+                table = json.load(f)
+        if sig in table:
+            self.set_tiling(*table[sig])
+            return tuple(table[sig])
+        steps = min(int(trial_steps), max(1, p.time_steps - 1))
+        best, best_t = (0, 0), float("inf")
+        for xc in xchunks:
+            for sc in shell_chunks:
+                self.set_tiling(xc, sc)
+                self.reset()
+                self.run(1)  # warm-up
+                start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                torch.cuda.synchronize()
+                start.record()
+                self.run(min(steps, p.time_steps - self.step_index))
+                end.record()
+                end.synchronize()
+                t = start.elapsed_time(end)
+                if t < best_t:
+                    best, best_t = (xc, sc), t
+        self.set_tiling(*best)
+        self.reset()
+        if cache and path:
+            table[sig] = list(best)
+            with open(path, "w") as f:
+                json.dump(table, f)
+        return best
+
     def reset(self):
         for fd in self._fields:
             for t in fd.values():

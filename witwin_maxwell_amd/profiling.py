@@ -43,37 +43,51 @@ def kernel_stats(directory):
 
 
 def _short(name):
+    """Kernel family of a rocprofv3 kernel name (template arguments kept for the
+    step kernels, whose variants are timed separately)."""
+    if "__amd_rocclr_copyBuffer" in name:
+        return "copy"
+    for key in ("lean_step_kernel", "fused_step_kernel", "lean_batch_kernel", "fused_batch_kernel"):
+        if key in name:
+            return name[name.index(key):].split("(")[0]
     for key in ("update_e_kernel", "update_h_kernel", "apply_terms_kernel", "accumulate_planes_kernel",
-                "accumulate_points_kernel", "accumulate_kernel", "fused_step_kernel", "copy"):
+                "accumulate_points_kernel", "accumulate_kernel", "shifted_values_kernel"):
         if key in name:
             return key
     return name[:60]
 
 
 def counter_per_dispatch(directory, counter):
+    """Average counter value per dispatch of each kernel family; for the copy
+    probe, the largest dispatch (the probe's 256 MiB copy, not the small
+    device-to-device copies a run also makes)."""
     vals = defaultdict(list)
     for path in _find(directory, "counter_collection.csv"):
         with open(path) as fh:
             for r in csv.DictReader(fh):
                 if r.get("Counter_Name") != counter:
                     continue
-                name = r.get("Kernel_Name", "")
-                vals[_short(name) if "copy" not in name.lower() and "elementwise" not in name.lower() else "copy"].append(
-                    float(r["Counter_Value"]))
-    return {k: sum(v) / len(v) for k, v in vals.items()}
+                vals[_short(r.get("Kernel_Name", ""))].append(float(r["Counter_Value"]))
+    return {k: (max(v) if k == "copy" else sum(v) / len(v)) for k, v in vals.items()}
 
 
 def traffic(fetch_dir, write_dir):
+    """HBM bytes per dispatch: FETCH_SIZE + WRITE_SIZE (KiB) of each kernel, each
+    scaled by bytes / counter reading of the copy probe, which reads and writes
+    COPY_PROBE_BYTES."""
     fetch = counter_per_dispatch(fetch_dir, "FETCH_SIZE")
     write = counter_per_dispatch(write_dir, "WRITE_SIZE")
-    # counters are in KiB; calibrate against the copy probe (reads and writes COPY_PROBE_BYTES)
-    f_scale = COPY_PROBE_BYTES / (fetch["copy"] * 1024.0) if fetch.get("copy") else 1.0
-    w_scale = COPY_PROBE_BYTES / (write["copy"] * 1024.0) if write.get("copy") else 1.0
+    if not fetch.get("copy") or not write.get("copy"):
+        raise RuntimeError("the runs hold no copy-probe dispatch (run bench.py with --copy-probe)")
+    f_scale = COPY_PROBE_BYTES / (fetch["copy"] * 1024.0)
+    w_scale = COPY_PROBE_BYTES / (write["copy"] * 1024.0)
     out = {"_calibration": {"fetch_scale": f_scale, "write_scale": w_scale}}
     for k in fetch:
         if k == "copy":
             continue
-        out[k] = fetch[k] * 1024.0 * f_scale + write.get(k, 0.0) * 1024.0 * w_scale
+        f = fetch[k] * 1024.0 * f_scale
+        w = write.get(k, 0.0) * 1024.0 * w_scale
+        out[k] = {"fetch_bytes": f, "write_bytes": w, "total_bytes": f + w}
     return out
 
 
