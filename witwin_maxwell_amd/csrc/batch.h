@@ -10,6 +10,7 @@
 
 #include "maxwell_fdtd.h"
 #include "terms.h"
+#include "yee.h"
 
 namespace mw {
 
@@ -34,9 +35,9 @@ __global__ __launch_bounds__(256) void shifted_values_batch_kernel(const BatchTe
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= cells) return;
   const int n = *counter;
-  const double time = (double)n * b.dt + t.t_offset - t.delay[p];
-  t.values[p] = (float)(waveform(t.wave_kind, time, t.frequency, t.fwidth, t.amplitude, t.phase, t.delay0) *
-                        (double)t.patch[p]);
+  const double time = (double)n * b.dt + t.t_offset - G(t.delay)[p];
+  G(t.values)[p] = (float)(waveform(t.wave_kind, time, t.frequency, t.fwidth, t.amplitude, t.phase, t.delay0) *
+                           (double)G(t.patch)[p]);
 }
 
 __global__ __launch_bounds__(256) void aux_line_h_batch_kernel(const mw_aux_line* __restrict__ lines, int count) {
@@ -45,7 +46,7 @@ __global__ __launch_bounds__(256) void aux_line_h_batch_kernel(const mw_aux_line
   const mw_aux_line a = lines[s];
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= a.n_e - 1) return;
-  a.h[m] = a.decay_h[m] * a.h[m] - a.curl_h[m] * (a.e[m + 1] - a.e[m]);
+  G(a.h)[m] = G(a.decay_h)[m] * G(a.h)[m] - G(a.curl_h)[m] * (G(a.e)[m + 1] - G(a.e)[m]);
 }
 
 __global__ __launch_bounds__(256) void aux_line_e_batch_kernel(const mw_aux_line* __restrict__ lines, int count,
@@ -55,9 +56,9 @@ __global__ __launch_bounds__(256) void aux_line_e_batch_kernel(const mw_aux_line
   const mw_aux_line a = lines[s];
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= a.n_e) return;
-  if (m == a.n_e - 1) { a.e[m] = 0.0f; return; }
-  if (m == 0) { a.e[0] = a.source[*counter]; return; }
-  a.e[m] = a.decay_e[m] * a.e[m] - a.curl_e[m] * (a.h[m] - a.h[m - 1]);
+  if (m == a.n_e - 1) { G(a.e)[m] = 0.0f; return; }
+  if (m == 0) { G(a.e)[0] = G(a.source)[*counter]; return; }
+  G(a.e)[m] = G(a.decay_e)[m] * G(a.e)[m] - G(a.curl_e)[m] * (G(a.h)[m] - G(a.h)[m - 1]);
 }
 
 // One observer plane of one scene.  field[p] is the component buffer that holds
@@ -90,12 +91,12 @@ __global__ __launch_bounds__(256) void accumulate_planes_batch_kernel(const Batc
   if (q.axis == 0) o = ((uint32_t)q.index * sy + ia) * sz + ib;
   else if (q.axis == 1) o = ((uint32_t)ia * sy + q.index) * sz + ib;
   else o = ((uint32_t)ia * sy + ib) * sz + q.index;
-  const float x = q.field[n & 1][o];
-  const float* wc = q.w_cos + (int64_t)n * q.nf;
-  const float* ws = q.w_sin + (int64_t)n * q.nf;
+  const float x = G(q.field[n & 1])[o];
+  const auto wc = G(q.w_cos + (int64_t)n * q.nf);
+  const auto ws = G(q.w_sin + (int64_t)n * q.nf);
   for (int f = 0; f < q.nf; ++f) {
-    q.re[f * cells + l] += x * wc[f];
-    q.im[f * cells + l] += x * ws[f];
+    G(q.re)[f * cells + l] += x * wc[f];
+    G(q.im)[f * cells + l] += x * ws[f];
   }
 }
 
@@ -114,12 +115,12 @@ __global__ __launch_bounds__(64) void accumulate_points_batch_kernel(const Batch
   if (p >= count) return;
   const BatchPoint& q = points[p];
   const int n = *counter;
-  const float x = q.field[n & 1][q.linear];
-  const float* wc = q.w_cos + (int64_t)n * q.nf;
-  const float* ws = q.w_sin + (int64_t)n * q.nf;
+  const float x = G(q.field[n & 1])[q.linear];
+  const auto wc = G(q.w_cos + (int64_t)n * q.nf);
+  const auto ws = G(q.w_sin + (int64_t)n * q.nf);
   for (int f = 0; f < q.nf; ++f) {
-    q.re[(int64_t)f * q.stride] += x * wc[f];
-    q.im[(int64_t)f * q.stride] += x * ws[f];
+    G(q.re)[(int64_t)f * q.stride] += x * wc[f];
+    G(q.im)[(int64_t)f * q.stride] += x * ws[f];
   }
 }
 

@@ -467,17 +467,20 @@ int check_window(const mw_program* p, const mw_window* w) {
   return MW_OK;
 }
 
-template <bool CPML, bool ADE, bool DFT>
-void launch_batch_general(dim3 grid, hipStream_t s, const mw::BatchView& v) {
-  hipLaunchKernelGGL((mw::fused_batch_kernel<CPML, ADE, DFT>), grid, dim3(mw::kFLanes, mw::kFRows, 1), 0, s, v);
+template <bool ADE, bool DFT>
+void launch_batch_lean(dim3 grid, hipStream_t s, const mw::FusedArgs* args, const int* prefix, int count, int total,
+                       const int* counter, int mode) {
+  const dim3 block(mw::kFLanes, mw::kFRows, 1);
+  if (mode == 2) hipLaunchKernelGGL((mw::lean_batch_kernel<ADE, DFT, 2>), grid, block, 0, s, args, prefix, count, total, counter);
+  else if (mode == 1) hipLaunchKernelGGL((mw::lean_batch_kernel<ADE, DFT, 1>), grid, block, 0, s, args, prefix, count, total, counter);
+  else hipLaunchKernelGGL((mw::lean_batch_kernel<ADE, DFT, 0>), grid, block, 0, s, args, prefix, count, total, counter);
 }
 
-template <bool ADE, bool DFT>
-void launch_batch_lean(dim3 grid, hipStream_t s, const mw::BatchView& v, int mode) {
-  const dim3 block(mw::kFLanes, mw::kFRows, 1);
-  if (mode == 2) hipLaunchKernelGGL((mw::lean_batch_kernel<ADE, DFT, 2>), grid, block, 0, s, v);
-  else if (mode == 1) hipLaunchKernelGGL((mw::lean_batch_kernel<ADE, DFT, 1>), grid, block, 0, s, v);
-  else hipLaunchKernelGGL((mw::lean_batch_kernel<ADE, DFT, 0>), grid, block, 0, s, v);
+template <bool CPML, bool ADE, bool DFT>
+void launch_batch_shell_e(dim3 grid, hipStream_t s, const mw::ShellArgs* args, const int* prefix, int count, int total,
+                          const int* counter) {
+  hipLaunchKernelGGL((mw::shell_e_batch_kernel<CPML, ADE, DFT>), grid, dim3(64, 4, 1), 0, s, args, prefix, count, total,
+                     counter);
 }
 
 }  // namespace
@@ -487,10 +490,11 @@ struct mw_batch {
   int total_steps = 0;
   bool cpml = false, ade = false, dft = false;
   int mode = 0;
-  mw::FusedArgs* d_args = nullptr;
-  int* d_prefix = nullptr;
+  mw::FusedArgs* d_args = nullptr;   // 2 per scene (step parity): lean interior
+  mw::ShellArgs* d_shell = nullptr;  // 2 per scene: PML shell
+  int* d_shell_prefix = nullptr;
   int* d_lean_prefix = nullptr;
-  int total = 0, total_lean = 0;
+  int total_shell = 0, total_lean = 0;
   int* d_counter = nullptr;
   mw::BatchTerm* d_terms = nullptr;
   int n_terms = 0;
@@ -515,8 +519,8 @@ void batch_free(mw_batch* b) {
   if (!b) return;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  void* bufs[] = {b->d_args, b->d_prefix, b->d_lean_prefix, b->d_counter, b->d_terms, b->d_aux, b->d_planes,
-                  b->d_points};
+  void* bufs[] = {b->d_args, b->d_shell, b->d_shell_prefix, b->d_lean_prefix, b->d_counter, b->d_terms, b->d_aux,
+                  b->d_planes, b->d_points};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
   if (b->ev_in) (void)hipEventDestroy(b->ev_in);
@@ -549,22 +553,40 @@ int batch_step(const mw_batch* b, hipStream_t s, bool observe = true) {
                        0, s, b->d_terms, b->n_terms, b->d_counter);
     if ((rc = check_launch("batch shifted_values"))) return rc;
   }
-  mw::BatchView v{b->d_args, b->d_prefix, b->count, b->total, b->d_counter};
-  const dim3 grid(((b->total + 7) / 8) * 8);
-  if (b->cpml) {
-    if (b->ade) { if (b->dft) launch_batch_general<true, true, true>(grid, s, v); else launch_batch_general<true, true, false>(grid, s, v); }
-    else { if (b->dft) launch_batch_general<true, false, true>(grid, s, v); else launch_batch_general<true, false, false>(grid, s, v); }
-  } else {
-    if (b->ade) { if (b->dft) launch_batch_general<false, true, true>(grid, s, v); else launch_batch_general<false, true, false>(grid, s, v); }
-    else { if (b->dft) launch_batch_general<false, false, true>(grid, s, v); else launch_batch_general<false, false, false>(grid, s, v); }
+  // shell H and lean read only the step's inputs; shell E reads the complete H
+  const dim3 sgrid(((b->total_shell + 7) / 8) * 8);
+  if (b->total_shell > 0) {
+    if (b->cpml) hipLaunchKernelGGL(mw::shell_h_batch_kernel<true>, sgrid, dim3(64, 4, 1), 0, s, b->d_shell,
+                                    b->d_shell_prefix, b->count, b->total_shell, b->d_counter);
+    else hipLaunchKernelGGL(mw::shell_h_batch_kernel<false>, sgrid, dim3(64, 4, 1), 0, s, b->d_shell,
+                            b->d_shell_prefix, b->count, b->total_shell, b->d_counter);
+    if ((rc = check_launch("batch fused_step (shell H)"))) return rc;
   }
-  if ((rc = check_launch("batch fused_step (general)"))) return rc;
   if (b->total_lean > 0) {
-    mw::BatchView lv{b->d_args, b->d_lean_prefix, b->count, b->total_lean, b->d_counter};
     const dim3 lgrid(((b->total_lean + 7) / 8) * 8);
-    if (b->ade) { if (b->dft) launch_batch_lean<true, true>(lgrid, s, lv, b->mode); else launch_batch_lean<true, false>(lgrid, s, lv, b->mode); }
-    else { if (b->dft) launch_batch_lean<false, true>(lgrid, s, lv, b->mode); else launch_batch_lean<false, false>(lgrid, s, lv, b->mode); }
+    const int* lp = b->d_lean_prefix;
+    if (b->ade) {
+      if (b->dft) launch_batch_lean<true, true>(lgrid, s, b->d_args, lp, b->count, b->total_lean, b->d_counter, b->mode);
+      else launch_batch_lean<true, false>(lgrid, s, b->d_args, lp, b->count, b->total_lean, b->d_counter, b->mode);
+    } else {
+      if (b->dft) launch_batch_lean<false, true>(lgrid, s, b->d_args, lp, b->count, b->total_lean, b->d_counter, b->mode);
+      else launch_batch_lean<false, false>(lgrid, s, b->d_args, lp, b->count, b->total_lean, b->d_counter, b->mode);
+    }
     if ((rc = check_launch("batch fused_step (lean)"))) return rc;
+  }
+  if (b->total_shell > 0) {
+    const mw::ShellArgs* sa = b->d_shell;
+    const int* sp = b->d_shell_prefix;
+    const int n = b->count, t = b->total_shell;
+    const int* ctr = b->d_counter;
+    if (b->cpml) {
+      if (b->ade) { if (b->dft) launch_batch_shell_e<true, true, true>(sgrid, s, sa, sp, n, t, ctr); else launch_batch_shell_e<true, true, false>(sgrid, s, sa, sp, n, t, ctr); }
+      else { if (b->dft) launch_batch_shell_e<true, false, true>(sgrid, s, sa, sp, n, t, ctr); else launch_batch_shell_e<true, false, false>(sgrid, s, sa, sp, n, t, ctr); }
+    } else {
+      if (b->ade) { if (b->dft) launch_batch_shell_e<false, true, true>(sgrid, s, sa, sp, n, t, ctr); else launch_batch_shell_e<false, true, false>(sgrid, s, sa, sp, n, t, ctr); }
+      else { if (b->dft) launch_batch_shell_e<false, false, true>(sgrid, s, sa, sp, n, t, ctr); else launch_batch_shell_e<false, false, false>(sgrid, s, sa, sp, n, t, ctr); }
+    }
+    if ((rc = check_launch("batch fused_step (shell E)"))) return rc;
   }
   if (b->n_aux > 0) {
     hipLaunchKernelGGL(mw::aux_line_e_batch_kernel, dim3((b->max_aux + 255) / 256, b->n_aux), dim3(256), 0, s,
@@ -962,7 +984,8 @@ int mw_batch_create(const mw_program* const* programs, int32_t count, int32_t xc
   auto bail = [&](int rc) { batch_free(b); return rc; };
   int rc;
   std::vector<mw::FusedArgs> args;
-  std::vector<int> prefix(1, 0), lean_prefix(1, 0);
+  std::vector<mw::ShellArgs> shells;
+  std::vector<int> prefix(1, 0), lean_prefix(1, 0);  // shell blocks, lean tiles
   std::vector<mw::BatchTerm> terms;
   std::vector<mw_aux_line> aux;
   std::vector<mw::BatchPlane> planes;
@@ -995,12 +1018,15 @@ int mw_batch_create(const mw_program* const* programs, int32_t count, int32_t xc
     for (int par = 0; par < 2; ++par) {
       mw::FusedArgs a;
       int nlean = 0;
-      build_fused_args(p, par, xchunk, &a, &nlean);
+      build_fused_args(p, par, xchunk > 0 ? xchunk : p->xchunk, &a, &nlean);
       a.dft_cos = p->dft_cos;  // the batch kernels add step * nf
       a.dft_sin = p->dft_sin;
       args.push_back(a);
+      mw::ShellArgs sh;
+      build_shell_args(a, nlean, p->shell_chunk, &sh);
+      shells.push_back(sh);
       if (par == 0) {
-        prefix.push_back(prefix.back() + a.tiles_z * a.tiles_y * a.tiles_x);
+        prefix.push_back(prefix.back() + sh.total);
         lean_prefix.push_back(lean_prefix.back() + nlean);
       }
     }
@@ -1068,7 +1094,7 @@ int mw_batch_create(const mw_program* const* programs, int32_t count, int32_t xc
       }
     }
   }
-  b->total = prefix.back();
+  b->total_shell = prefix.back();
   b->total_lean = lean_prefix.back();
   b->n_terms = (int)terms.size();
   b->n_aux = (int)aux.size();
@@ -1076,7 +1102,8 @@ int mw_batch_create(const mw_program* const* programs, int32_t count, int32_t xc
   b->n_points = (int)points.size();
   if (b->n_terms > 65535 || b->n_aux > 65535 || b->n_planes > 65535)
     return bail(fail(MW_ERR_ARGUMENT, "batch: too many terms, lines or planes for one launch"));
-  if ((rc = upload(args, &b->d_args, "arguments")) || (rc = upload(prefix, &b->d_prefix, "tile table")) ||
+  if ((rc = upload(args, &b->d_args, "arguments")) || (rc = upload(shells, &b->d_shell, "shell arguments")) ||
+      (rc = upload(prefix, &b->d_shell_prefix, "shell block table")) ||
       (rc = upload(lean_prefix, &b->d_lean_prefix, "lean tile table")) || (rc = upload(terms, &b->d_terms, "terms")) ||
       (rc = upload(aux, &b->d_aux, "auxiliary lines")) || (rc = upload(planes, &b->d_planes, "planes")) ||
       (rc = upload(points, &b->d_points, "points")))

@@ -80,8 +80,9 @@ __device__ __forceinline__ bool shell_cell(const ShellArgs& a, int logical, Shel
 }
 
 // psi <- b*psi_in + c*d, written to psi_out (H-side ping-pong); 0 outside the slab.
-__device__ __forceinline__ float psi_pp(const float* in, float* out, int low, int hs, int hl, int axis, int coord,
-                                        int i, int j, int k, int sy, int sz, float b, float c, float d) {
+__device__ __forceinline__ float psi_pp(gptr<const float> __restrict__ in, gptr<float> __restrict__ out, int low,
+                                        int hs, int hl, int axis, int coord, int i, int j, int k, int sy, int sz,
+                                        float b, float c, float d) {
   const int local = slab_local(coord, low, hs, hl);
   if (local < 0) return 0.0f;
   const uint32_t o = slab_offset(axis, i, j, k, local, sy, sz, low + hl);
@@ -90,100 +91,162 @@ __device__ __forceinline__ float psi_pp(const float* in, float* out, int low, in
   return nv;
 }
 
+// psi <- b*psi + c*d in place (E side); 0 outside the slab.
+__device__ __forceinline__ float psi_ip(gptr<float> __restrict__ psi, int low, int hs, int hl, int axis, int coord,
+                                        int i, int j, int k, int sy, int sz, float b, float c, float d) {
+  const int local = slab_local(coord, low, hs, hl);
+  if (local < 0) return 0.0f;
+  const uint32_t o = slab_offset(axis, i, j, k, local, sy, sz, low + hl);
+  const float nv = b * psi[o] + c * d;
+  psi[o] = nv;
+  return nv;
+}
+
+__device__ __forceinline__ float coef_r(gptr<const float> __restrict__ arr, float value, uint32_t o) {
+  return arr ? arr[o] : value;
+}
+
+// The shell kernels take every buffer as a restrict-qualified parameter of an
+// inlined function, so the compiler may issue all loads of a plane before its
+// stores instead of waiting for each store in case they alias.
+
 template <bool CPML>
-__global__ __launch_bounds__(256) void shell_h_kernel(ShellArgs sa) {
-  ShellCell c;
-  if (!shell_cell(sa, xcd_logical_block(), c)) return;
+__device__ __forceinline__ void shell_h_cells(
+    const ShellArgs& sa, const ShellCell& c, int step, gptr<const float> __restrict__ ex,
+    gptr<const float> __restrict__ ey, gptr<const float> __restrict__ ez, gptr<const float> __restrict__ hxi,
+    gptr<const float> __restrict__ hyi, gptr<const float> __restrict__ hzi, gptr<float> __restrict__ hxo,
+    gptr<float> __restrict__ hyo, gptr<float> __restrict__ hzo, gptr<const float> __restrict__ pi0,
+    gptr<const float> __restrict__ pi1, gptr<const float> __restrict__ pi2, gptr<const float> __restrict__ pi3,
+    gptr<const float> __restrict__ pi4, gptr<const float> __restrict__ pi5, gptr<float> __restrict__ po0,
+    gptr<float> __restrict__ po1, gptr<float> __restrict__ po2, gptr<float> __restrict__ po3,
+    gptr<float> __restrict__ po4, gptr<float> __restrict__ po5) {
   const FusedArgs& a = sa.f;
+  const mw_cpml& cp = a.cp;
   const ShellBox& bx = sa.box[c.b];
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   const int j = c.j, k = c.k;
   const bool inside = j < bx.hi[1] && k < bx.hi[2];
   const bool jv = inside && j < ny - 1, kv = inside && k < nz - 1;
-  const float* __restrict__ ex = a.in.e[0];
-  const float* __restrict__ ey = a.in.e[1];
-  const float* __restrict__ ez = a.in.e[2];
-  const float ihy = jv ? a.g.inv_h[1][j] : 0.0f;
-  const float ihz = kv ? a.g.inv_h[2][k] : 0.0f;
+  // Every load of a plane is unconditional, at offset 0 when the sample does not
+  // exist (the value is then discarded), so all of them issue before the plane's
+  // stores; only the stores are predicated.
+  const float ihy = G(a.g.inv_h[1])[jv ? j : 0];
+  const float ihz = G(a.g.inv_h[2])[kv ? k : 0];
   float bhy = 0, chy = 0, khy = 1, bhz = 0, chz = 0, khz = 1;
   if (CPML) {
-    if (jv) { bhy = a.cp.b_h[1][j]; chy = a.cp.c_h[1][j]; khy = a.cp.inv_kappa_h[1][j]; }
-    if (kv) { bhz = a.cp.b_h[2][k]; chz = a.cp.c_h[2][k]; khz = a.cp.inv_kappa_h[2][k]; }
+    bhy = G(cp.b_h[1])[jv ? j : 0]; chy = G(cp.c_h[1])[jv ? j : 0]; khy = G(cp.inv_kappa_h[1])[jv ? j : 0];
+    bhz = G(cp.b_h[2])[kv ? k : 0]; chz = G(cp.c_h[2])[kv ? k : 0]; khz = G(cp.inv_kappa_h[2])[kv ? k : 0];
   }
-  // carried x-neighbours Ez(i, j, k), Ey(i, j, k)
-  float ez_c = (kv && inside) ? ez[((uint32_t)c.i0 * ny + j) * (nz - 1) + k] : 0.0f;
-  float ey_c = (jv && inside) ? ey[((uint32_t)c.i0 * (ny - 1) + j) * nz + k] : 0.0f;
+  // psi slab positions that do not depend on the plane (-1 outside the slab)
+  const int l6 = slab_local(j, cp.psi_low[6], cp.psi_high_start[6], cp.psi_high_len[6]);
+  const int l7 = slab_local(k, cp.psi_low[7], cp.psi_high_start[7], cp.psi_high_len[7]);
+  const int l9 = slab_local(k, cp.psi_low[9], cp.psi_high_start[9], cp.psi_high_len[9]);
+  const int l11 = slab_local(j, cp.psi_low[11], cp.psi_high_start[11], cp.psi_high_len[11]);
+  const uint32_t sEx = (uint32_t)ny * nz, sEy = (uint32_t)(ny - 1) * nz, sEz = (uint32_t)ny * (nz - 1);
+  const uint32_t sHx = (uint32_t)(ny - 1) * (nz - 1), sHy = (uint32_t)ny * (nz - 1), sHz = (uint32_t)(ny - 1) * nz;
+  float ez_c = ez[kv ? (uint32_t)c.i0 * sEz + (uint32_t)j * (nz - 1) + k : 0];
+  float ey_c = ey[jv ? (uint32_t)c.i0 * sEy + (uint32_t)j * nz + k : 0];
   for (int i = c.i0; i < c.i1; ++i) {
     const bool ht = terms_touch(a.h_terms, a.n_h_terms, i, c.jrow0, c.jrow1, c.krow0, c.krow1);
     const bool iv = i < nx - 1;
-    float ez_n = 0.0f, ey_n = 0.0f;
-    if (iv && inside) {
-      if (kv) ez_n = ez[((uint32_t)(i + 1) * ny + j) * (nz - 1) + k];
-      if (jv) ey_n = ey[((uint32_t)(i + 1) * (ny - 1) + j) * nz + k];
+    const bool dx = jv && kv, dy = iv && kv, dz = iv && jv;  // Hx, Hy, Hz exist here
+    const uint32_t ox = dx ? i * sHx + (uint32_t)j * (nz - 1) + k : 0;
+    const uint32_t oy = dy ? i * sHy + (uint32_t)j * (nz - 1) + k : 0;
+    const uint32_t oz = dz ? i * sHz + (uint32_t)j * nz + k : 0;
+    const uint32_t oex = (iv && inside) ? i * sEx + (uint32_t)j * nz + k : 0;
+    // loads
+    const float ez_n = ez[dy ? (i + 1) * sEz + (uint32_t)j * (nz - 1) + k : 0];
+    const float ey_n = ey[dz ? (i + 1) * sEy + (uint32_t)j * nz + k : 0];
+    const float ez_jp = ez[dx ? i * sEz + (uint32_t)(j + 1) * (nz - 1) + k : 0];
+    const float ey_kp = ey[dx ? i * sEy + (uint32_t)j * nz + k + 1 : 0];
+    const float ex_c = ex[oex];
+    const float ex_kp = ex[dy ? oex + 1 : 0];
+    const float ex_jp = ex[dz ? oex + nz : 0];
+    const float hx_old = hxi[ox], hy_old = hyi[oy], hz_old = hzi[oz];
+    const float ihx = G(a.g.inv_h[0])[iv ? i : 0];
+    float bhx = 0, chx = 0, khx = 1;
+    float p6 = 0, p7 = 0, p8 = 0, p9 = 0, p10 = 0, p11 = 0;
+    uint32_t q6 = 0, q7 = 0, q8 = 0, q9 = 0, q10 = 0, q11 = 0;
+    int l8 = -1, l10 = -1;
+    if (CPML) {
+      bhx = G(cp.b_h[0])[iv ? i : 0]; chx = G(cp.c_h[0])[iv ? i : 0]; khx = G(cp.inv_kappa_h[0])[iv ? i : 0];
+      l8 = slab_local(i, cp.psi_low[8], cp.psi_high_start[8], cp.psi_high_len[8]);
+      l10 = slab_local(i, cp.psi_low[10], cp.psi_high_start[10], cp.psi_high_len[10]);
+      if (dx && l6 >= 0) q6 = slab_offset(1, i, j, k, l6, ny - 1, nz - 1, cp.psi_low[6] + cp.psi_high_len[6]);
+      if (dx && l7 >= 0) q7 = slab_offset(2, i, j, k, l7, ny - 1, nz - 1, cp.psi_low[7] + cp.psi_high_len[7]);
+      if (dy && l8 >= 0) q8 = slab_offset(0, i, j, k, l8, ny, nz - 1, cp.psi_low[8] + cp.psi_high_len[8]);
+      if (dy && l9 >= 0) q9 = slab_offset(2, i, j, k, l9, ny, nz - 1, cp.psi_low[9] + cp.psi_high_len[9]);
+      if (dz && l10 >= 0) q10 = slab_offset(0, i, j, k, l10, ny - 1, nz, cp.psi_low[10] + cp.psi_high_len[10]);
+      if (dz && l11 >= 0) q11 = slab_offset(1, i, j, k, l11, ny - 1, nz, cp.psi_low[11] + cp.psi_high_len[11]);
+      // an axis without PML layers has no psi buffer: the test is uniform
+      if (pi0) p6 = pi0[q6];
+      if (pi1) p7 = pi1[q7];
+      if (pi2) p8 = pi2[q8];
+      if (pi3) p9 = pi3[q9];
+      if (pi4) p10 = pi4[q10];
+      if (pi5) p11 = pi5[q11];
     }
-    if (jv && kv) {  // Hx (nx, ny-1, nz-1): a = dEz/dy, b = dEy/dz, curl = a - b
-      const uint32_t o = ((uint32_t)i * (ny - 1) + j) * (nz - 1) + k;
-      const float da = (ez[((uint32_t)i * ny + j + 1) * (nz - 1) + k] - ez_c) * ihy;
-      const float db = (ey[((uint32_t)i * (ny - 1) + j) * nz + k + 1] - ey_c) * ihz;
-      float curl;
+    // Hx: a = dEz/dy, b = dEy/dz, curl = a - b
+    {
+      const float da = (ez_jp - ez_c) * ihy, db = (ey_kp - ey_c) * ihz;
+      float curl = da - db;
       if (CPML) {
-        const float pa = psi_pp(a.cp.psi[6], a.psi_h_out[0], a.cp.psi_low[6], a.cp.psi_high_start[6],
-                                a.cp.psi_high_len[6], 1, j, i, j, k, ny - 1, nz - 1, bhy, chy, da);
-        const float pb = psi_pp(a.cp.psi[7], a.psi_h_out[1], a.cp.psi_low[7], a.cp.psi_high_start[7],
-                                a.cp.psi_high_len[7], 2, k, i, j, k, ny - 1, nz - 1, bhz, chz, db);
+        const float pa = l6 >= 0 ? bhy * p6 + chy * da : 0.0f;
+        const float pb = l7 >= 0 ? bhz * p7 + chz * db : 0.0f;
+        if (dx && l6 >= 0) po0[q6] = pa;
+        if (dx && l7 >= 0) po1[q7] = pb;
         curl = (da * khy + pa) - (db * khz + pb);
-      } else {
-        curl = da - db;
       }
-      float h = a.in.h[0][o] * coef(a.cf.decay[3], a.cf.decay_value[3], o) - coef(a.cf.curl[3], a.cf.curl_value[3], o) * curl;
-      if (ht) h += term_value(a.h_terms, a.n_h_terms, 3, i, j, k, a.step, a.dt);
-      a.out.h[0][o] = h;
+      float h = hx_old * coef(a.cf.decay[3], a.cf.decay_value[3], ox) - coef(a.cf.curl[3], a.cf.curl_value[3], ox) * curl;
+      if (ht && dx) h += term_value(a.h_terms, a.n_h_terms, 3, i, j, k, step, a.dt);
+      if (dx) hxo[ox] = h;
     }
-    if (iv && inside && (kv || jv)) {
-      const float ihx = a.g.inv_h[0][i];
-      float bhx = 0, chx = 0, khx = 1;
-      if (CPML) { bhx = a.cp.b_h[0][i]; chx = a.cp.c_h[0][i]; khx = a.cp.inv_kappa_h[0][i]; }
-      const float ex_c = ex[((uint32_t)i * ny + j) * nz + k];
-      if (kv) {  // Hy (nx-1, ny, nz-1): a = dEz/dx, b = dEx/dz, curl = b - a
-        const uint32_t o = ((uint32_t)i * ny + j) * (nz - 1) + k;
-        const float da = (ez_n - ez_c) * ihx;
-        const float db = (ex[((uint32_t)i * ny + j) * nz + k + 1] - ex_c) * ihz;
-        float curl;
-        if (CPML) {
-          const float pa = psi_pp(a.cp.psi[8], a.psi_h_out[2], a.cp.psi_low[8], a.cp.psi_high_start[8],
-                                  a.cp.psi_high_len[8], 0, i, i, j, k, ny, nz - 1, bhx, chx, da);
-          const float pb = psi_pp(a.cp.psi[9], a.psi_h_out[3], a.cp.psi_low[9], a.cp.psi_high_start[9],
-                                  a.cp.psi_high_len[9], 2, k, i, j, k, ny, nz - 1, bhz, chz, db);
-          curl = (db * khz + pb) - (da * khx + pa);
-        } else {
-          curl = db - da;
-        }
-        float h = a.in.h[1][o] * coef(a.cf.decay[4], a.cf.decay_value[4], o) - coef(a.cf.curl[4], a.cf.curl_value[4], o) * curl;
-        if (ht) h += term_value(a.h_terms, a.n_h_terms, 4, i, j, k, a.step, a.dt);
-        a.out.h[1][o] = h;
+    // Hy: a = dEz/dx, b = dEx/dz, curl = b - a
+    {
+      const float da = (ez_n - ez_c) * ihx, db = (ex_kp - ex_c) * ihz;
+      float curl = db - da;
+      if (CPML) {
+        const float pa = l8 >= 0 ? bhx * p8 + chx * da : 0.0f;
+        const float pb = l9 >= 0 ? bhz * p9 + chz * db : 0.0f;
+        if (dy && l8 >= 0) po2[q8] = pa;
+        if (dy && l9 >= 0) po3[q9] = pb;
+        curl = (db * khz + pb) - (da * khx + pa);
       }
-      if (jv) {  // Hz (nx-1, ny-1, nz): a = dEy/dx, b = dEx/dy, curl = a - b
-        const uint32_t o = ((uint32_t)i * (ny - 1) + j) * nz + k;
-        const float da = (ey_n - ey_c) * ihx;
-        const float db = (ex[((uint32_t)i * ny + j + 1) * nz + k] - ex_c) * ihy;
-        float curl;
-        if (CPML) {
-          const float pa = psi_pp(a.cp.psi[10], a.psi_h_out[4], a.cp.psi_low[10], a.cp.psi_high_start[10],
-                                  a.cp.psi_high_len[10], 0, i, i, j, k, ny - 1, nz, bhx, chx, da);
-          const float pb = psi_pp(a.cp.psi[11], a.psi_h_out[5], a.cp.psi_low[11], a.cp.psi_high_start[11],
-                                  a.cp.psi_high_len[11], 1, j, i, j, k, ny - 1, nz, bhy, chy, db);
-          curl = (da * khx + pa) - (db * khy + pb);
-        } else {
-          curl = da - db;
-        }
-        float h = a.in.h[2][o] * coef(a.cf.decay[5], a.cf.decay_value[5], o) - coef(a.cf.curl[5], a.cf.curl_value[5], o) * curl;
-        if (ht) h += term_value(a.h_terms, a.n_h_terms, 5, i, j, k, a.step, a.dt);
-        a.out.h[2][o] = h;
+      float h = hy_old * coef(a.cf.decay[4], a.cf.decay_value[4], oy) - coef(a.cf.curl[4], a.cf.curl_value[4], oy) * curl;
+      if (ht && dy) h += term_value(a.h_terms, a.n_h_terms, 4, i, j, k, step, a.dt);
+      if (dy) hyo[oy] = h;
+    }
+    // Hz: a = dEy/dx, b = dEx/dy, curl = a - b
+    {
+      const float da = (ey_n - ey_c) * ihx, db = (ex_jp - ex_c) * ihy;
+      float curl = da - db;
+      if (CPML) {
+        const float pa = l10 >= 0 ? bhx * p10 + chx * da : 0.0f;
+        const float pb = l11 >= 0 ? bhy * p11 + chy * db : 0.0f;
+        if (dz && l10 >= 0) po4[q10] = pa;
+        if (dz && l11 >= 0) po5[q11] = pb;
+        curl = (da * khx + pa) - (db * khy + pb);
       }
+      float h = hz_old * coef(a.cf.decay[5], a.cf.decay_value[5], oz) - coef(a.cf.curl[5], a.cf.curl_value[5], oz) * curl;
+      if (ht && dz) h += term_value(a.h_terms, a.n_h_terms, 5, i, j, k, step, a.dt);
+      if (dz) hzo[oz] = h;
     }
     ez_c = ez_n;
     ey_c = ey_n;
   }
+}
+
+// H of the shell cells of block `logical` for step `step`.
+template <bool CPML>
+__device__ __forceinline__ void shell_h_tile(const ShellArgs& sa, int logical, int step) {
+  ShellCell c;
+  if (!shell_cell(sa, logical, c)) return;
+  const FusedArgs& a = sa.f;
+  shell_h_cells<CPML>(sa, c, step, G(a.in.e[0]), G(a.in.e[1]), G(a.in.e[2]), G(a.in.h[0]), G(a.in.h[1]),
+                      G(a.in.h[2]), G(a.out.h[0]), G(a.out.h[1]), G(a.out.h[2]), G(a.cp.psi[6]), G(a.cp.psi[7]),
+                      G(a.cp.psi[8]), G(a.cp.psi[9]), G(a.cp.psi[10]), G(a.cp.psi[11]), G(a.psi_h_out[0]),
+                      G(a.psi_h_out[1]), G(a.psi_h_out[2]), G(a.psi_h_out[3]), G(a.psi_h_out[4]),
+                      G(a.psi_h_out[5]));
 }
 
 // Derivative at a boundary sample of the single-pass step: PMC mirrors (the
@@ -194,139 +257,238 @@ __device__ __forceinline__ float shell_edge(bool low_end, int mode, float first,
 }
 
 template <bool DFT>
-__device__ __forceinline__ void shell_dft(const FusedArgs& a, int c, uint64_t n, uint32_t o, float e) {
+__device__ __forceinline__ void shell_dft(const FusedArgs& a, gptr<float> __restrict__ re, gptr<float> __restrict__ im,
+                                          uint64_t n, uint32_t o, float e, gptr<const float> __restrict__ wc,
+                                          gptr<const float> __restrict__ ws) {
   if (!DFT) return;
   for (int f = 0; f < a.nf_dft; ++f) {
-    a.dft_re[c][f * n + o] += e * a.dft_cos[f];
-    a.dft_im[c][f * n + o] += e * a.dft_sin[f];
+    re[f * n + o] += e * wc[f];
+    im[f * n + o] += e * ws[f];
   }
 }
 
 template <bool CPML, bool ADE, bool DFT>
-__global__ __launch_bounds__(256) void shell_e_kernel(ShellArgs sa) {
-  ShellCell c;
-  if (!shell_cell(sa, xcd_logical_block(), c)) return;
+__device__ __forceinline__ void shell_e_cells(
+    const ShellArgs& sa, const ShellCell& c, int step, gptr<const float> __restrict__ wc,
+    gptr<const float> __restrict__ ws, gptr<const float> __restrict__ hx, gptr<const float> __restrict__ hy,
+    gptr<const float> __restrict__ hz, gptr<const float> __restrict__ exi, gptr<const float> __restrict__ eyi,
+    gptr<const float> __restrict__ ezi, gptr<float> __restrict__ exo, gptr<float> __restrict__ eyo,
+    gptr<float> __restrict__ ezo, gptr<float> __restrict__ ps0, gptr<float> __restrict__ ps1,
+    gptr<float> __restrict__ ps2, gptr<float> __restrict__ ps3, gptr<float> __restrict__ ps4,
+    gptr<float> __restrict__ ps5, gptr<const float> __restrict__ dx_, gptr<const float> __restrict__ dy_,
+    gptr<const float> __restrict__ dz_, gptr<const float> __restrict__ cx_, gptr<const float> __restrict__ cy_,
+    gptr<const float> __restrict__ cz_, gptr<float> __restrict__ dre0, gptr<float> __restrict__ dim0,
+    gptr<float> __restrict__ dre1, gptr<float> __restrict__ dim1, gptr<float> __restrict__ dre2,
+    gptr<float> __restrict__ dim2) {
   const FusedArgs& a = sa.f;
+  const mw_cpml& cp = a.cp;
   const ShellBox& bx = sa.box[c.b];
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
   const int j = c.j, k = c.k;
-  if (j >= bx.hi[1] || k >= bx.hi[2]) {
-    // idle lanes still take part in nothing below (no shuffles in this kernel)
-    return;
-  }
-  const float* __restrict__ hx = a.out.h[0];  // H^{n+1/2}, complete after shell_h and lean
-  const float* __restrict__ hy = a.out.h[1];
-  const float* __restrict__ hz = a.out.h[2];
+  if (j >= bx.hi[1] || k >= bx.hi[2]) return;  // idle lane (no shuffles in this kernel)
   const int lx = a.g.bc_lo[0], hxm = a.g.bc_hi[0], ly = a.g.bc_lo[1], hym = a.g.bc_hi[1], lz = a.g.bc_lo[2],
             hzm = a.g.bc_hi[2];
   const bool jin = j > 0 && j < ny - 1;
   const bool kin = k > 0 && k < nz - 1;
-  const float iey = a.g.inv_e[1][j];
-  const float iez = a.g.inv_e[2][k];
+  const float iey = G(a.g.inv_e[1])[j];
+  const float iez = G(a.g.inv_e[2])[k];
   float bey = 0, cey = 0, key = 1, bez = 0, cez = 0, kez = 1;
   if (CPML) {
-    bey = a.cp.b_e[1][j]; cey = a.cp.c_e[1][j]; key = a.cp.inv_kappa_e[1][j];
-    bez = a.cp.b_e[2][k]; cez = a.cp.c_e[2][k]; kez = a.cp.inv_kappa_e[2][k];
+    bey = G(cp.b_e[1])[j]; cey = G(cp.c_e[1])[j]; key = G(cp.inv_kappa_e[1])[j];
+    bez = G(cp.b_e[2])[k]; cez = G(cp.c_e[2])[k]; kez = G(cp.inv_kappa_e[2])[k];
   }
   const bool jH = j < ny - 1, kH = k < nz - 1;
   const uint32_t sEx = (uint32_t)ny * nz, sEy = (uint32_t)(ny - 1) * nz, sEz = (uint32_t)ny * (nz - 1);
-  // carried Hz(i-1, j, k), Hy(i-1, j, k)
-  float hz_m = 0.0f, hy_m = 0.0f;
-  if (c.i0 > 0 && c.i0 - 1 < nx - 1) {
-    if (jH) hz_m = hz[((uint32_t)(c.i0 - 1) * (ny - 1) + j) * nz + k];
-    if (kH) hy_m = hy[((uint32_t)(c.i0 - 1) * ny + j) * (nz - 1) + k];
-  }
+  const uint32_t sHx = (uint32_t)(ny - 1) * (nz - 1), sHy = (uint32_t)ny * (nz - 1), sHz = (uint32_t)(ny - 1) * nz;
+  // psi slab positions that do not depend on the plane (-1 outside the slab)
+  const int l0 = slab_local(j, cp.psi_low[0], cp.psi_high_start[0], cp.psi_high_len[0]);
+  const int l1 = slab_local(k, cp.psi_low[1], cp.psi_high_start[1], cp.psi_high_len[1]);
+  const int l3 = slab_local(k, cp.psi_low[3], cp.psi_high_start[3], cp.psi_high_len[3]);
+  const int l5 = slab_local(j, cp.psi_low[5], cp.psi_high_start[5], cp.psi_high_len[5]);
+  // carried Hz(i-1, j, k), Hy(i-1, j, k); every load below is unconditional (offset
+  // 0 for a sample that does not exist, whose value is discarded)
+  const bool pv = c.i0 > 0 && c.i0 - 1 < nx - 1;
+  float hz_m = hz[(pv && jH) ? (c.i0 - 1) * sHz + (uint32_t)j * nz + k : 0];
+  float hy_m = hy[(pv && kH) ? (c.i0 - 1) * sHy + (uint32_t)j * (nz - 1) + k : 0];
+  if (!(pv && jH)) hz_m = 0.0f;
+  if (!(pv && kH)) hy_m = 0.0f;
   for (int i = c.i0; i < c.i1; ++i) {
     const bool et = terms_touch(a.e_terms, a.n_e_terms, i, c.jrow0, c.jrow1, c.krow0, c.krow1);
     const bool iH = i < nx - 1;
-    const float hz_c = (iH && jH) ? hz[((uint32_t)i * (ny - 1) + j) * nz + k] : 0.0f;
-    const float hy_c = (iH && kH) ? hy[((uint32_t)i * ny + j) * (nz - 1) + k] : 0.0f;
-    const float hx_c = (jH && kH) ? hx[((uint32_t)i * (ny - 1) + j) * (nz - 1) + k] : 0.0f;
     const bool iin = i > 0 && i < nx - 1;
+    const uint32_t ox = iH ? i * sEx + (uint32_t)j * nz + k : 0;
+    const uint32_t oy = jH ? i * sEy + (uint32_t)j * nz + k : 0;
+    const uint32_t oz = kH ? i * sEz + (uint32_t)j * (nz - 1) + k : 0;
+    const uint32_t ohz = i * sHz + (uint32_t)j * nz + k;        // valid when iH && jH
+    const uint32_t ohy = i * sHy + (uint32_t)j * (nz - 1) + k;  // valid when iH && kH
+    const uint32_t ohx = i * sHx + (uint32_t)j * (nz - 1) + k;  // valid when jH && kH
+    // loads
+    float hz_c = hz[(iH && jH) ? ohz : 0];
+    float hy_c = hy[(iH && kH) ? ohy : 0];
+    float hx_c = hx[(jH && kH) ? ohx : 0];
+    float hz_jm = hz[(iH && j > 0) ? ohz - nz : 0];
+    float hy_km = hy[(iH && k > 0) ? ohy - 1 : 0];
+    float hx_km = hx[(jH && k > 0) ? ohx - 1 : 0];
+    float hx_jm = hx[(kH && j > 0) ? ohx - (nz - 1) : 0];
+    const float exv = exi[ox], eyv = eyi[oy], ezv = ezi[oz];
+    const float cdx = dx_ ? dx_[ox] : a.cf.decay_value[0], ccx = cx_ ? cx_[ox] : a.cf.curl_value[0];
+    const float cdy = dy_ ? dy_[oy] : a.cf.decay_value[1], ccy = cy_ ? cy_[oy] : a.cf.curl_value[1];
+    const float cdz = dz_ ? dz_[oz] : a.cf.decay_value[2], ccz = cz_ ? cz_[oz] : a.cf.curl_value[2];
+    const float iex = G(a.g.inv_e[0])[i];
+    float bex = 0, cex = 0, kex = 1;
+    float p0 = 0, p1 = 0, p2 = 0, p3 = 0, p4 = 0, p5 = 0;
+    uint32_t q0 = 0, q1 = 0, q2 = 0, q3 = 0, q4 = 0, q5 = 0;
+    int l2 = -1, l4 = -1;
+    if (CPML) {
+      bex = G(cp.b_e[0])[i]; cex = G(cp.c_e[0])[i]; kex = G(cp.inv_kappa_e[0])[i];
+      l2 = slab_local(i, cp.psi_low[2], cp.psi_high_start[2], cp.psi_high_len[2]);
+      l4 = slab_local(i, cp.psi_low[4], cp.psi_high_start[4], cp.psi_high_len[4]);
+      if (iH && l0 >= 0) q0 = slab_offset(1, i, j, k, l0, ny, nz, cp.psi_low[0] + cp.psi_high_len[0]);
+      if (iH && l1 >= 0) q1 = slab_offset(2, i, j, k, l1, ny, nz, cp.psi_low[1] + cp.psi_high_len[1]);
+      if (jH && l2 >= 0) q2 = slab_offset(0, i, j, k, l2, ny - 1, nz, cp.psi_low[2] + cp.psi_high_len[2]);
+      if (jH && l3 >= 0) q3 = slab_offset(2, i, j, k, l3, ny - 1, nz, cp.psi_low[3] + cp.psi_high_len[3]);
+      if (kH && l4 >= 0) q4 = slab_offset(0, i, j, k, l4, ny, nz - 1, cp.psi_low[4] + cp.psi_high_len[4]);
+      if (kH && l5 >= 0) q5 = slab_offset(1, i, j, k, l5, ny, nz - 1, cp.psi_low[5] + cp.psi_high_len[5]);
+      // an axis without PML layers has no psi buffer: the test is uniform
+      if (ps0) p0 = ps0[q0];
+      if (ps1) p1 = ps1[q1];
+      if (ps2) p2 = ps2[q2];
+      if (ps3) p3 = ps3[q3];
+      if (ps4) p4 = ps4[q4];
+      if (ps5) p5 = ps5[q5];
+    }
+    if (!(iH && jH)) hz_c = 0.0f;
+    if (!(iH && kH)) hy_c = 0.0f;
+    if (!(jH && kH)) hx_c = 0.0f;
+    if (!(iH && j > 0)) hz_jm = 0.0f;
+    if (!(iH && k > 0)) hy_km = 0.0f;
+    if (!(jH && k > 0)) hx_km = 0.0f;
+    if (!(kH && j > 0)) hx_jm = 0.0f;
     if (iH) {  // Ex (nx-1, ny, nz): a = dHz/dy, b = dHy/dz, curl = a - b
-      const uint32_t o = i * sEx + (uint32_t)j * nz + k;
-      const float e_old = a.in.e[0][o];
-      const float jsum = ADE ? ade_advance(a.ad, 0, o, e_old) : 0.0f;
+      const float jsum = ADE ? ade_advance(a.ad, 0, ox, exv) : 0.0f;
       const bool pec = face_pec(j, ny, ly, hym) || face_pec(k, nz, lz, hzm);
-      float e_new = e_old;
+      float e_new = exv;
       if (!pec && !(face_inactive(j, ny, ly, hym) || face_inactive(k, nz, lz, hzm))) {
-        const float hz_jm = j > 0 ? hz[((uint32_t)i * (ny - 1) + j - 1) * nz + k] : 0.0f;
-        const float hy_km = k > 0 ? hy[((uint32_t)i * ny + j) * (nz - 1) + k - 1] : 0.0f;
         const float da = jin ? (hz_c - hz_jm) * iey : shell_edge(j == 0, j == 0 ? ly : hym, hz_c, hz_jm, iey);
         const float db = kin ? (hy_c - hy_km) * iez : shell_edge(k == 0, k == 0 ? lz : hzm, hy_c, hy_km, iez);
-        float curl;
+        float curl = da - db;
         if (CPML) {
-          const float pa = psi_step(psi_view(a.cp, 0), 1, j, i, j, k, ny, nz, bey, cey, da);
-          const float pb = psi_step(psi_view(a.cp, 1), 2, k, i, j, k, ny, nz, bez, cez, db);
+          const float pa = l0 >= 0 ? bey * p0 + cey * da : 0.0f;
+          const float pb = l1 >= 0 ? bez * p1 + cez * db : 0.0f;
+          if (l0 >= 0) ps0[q0] = pa;
+          if (l1 >= 0) ps1[q1] = pb;
           curl = (da * key + pa) - (db * kez + pb);
-        } else {
-          curl = da - db;
         }
-        e_new = e_old * coef(a.cf.decay[0], a.cf.decay_value[0], o) + coef(a.cf.curl[0], a.cf.curl_value[0], o) * curl;
+        e_new = exv * cdx + ccx * curl;
       }
-      if (ADE) e_new -= a.ad.dt * jsum * a.ad.inv_eps[0][o];
-      if (et) e_new += term_value(a.e_terms, a.n_e_terms, 0, i, j, k, a.step, a.dt);
+      if (ADE) e_new -= a.ad.dt * jsum * G(a.ad.inv_eps[0])[ox];
+      if (et) e_new += term_value(a.e_terms, a.n_e_terms, 0, i, j, k, step, a.dt);
       if (pec) e_new = 0.0f;
-      a.out.e[0][o] = e_new;
-      shell_dft<DFT>(a, 0, (uint64_t)sEx * (nx - 1), o, e_new);
+      exo[ox] = e_new;
+      shell_dft<DFT>(a, dre0, dim0, (uint64_t)sEx * (nx - 1), ox, e_new, wc, ws);
     }
     if (jH) {  // Ey (nx, ny-1, nz): a = dHz/dx, b = dHx/dz, curl = b - a
-      const uint32_t o = i * sEy + (uint32_t)j * nz + k;
-      const float e_old = a.in.e[1][o];
-      const float jsum = ADE ? ade_advance(a.ad, 1, o, e_old) : 0.0f;
+      const float jsum = ADE ? ade_advance(a.ad, 1, oy, eyv) : 0.0f;
       const bool pec = face_pec(i, nx, lx, hxm) || face_pec(k, nz, lz, hzm);
-      float e_new = e_old;
+      float e_new = eyv;
       if (!pec && !(face_inactive(i, nx, lx, hxm) || face_inactive(k, nz, lz, hzm))) {
-        const float iex = a.g.inv_e[0][i];
-        const float hx_km = k > 0 ? hx[((uint32_t)i * (ny - 1) + j) * (nz - 1) + k - 1] : 0.0f;
         const float da = iin ? (hz_c - hz_m) * iex : shell_edge(i == 0, i == 0 ? lx : hxm, hz_c, hz_m, iex);
         const float db = kin ? (hx_c - hx_km) * iez : shell_edge(k == 0, k == 0 ? lz : hzm, hx_c, hx_km, iez);
-        float curl;
+        float curl = db - da;
         if (CPML) {
-          const float pa = psi_step(psi_view(a.cp, 2), 0, i, i, j, k, ny - 1, nz, a.cp.b_e[0][i], a.cp.c_e[0][i], da);
-          const float pb = psi_step(psi_view(a.cp, 3), 2, k, i, j, k, ny - 1, nz, bez, cez, db);
-          curl = (db * kez + pb) - (da * a.cp.inv_kappa_e[0][i] + pa);
-        } else {
-          curl = db - da;
+          const float pa = l2 >= 0 ? bex * p2 + cex * da : 0.0f;
+          const float pb = l3 >= 0 ? bez * p3 + cez * db : 0.0f;
+          if (l2 >= 0) ps2[q2] = pa;
+          if (l3 >= 0) ps3[q3] = pb;
+          curl = (db * kez + pb) - (da * kex + pa);
         }
-        e_new = e_old * coef(a.cf.decay[1], a.cf.decay_value[1], o) + coef(a.cf.curl[1], a.cf.curl_value[1], o) * curl;
+        e_new = eyv * cdy + ccy * curl;
       }
-      if (ADE) e_new -= a.ad.dt * jsum * a.ad.inv_eps[1][o];
-      if (et) e_new += term_value(a.e_terms, a.n_e_terms, 1, i, j, k, a.step, a.dt);
+      if (ADE) e_new -= a.ad.dt * jsum * G(a.ad.inv_eps[1])[oy];
+      if (et) e_new += term_value(a.e_terms, a.n_e_terms, 1, i, j, k, step, a.dt);
       if (pec) e_new = 0.0f;
-      a.out.e[1][o] = e_new;
-      shell_dft<DFT>(a, 1, (uint64_t)sEy * nx, o, e_new);
+      eyo[oy] = e_new;
+      shell_dft<DFT>(a, dre1, dim1, (uint64_t)sEy * nx, oy, e_new, wc, ws);
     }
     if (kH) {  // Ez (nx, ny, nz-1): a = dHy/dx, b = dHx/dy, curl = a - b
-      const uint32_t o = i * sEz + (uint32_t)j * (nz - 1) + k;
-      const float e_old = a.in.e[2][o];
-      const float jsum = ADE ? ade_advance(a.ad, 2, o, e_old) : 0.0f;
+      const float jsum = ADE ? ade_advance(a.ad, 2, oz, ezv) : 0.0f;
       const bool pec = face_pec(i, nx, lx, hxm) || face_pec(j, ny, ly, hym);
-      float e_new = e_old;
+      float e_new = ezv;
       if (!pec && !(face_inactive(i, nx, lx, hxm) || face_inactive(j, ny, ly, hym))) {
-        const float iex = a.g.inv_e[0][i];
-        const float hx_jm = j > 0 ? hx[((uint32_t)i * (ny - 1) + j - 1) * (nz - 1) + k] : 0.0f;
         const float da = iin ? (hy_c - hy_m) * iex : shell_edge(i == 0, i == 0 ? lx : hxm, hy_c, hy_m, iex);
         const float db = jin ? (hx_c - hx_jm) * iey : shell_edge(j == 0, j == 0 ? ly : hym, hx_c, hx_jm, iey);
-        float curl;
+        float curl = da - db;
         if (CPML) {
-          const float pa = psi_step(psi_view(a.cp, 4), 0, i, i, j, k, ny, nz - 1, a.cp.b_e[0][i], a.cp.c_e[0][i], da);
-          const float pb = psi_step(psi_view(a.cp, 5), 1, j, i, j, k, ny, nz - 1, bey, cey, db);
-          curl = (da * a.cp.inv_kappa_e[0][i] + pa) - (db * key + pb);
-        } else {
-          curl = da - db;
+          const float pa = l4 >= 0 ? bex * p4 + cex * da : 0.0f;
+          const float pb = l5 >= 0 ? bey * p5 + cey * db : 0.0f;
+          if (l4 >= 0) ps4[q4] = pa;
+          if (l5 >= 0) ps5[q5] = pb;
+          curl = (da * kex + pa) - (db * key + pb);
         }
-        e_new = e_old * coef(a.cf.decay[2], a.cf.decay_value[2], o) + coef(a.cf.curl[2], a.cf.curl_value[2], o) * curl;
+        e_new = ezv * cdz + ccz * curl;
       }
-      if (ADE) e_new -= a.ad.dt * jsum * a.ad.inv_eps[2][o];
-      if (et) e_new += term_value(a.e_terms, a.n_e_terms, 2, i, j, k, a.step, a.dt);
+      if (ADE) e_new -= a.ad.dt * jsum * G(a.ad.inv_eps[2])[oz];
+      if (et) e_new += term_value(a.e_terms, a.n_e_terms, 2, i, j, k, step, a.dt);
       if (pec) e_new = 0.0f;
-      a.out.e[2][o] = e_new;
-      shell_dft<DFT>(a, 2, (uint64_t)sEz * nx, o, e_new);
+      ezo[oz] = e_new;
+      shell_dft<DFT>(a, dre2, dim2, (uint64_t)sEz * nx, oz, e_new, wc, ws);
     }
     hz_m = hz_c;
     hy_m = hy_c;
   }
+}
+
+// E of the shell cells of block `logical` for step `step` with that step's DFT
+// weights wc / ws.
+template <bool CPML, bool ADE, bool DFT>
+__device__ __forceinline__ void shell_e_tile(const ShellArgs& sa, int logical, int step, const float* wc,
+                                             const float* ws) {
+  ShellCell c;
+  if (!shell_cell(sa, logical, c)) return;
+  const FusedArgs& a = sa.f;
+  shell_e_cells<CPML, ADE, DFT>(sa, c, step, G(wc), G(ws), G(a.out.h[0]), G(a.out.h[1]), G(a.out.h[2]), G(a.in.e[0]),
+                                G(a.in.e[1]), G(a.in.e[2]), G(a.out.e[0]), G(a.out.e[1]), G(a.out.e[2]),
+                                G(a.cp.psi[0]), G(a.cp.psi[1]), G(a.cp.psi[2]), G(a.cp.psi[3]), G(a.cp.psi[4]),
+                                G(a.cp.psi[5]), G(a.cf.decay[0]), G(a.cf.decay[1]), G(a.cf.decay[2]),
+                                G(a.cf.curl[0]), G(a.cf.curl[1]), G(a.cf.curl[2]), G(a.dft_re[0]), G(a.dft_im[0]),
+                                G(a.dft_re[1]), G(a.dft_im[1]), G(a.dft_re[2]), G(a.dft_im[2]));
+}
+
+template <bool CPML>
+__global__ __launch_bounds__(256) void shell_h_kernel(ShellArgs sa) {
+  shell_h_tile<CPML>(sa, xcd_logical_block(), sa.f.step);
+}
+
+template <bool CPML, bool ADE, bool DFT>
+__global__ __launch_bounds__(256) void shell_e_kernel(ShellArgs sa) {
+  shell_e_tile<CPML, ADE, DFT>(sa, xcd_logical_block(), sa.f.step, sa.f.dft_cos, sa.f.dft_sin);
+}
+
+// Batches: args[2 s + p] is scene s's shell for steps of parity p (dft_cos /
+// dft_sin the base of its weight tables), prefix[s] its first block; the step
+// index comes from the device counter.
+template <bool CPML>
+__global__ __launch_bounds__(256) void shell_h_batch_kernel(const ShellArgs* __restrict__ args,
+                                                            const int* __restrict__ prefix, int nscenes, int total,
+                                                            const int* __restrict__ counter) {
+  const int logical = xcd_logical_block();
+  if (logical >= total) return;
+  const int s = table_scene(prefix, nscenes, logical);
+  const int n = *counter;
+  shell_h_tile<CPML>(args[2 * s + (n & 1)], logical - prefix[s], n);
+}
+
+template <bool CPML, bool ADE, bool DFT>
+__global__ __launch_bounds__(256) void shell_e_batch_kernel(const ShellArgs* __restrict__ args,
+                                                            const int* __restrict__ prefix, int nscenes, int total,
+                                                            const int* __restrict__ counter) {
+  const int logical = xcd_logical_block();
+  if (logical >= total) return;
+  const int s = table_scene(prefix, nscenes, logical);
+  const int n = *counter;
+  const ShellArgs& sa = args[2 * s + (n & 1)];
+  const int64_t w = (int64_t)n * sa.f.nf_dft;
+  shell_e_tile<CPML, ADE, DFT>(sa, logical - prefix[s], n, sa.f.dft_cos + w, sa.f.dft_sin + w);
 }
 
 }  // namespace mw

@@ -27,6 +27,21 @@ constexpr int kTileY = 4;
 // with scalar branches.
 __device__ __forceinline__ int wave_row() { return __builtin_amdgcn_readfirstlane(threadIdx.y); }
 
+// Global-memory views.  Pointers read out of argument tables in device memory
+// (the scene batches) are generic to the compiler, which then emits flat loads
+// and stores (64-bit VGPR addresses, counted against both the vector- and the
+// scalar-memory counters).  Every buffer the kernels dereference is global
+// memory, so each dereference goes through an address_space(1) view, which
+// gives global_load/global_store with an SGPR base, as for pointers passed
+// directly as kernel arguments.  Only device buffers are viewed this way, never
+// the address of a kernel-argument struct member.
+template <class T>
+using gptr = __attribute__((address_space(1))) T*;
+template <class T>
+__device__ __forceinline__ gptr<T> G(T* p) {
+  return (gptr<T>)p;
+}
+
 // Slab index of psi sample `coord` along its axis, or -1 outside the PML layers
 // (reference kernels/common.cuh:65-78).
 __device__ __forceinline__ int slab_local(int coord, int low, int high_start, int high_len) {
@@ -58,13 +73,13 @@ __device__ __forceinline__ float psi_step(const PsiView& v, int axis, int coord,
   const int local = slab_local(coord, v.low, v.high_start, v.high_len);
   if (local < 0) return 0.0f;
   const uint32_t o = slab_offset(axis, i, j, k, local, sy, sz, v.low + v.high_len);
-  const float nv = b * v.data[o] + c * d;
-  v.data[o] = nv;
+  const float nv = b * G(v.data)[o] + c * d;
+  G(v.data)[o] = nv;
   return nv;
 }
 
 __device__ __forceinline__ float coef(const float* arr, float value, uint32_t o) {
-  return arr ? arr[o] : value;
+  return arr ? G(arr)[o] : value;
 }
 
 // ---------------------------------------------------------------- magnetic
@@ -189,24 +204,24 @@ __device__ __forceinline__ float ade_advance(const mw_ade& a, int c, uint32_t o,
   const int n = a.count[c];
   for (int p = 0; p < n; ++p) {
     const mw_pole& pl = a.poles[c][p];
-    const float drive = pl.drive[o];
+    const float drive = G(pl.drive)[o];
     if (pl.kind == 0) {
-      const float prev = pl.polarization[o];
+      const float prev = G(pl.polarization)[o];
       const float next = pl.decay * prev + drive * e_old;
-      pl.polarization[o] = next;
+      G(pl.polarization)[o] = next;
       const float cur = (next - prev) * (1.0f / a.dt);
-      pl.current[o] = cur;
+      G(pl.current)[o] = cur;
       jsum += cur;
     } else if (pl.kind == 1) {
-      const float cur = pl.decay * pl.current[o] + drive * e_old;
-      pl.current[o] = cur;
+      const float cur = pl.decay * G(pl.current)[o] + drive * e_old;
+      G(pl.current)[o] = cur;
       jsum += cur;
     } else {
-      const float pc = pl.current[o];
-      const float pp = pl.polarization[o];
+      const float pc = G(pl.current)[o];
+      const float pp = G(pl.polarization)[o];
       const float nc = pl.decay * pc - pl.restoring * pp + drive * e_old;
-      pl.current[o] = nc;
-      pl.polarization[o] = pp + a.dt * nc;
+      G(pl.current)[o] = nc;
+      G(pl.polarization)[o] = pp + a.dt * nc;
       jsum += nc;
     }
   }
