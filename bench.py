@@ -258,7 +258,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--two-kernel", action="store_true", help="use the two-launch (H, E) step instead of the single pass")
     ap.add_argument("--no-tune", action="store_true", help="keep the library's default single-pass tiling")
-    ap.add_argument("--tiling", default="", help="XCHUNK,SHELL_CHUNK: fixed single-pass tiling instead of autotuning")
+    ap.add_argument("--tiling", default="",
+                    help="XCHUNK,SHELL_CHUNK[,Z_TILING]: fixed single-pass tiling instead of autotuning")
     ap.add_argument("--copy-probe", action="store_true",
                     help="also run torch copies of 256 MiB (counter calibration for rocprofv3 --pmc runs)")
     ap.add_argument("--workload", choices=("forward", "gradient", "batch"), default="forward",
@@ -299,7 +300,7 @@ def main():
     elif runner.fused and not args.no_tune:
         # launch geometry of the single-pass step, timed on this run and reset
         tiling = runner.autotune(trial_steps=12, cache=False)
-        print(f"[bench] tiling (xchunk, shell_chunk) = {tiling}", file=sys.stderr, flush=True)
+        print(f"[bench] tiling (xchunk, shell_chunk, z_tiling) = {tiling}", file=sys.stderr, flush=True)
     print(f"[bench] plan {plan.shape} compiled in {compile_s:.1f}s; warmup {args.warmup} steps", file=sys.stderr,
           flush=True)
     runner.run(args.warmup)
@@ -357,7 +358,7 @@ def main():
                                    f"soft +z plane wave, 2 flux monitors x 8 freq",
                        "nodes": list(plan.shape), "courant": 0.99,
                        "step": "single-pass" if runner.fused else "two-kernel",
-                       "tiling": None if tiling is None else {"xchunk": tiling[0], "shell_chunk": tiling[1]}},
+                       "tiling": None if tiling is None else dict(zip(("xchunk", "shell_chunk", "z_tiling"), tiling))},
             "roofline": roofline, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
 

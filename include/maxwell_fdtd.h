@@ -214,7 +214,11 @@ typedef struct mw_program {
   const mw_term* h_terms;
   const mw_term* e_terms;
   /* observers: weight tables are (total_steps, nf) float32 in device memory */
-  int32_t nf_obs, n_planes, n_points, pad2;
+  int32_t nf_obs, n_planes, n_points;
+  /* single-pass tiling of the z ends: 0 or 1 = z-edge tiles beside the interior
+   * tiles, 2 = interior tiles aligned to the z PML slabs with the z ends in the
+   * per-cell shell */
+  int32_t z_tiling;
   const float* obs_e_cos;
   const float* obs_e_sin;
   const float* obs_h_cos;

@@ -18,7 +18,7 @@ CSRC = os.path.join(PKG, "csrc")
 LIBDIR = os.path.join(PKG, "_lib")
 LIBNAME = "libmaxwell_fdtd.so"
 SOURCES = ["maxwell_fdtd.hip"]
-HEADERS = ["yee.h", "terms.h", "fused.h", "shell.h", "adjoint.h", "batch.h"]
+HEADERS = ["yee.h", "terms.h", "fused.h", "shell.h", "zedge.h", "adjoint.h", "batch.h"]
 
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",

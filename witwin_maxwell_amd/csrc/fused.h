@@ -62,9 +62,11 @@ struct FusedArgs {
   // halo), rows y_off + 4 ty .. +3 and planes x_off + xchunk tx .. +xchunk-1; the
   // offsets align the lean box below with the PML interfaces
   int z_off, y_off, x_off;
-  // tiles [lz0, lz1) x [ly0, ly1) x [lx0, lx1) run the lean kernel; the general
-  // kernel skips them
+  // tiles [lz0, lz1) x [ly0, ly1) x [lx0, lx1) run the lean kernel
   int lz0, lz1, ly0, ly1, lx0, lx1;
+  // 1: the z tiles outside [lz0, lz1) over the lean y-x range are z-edge tiles
+  // (zedge.h); 0: those columns belong to the per-cell shell (shell.h)
+  int zedge;
 };
 
 __device__ __forceinline__ float lane_up(float v) { return __shfl_up(v, 1, 64); }      // lane-1's value

@@ -14,6 +14,7 @@
 #include "fused.h"
 #include "shell.h"
 #include "terms.h"
+#include "zedge.h"
 #include "yee.h"
 
 namespace {
@@ -251,6 +252,36 @@ AxisTiling tile_axis(int size, int width, int lead, int lo, int hi) {
   return t;
 }
 
+// z tiling of the single-pass step: every tile is either a lean tile (all of
+// its samples, the halo column before them included, inside [lo, hi)) or a
+// z-edge tile (zedge.h), which costs more per tile.  Pick the origin among the
+// `width` possible ones with the fewest tiles, counting a z-edge tile as 1.3 and
+// preferring any origin that leaves at least one lean tile.
+AxisTiling tile_axis_z(int size, int width, int lo, int hi) {
+  AxisTiling best{0, 0, 0, 0};
+  double best_cost = 1e30;
+  for (int off = -(width - 1); off <= 0; ++off) {
+    const int count = (size - off + width - 1) / width;
+    int t0 = -1, t1 = -1;
+    for (int t = 0; t < count; ++t) {
+      const int first = off + width * t, last = first + width - 1;
+      if (first - 1 >= lo && last <= hi - 1) {
+        if (t0 < 0) t0 = t;
+        t1 = t + 1;
+      }
+    }
+    if (t0 < 0) t0 = t1 = 0;
+    // without a lean tile there are no z-edge tiles either: every column then goes
+    // to the per-cell shell, the dearest path, so such an origin is a last resort
+    const double cost = t1 > t0 ? (t1 - t0) + 1.3 * (count - (t1 - t0)) : 1e9 + count;
+    if (cost < best_cost - 1e-9) {
+      best_cost = cost;
+      best = AxisTiling{off, count, t0, t1};
+    }
+  }
+  return best;
+}
+
 int launch_shifted_values(const mw_term* t, int n, int step, double dt, hipStream_t s) {
   mw::TermSet set{};
   int64_t cells = 0;
@@ -306,7 +337,8 @@ int build_fused_args(const mw_program* p, int n, int chunk_override, mw::FusedAr
   const int klo = std::max({1, low(7), low(1)}), khi = std::min({g->nz - 1, high(7, g->nz - 1), high(1, g->nz)});
   // z: owned columns start one after the halo lane; rows: row j-1 is recomputed
   // (jlo already accounts for it); planes: one halo plane before each chunk
-  const AxisTiling tz = tile_axis(g->nz, mw::kFOwnCols, 1, klo, khi);
+  const bool zedge = p->z_tiling != 2;
+  const AxisTiling tz = zedge ? tile_axis_z(g->nz, mw::kFOwnCols, klo, khi) : tile_axis(g->nz, mw::kFOwnCols, 1, klo, khi);
   const AxisTiling ty = tile_axis(g->ny, mw::kFRows, 0, jlo, jhi);
   const int cols = tz.count * ty.count;
   const int want_chunks = std::max(1, (4096 + cols - 1) / cols);
@@ -335,6 +367,7 @@ int build_fused_args(const mw_program* p, int n, int chunk_override, mw::FusedAr
   a.lz0 = tz.lean0; a.lz1 = tz.lean1;
   a.ly0 = ty.lean0; a.ly1 = ty.lean1;
   a.lx0 = tx.lean0; a.lx1 = tx.lean1;
+  a.zedge = zedge ? 1 : 0;
   int nlean = (a.lz1 - a.lz0) * (a.ly1 - a.ly0) * (a.lx1 - a.lx0);
   if (nlean <= 0) { nlean = 0; a.lz0 = a.lz1 = a.ly0 = a.ly1 = a.lx0 = a.lx1 = 0; }
   *nlean_out = nlean;
@@ -345,9 +378,11 @@ int build_fused_args(const mw_program* p, int n, int chunk_override, mw::FusedAr
 // blocks resident on one XCD touch at the same time few and contiguous.
 constexpr int kShellChunk = 2;
 
-// The cells outside the lean box as at most six boxes (x slabs over the whole
-// cross-section, y slabs over the lean x range, z slabs over the lean x-y range),
-// each with its lane layout and x chunk.
+// The cells outside the lean box (and its z-edge tiles, when the z ends are
+// tiled that way) as at most six boxes: x slabs over the whole cross-section, y
+// slabs over the lean x range and, without z-edge tiles, z slabs over the lean
+// x-y range; each with its lane layout and x chunk.  Without a lean box, one box
+// holds the whole grid.
 void build_shell_args(const mw::FusedArgs& a, int nlean, int chunk, mw::ShellArgs* out) {
   mw::ShellArgs& sa = *out;
   sa.f = a;
@@ -360,9 +395,12 @@ void build_shell_args(const mw::FusedArgs& a, int nlean, int chunk, mw::ShellArg
     Y0 = a.y_off + mw::kFRows * a.ly0; Y1 = a.y_off + mw::kFRows * a.ly1;
     Z0 = a.z_off + mw::kFOwnCols * a.lz0; Z1 = a.z_off + mw::kFOwnCols * a.lz1;
   }
+  // with a lean box, its z ends [0, Z0) and [Z1, nz) over the lean x-y range are
+  // the z-edge tiles (zedge.h), not shell boxes
   const int boxes[6][6] = {{0, X0, 0, ny, 0, nz},  {X1, nx, 0, ny, 0, nz},  {X0, X1, 0, Y0, 0, nz},
                            {X0, X1, Y1, ny, 0, nz}, {X0, X1, Y0, Y1, 0, Z0}, {X0, X1, Y0, Y1, Z1, nz}};
-  for (const auto& bb : boxes) {
+  for (int bi = 0; bi < (a.zedge ? 4 : 6); ++bi) {
+    const int* bb = boxes[bi];
     const int xe = bb[1] - bb[0], ye = bb[3] - bb[2], ze = bb[5] - bb[4];
     if (xe <= 0 || ye <= 0 || ze <= 0) continue;
     mw::ShellBox& b = sa.box[sa.nboxes++];
@@ -402,8 +440,8 @@ int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_field
   const dim3 sgrid(((sa.total + 7) / 8) * 8);
   // shell H and lean read only this step's inputs and write disjoint cells
   if (sa.total > 0) {
-    if (cpml) hipLaunchKernelGGL(mw::shell_h_kernel<true>, sgrid, block, 0, s, sa);
-    else hipLaunchKernelGGL(mw::shell_h_kernel<false>, sgrid, block, 0, s, sa);
+    if (cpml) hipLaunchKernelGGL(mw::shell_h_kernel<true>, sgrid, dim3(64, 4, 1), 0, s, sa);
+    else hipLaunchKernelGGL(mw::shell_h_kernel<false>, sgrid, dim3(64, 4, 1), 0, s, sa);
     if ((rc = check_launch("fused_step (shell H)"))) return rc;
   }
   if (nlean > 0) {
@@ -411,6 +449,13 @@ int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_field
     if (ade) { if (dft) dispatch_lean<true, true>(lgrid, block, s, a, mode); else dispatch_lean<true, false>(lgrid, block, s, a, mode); }
     else { if (dft) dispatch_lean<false, true>(lgrid, block, s, a, mode); else dispatch_lean<false, false>(lgrid, block, s, a, mode); }
     if ((rc = check_launch("fused_step (lean)"))) return rc;
+  }
+  const int nze = mw::zedge_tiles(a);
+  if (nze > 0) {
+    const dim3 zgrid(((nze + 7) / 8) * 8);
+    if (ade) { if (dft) hipLaunchKernelGGL((mw::zedge_step_kernel<true, true>), zgrid, block, 0, s, a); else hipLaunchKernelGGL((mw::zedge_step_kernel<true, false>), zgrid, block, 0, s, a); }
+    else { if (dft) hipLaunchKernelGGL((mw::zedge_step_kernel<false, true>), zgrid, block, 0, s, a); else hipLaunchKernelGGL((mw::zedge_step_kernel<false, false>), zgrid, block, 0, s, a); }
+    if ((rc = check_launch("fused_step (z edges)"))) return rc;
   }
   // shell E reads the complete H^{n+1/2}
   if (sa.total > 0) {
@@ -494,7 +539,8 @@ struct mw_batch {
   mw::ShellArgs* d_shell = nullptr;  // 2 per scene: PML shell
   int* d_shell_prefix = nullptr;
   int* d_lean_prefix = nullptr;
-  int total_shell = 0, total_lean = 0;
+  int* d_zedge_prefix = nullptr;
+  int total_shell = 0, total_lean = 0, total_zedge = 0;
   int* d_counter = nullptr;
   mw::BatchTerm* d_terms = nullptr;
   int n_terms = 0;
@@ -519,8 +565,8 @@ void batch_free(mw_batch* b) {
   if (!b) return;
   if (b->exec) (void)hipGraphExecDestroy(b->exec);
   if (b->graph) (void)hipGraphDestroy(b->graph);
-  void* bufs[] = {b->d_args, b->d_shell, b->d_shell_prefix, b->d_lean_prefix, b->d_counter, b->d_terms, b->d_aux,
-                  b->d_planes, b->d_points};
+  void* bufs[] = {b->d_args, b->d_shell, b->d_shell_prefix, b->d_lean_prefix, b->d_zedge_prefix, b->d_counter,
+                  b->d_terms, b->d_aux, b->d_planes, b->d_points};
   for (void* q : bufs)
     if (q) (void)hipFree(q);
   if (b->ev_in) (void)hipEventDestroy(b->ev_in);
@@ -573,6 +619,18 @@ int batch_step(const mw_batch* b, hipStream_t s, bool observe = true) {
       else launch_batch_lean<false, false>(lgrid, s, b->d_args, lp, b->count, b->total_lean, b->d_counter, b->mode);
     }
     if ((rc = check_launch("batch fused_step (lean)"))) return rc;
+  }
+  if (b->total_zedge > 0) {
+    const dim3 zgrid(((b->total_zedge + 7) / 8) * 8), zb(mw::kFLanes, mw::kFRows, 1);
+    const int* zp = b->d_zedge_prefix;
+    if (b->ade) {
+      if (b->dft) hipLaunchKernelGGL((mw::zedge_batch_kernel<true, true>), zgrid, zb, 0, s, b->d_args, zp, b->count, b->total_zedge, b->d_counter);
+      else hipLaunchKernelGGL((mw::zedge_batch_kernel<true, false>), zgrid, zb, 0, s, b->d_args, zp, b->count, b->total_zedge, b->d_counter);
+    } else {
+      if (b->dft) hipLaunchKernelGGL((mw::zedge_batch_kernel<false, true>), zgrid, zb, 0, s, b->d_args, zp, b->count, b->total_zedge, b->d_counter);
+      else hipLaunchKernelGGL((mw::zedge_batch_kernel<false, false>), zgrid, zb, 0, s, b->d_args, zp, b->count, b->total_zedge, b->d_counter);
+    }
+    if ((rc = check_launch("batch fused_step (z edges)"))) return rc;
   }
   if (b->total_shell > 0) {
     const mw::ShellArgs* sa = b->d_shell;
@@ -713,6 +771,7 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
     return fail(MW_ERR_ARGUMENT, "steps [%d, %d) outside the program's %d steps", first, first + count, p->total_steps);
   if (p->xchunk < 0 || p->xchunk > 256 || p->shell_chunk < 0 || p->shell_chunk > 256)
     return fail(MW_ERR_ARGUMENT, "xchunk %d / shell_chunk %d outside [0, 256]", p->xchunk, p->shell_chunk);
+  if (p->z_tiling < 0 || p->z_tiling > 2) return fail(MW_ERR_ARGUMENT, "z_tiling %d outside [0, 2]", p->z_tiling);
   if ((rc = check_terms(g, p->h_terms, p->n_h_terms, "h_terms", p->total_steps)) ||
       (rc = check_terms(g, p->e_terms, p->n_e_terms, "e_terms", p->total_steps)))
     return rc;
@@ -985,7 +1044,7 @@ int mw_batch_create(const mw_program* const* programs, int32_t count, int32_t xc
   int rc;
   std::vector<mw::FusedArgs> args;
   std::vector<mw::ShellArgs> shells;
-  std::vector<int> prefix(1, 0), lean_prefix(1, 0);  // shell blocks, lean tiles
+  std::vector<int> prefix(1, 0), lean_prefix(1, 0), zedge_prefix(1, 0);  // shell blocks, lean / z-edge tiles
   std::vector<mw::BatchTerm> terms;
   std::vector<mw_aux_line> aux;
   std::vector<mw::BatchPlane> planes;
@@ -1028,6 +1087,7 @@ int mw_batch_create(const mw_program* const* programs, int32_t count, int32_t xc
       if (par == 0) {
         prefix.push_back(prefix.back() + sh.total);
         lean_prefix.push_back(lean_prefix.back() + nlean);
+        zedge_prefix.push_back(zedge_prefix.back() + mw::zedge_tiles(a));
       }
     }
     for (int f = 0; f < 2; ++f) {
@@ -1096,6 +1156,7 @@ int mw_batch_create(const mw_program* const* programs, int32_t count, int32_t xc
   }
   b->total_shell = prefix.back();
   b->total_lean = lean_prefix.back();
+  b->total_zedge = zedge_prefix.back();
   b->n_terms = (int)terms.size();
   b->n_aux = (int)aux.size();
   b->n_planes = (int)planes.size();
@@ -1104,6 +1165,7 @@ int mw_batch_create(const mw_program* const* programs, int32_t count, int32_t xc
     return bail(fail(MW_ERR_ARGUMENT, "batch: too many terms, lines or planes for one launch"));
   if ((rc = upload(args, &b->d_args, "arguments")) || (rc = upload(shells, &b->d_shell, "shell arguments")) ||
       (rc = upload(prefix, &b->d_shell_prefix, "shell block table")) ||
+      (rc = upload(zedge_prefix, &b->d_zedge_prefix, "z-edge tile table")) ||
       (rc = upload(lean_prefix, &b->d_lean_prefix, "lean tile table")) || (rc = upload(terms, &b->d_terms, "terms")) ||
       (rc = upload(aux, &b->d_aux, "auxiliary lines")) || (rc = upload(planes, &b->d_planes, "planes")) ||
       (rc = upload(points, &b->d_points, "points")))

@@ -85,7 +85,7 @@ class MwProgram(ctypes.Structure):
                 ("n_h_terms", ctypes.c_int32), ("n_e_terms", ctypes.c_int32),
                 ("h_terms", ctypes.POINTER(MwTerm)), ("e_terms", ctypes.POINTER(MwTerm)),
                 ("nf_obs", ctypes.c_int32), ("n_planes", ctypes.c_int32), ("n_points", ctypes.c_int32),
-                ("pad2", ctypes.c_int32), ("obs_e_cos", vp), ("obs_e_sin", vp), ("obs_h_cos", vp),
+                ("z_tiling", ctypes.c_int32), ("obs_e_cos", vp), ("obs_e_sin", vp), ("obs_h_cos", vp),
                 ("obs_h_sin", vp), ("planes", ctypes.POINTER(MwPlaneObs)), ("point_component", vp),
                 ("point_linear", vp), ("point_re", vp), ("point_im", vp), ("nf_dft", ctypes.c_int32),
                 ("shell_chunk", ctypes.c_int32), ("dft_cos", vp), ("dft_sin", vp), ("dft_re", vp * 3), ("dft_im", vp * 3),

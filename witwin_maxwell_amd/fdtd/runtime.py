@@ -318,23 +318,27 @@ class DeviceRun:
         self.step_index += count
         return self
 
-    def set_tiling(self, xchunk: int = 0, shell_chunk: int = 0):
-        """x planes per block of the single-pass interior and PML-shell launches
-        (0 = the library's choice)."""
+    def set_tiling(self, xchunk: int = 0, shell_chunk: int = 0, z_tiling: int = 0):
+        """Single-pass tiling: x planes per block of the interior and PML-shell
+        launches (0 = the library's choice) and how the z ends are tiled (0 or 1:
+        z-edge tiles beside the interior tiles; 2: interior tiles aligned to the z
+        PML slabs, z ends in the per-cell shell)."""
         self.program.xchunk = int(xchunk)
         self.program.shell_chunk = int(shell_chunk)
+        self.program.z_tiling = int(z_tiling)
         return self
 
-    def autotune(self, trial_steps: int = 12, xchunks=(0, 2, 3, 4, 6, 8, 12, 16), shell_chunks=(1, 2, 4),
+    def autotune(self, trial_steps: int = 12, xchunks=(0, 3, 4, 6, 8), shell_chunks=(2, 4), z_tilings=(1, 2),
                  cache=True):
         """Pick the single-pass tiling by timing ``trial_steps`` steps of each
         candidate on this run, then reset the run to step 0.  The choice is cached
         per (shape, CPML, dispersive, DFT) signature in the JSON file named by
-        WITWIN_MAXWELL_AMD_TUNE_CACHE, when set.  Returns (xchunk, shell_chunk)."""
+        WITWIN_MAXWELL_AMD_TUNE_CACHE, when set.  Returns (xchunk, shell_chunk,
+        z_tiling)."""
         import json
         import os
         if not self.fused:
-            return (0, 0)
+            return (0, 0, 0)
         p = self.plan
         sig = f"single|{p.shape}|{int(p.uses_cpml)}|{int(p.dispersive)}|{len(p.dft_frequencies)}"
         path = os.environ.get("WITWIN_MAXWELL_AMD_TUNE_CACHE")
@@ -346,21 +350,22 @@ class DeviceRun:
             self.set_tiling(*table[sig])
             return tuple(table[sig])
         steps = min(int(trial_steps), max(1, p.time_steps - 1))
-        best, best_t = (0, 0), float("inf")
-        for xc in xchunks:
-            for sc in shell_chunks:
-                self.set_tiling(xc, sc)
-                self.reset()
-                self.run(1)  # warm-up
-                start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                torch.cuda.synchronize()
-                start.record()
-                self.run(min(steps, p.time_steps - self.step_index))
-                end.record()
-                end.synchronize()
-                t = start.elapsed_time(end)
-                if t < best_t:
-                    best, best_t = (xc, sc), t
+        best, best_t = (0, 0, 0), float("inf")
+        for zt in z_tilings:
+            for xc in xchunks:
+                for sc in shell_chunks:
+                    self.set_tiling(xc, sc, zt)
+                    self.reset()
+                    self.run(1)  # warm-up
+                    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                    torch.cuda.synchronize()
+                    start.record()
+                    self.run(min(steps, p.time_steps - self.step_index))
+                    end.record()
+                    end.synchronize()
+                    t = start.elapsed_time(end)
+                    if t < best_t:
+                        best, best_t = (xc, sc, zt), t
         self.set_tiling(*best)
         self.reset()
         if cache and path:
