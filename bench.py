@@ -238,7 +238,7 @@ def bench_batch(args):
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": f"{n} scenes 64^3..192x192x128 cells (+2x10 CPML), 32x32x16 design region, "
                                    f"dipole, single-frequency flux plane, {T} steps, forward only",
-                       "total_nodes": nodes, "xchunk_graph_steps": list(choice)},
+                       "total_nodes": nodes, "xchunk_graph_steps_z_tiling": list(choice)},
             "detail": {"batched_s": t_batch, "plain_s": t_plain, "speedup": t_plain / t_batch,
                        "batched_gnode_updates_per_s": nodes * T / t_batch / 1e9,
                        "plain_gnode_updates_per_s": nodes * T / t_plain / 1e9,

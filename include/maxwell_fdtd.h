@@ -215,9 +215,9 @@ typedef struct mw_program {
   const mw_term* e_terms;
   /* observers: weight tables are (total_steps, nf) float32 in device memory */
   int32_t nf_obs, n_planes, n_points;
-  /* single-pass tiling of the z ends: 0 or 1 = z-edge tiles beside the interior
-   * tiles, 2 = interior tiles aligned to the z PML slabs with the z ends in the
-   * per-cell shell */
+  /* single-pass tiling of the z ends: 0 or 2 = interior tiles aligned to the z
+   * PML slabs with the z ends in the per-cell shell, 1 = z-edge tiles beside the
+   * interior tiles */
   int32_t z_tiling;
   const float* obs_e_cos;
   const float* obs_e_sin;

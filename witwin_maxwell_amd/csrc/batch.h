@@ -61,6 +61,16 @@ __global__ __launch_bounds__(256) void aux_line_e_batch_kernel(const mw_aux_line
   G(a.e)[m] = G(a.decay_e)[m] * G(a.e)[m] - G(a.curl_e)[m] * (G(a.h)[m] - G(a.h)[m - 1]);
 }
 
+// dft_column (terms.h) on global-memory views of table-held pointers.
+__device__ __forceinline__ void plane_dft_column(float x, int nf, int64_t n, int64_t l,
+                                                 gptr<const float> __restrict__ wc, gptr<const float> __restrict__ ws,
+                                                 gptr<float> __restrict__ re, gptr<float> __restrict__ im) {
+  for (int f = 0; f < nf; ++f) {
+    re[f * n + l] += x * wc[f];
+    im[f * n + l] += x * ws[f];
+  }
+}
+
 // One observer plane of one scene.  field[p] is the component buffer that holds
 // the state after a step of parity p (the single-pass step's output buffer).
 struct BatchPlane {
@@ -92,12 +102,8 @@ __global__ __launch_bounds__(256) void accumulate_planes_batch_kernel(const Batc
   else if (q.axis == 1) o = ((uint32_t)ia * sy + q.index) * sz + ib;
   else o = ((uint32_t)ia * sy + ib) * sz + q.index;
   const float x = G(q.field[n & 1])[o];
-  const auto wc = G(q.w_cos + (int64_t)n * q.nf);
-  const auto ws = G(q.w_sin + (int64_t)n * q.nf);
-  for (int f = 0; f < q.nf; ++f) {
-    G(q.re)[f * cells + l] += x * wc[f];
-    G(q.im)[f * cells + l] += x * ws[f];
-  }
+  plane_dft_column(x, q.nf, cells, l, G(q.w_cos + (int64_t)n * q.nf), G(q.w_sin + (int64_t)n * q.nf), G(q.re),
+                   G(q.im));
 }
 
 struct BatchPoint {

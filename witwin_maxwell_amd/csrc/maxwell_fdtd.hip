@@ -337,7 +337,7 @@ int build_fused_args(const mw_program* p, int n, int chunk_override, mw::FusedAr
   const int klo = std::max({1, low(7), low(1)}), khi = std::min({g->nz - 1, high(7, g->nz - 1), high(1, g->nz)});
   // z: owned columns start one after the halo lane; rows: row j-1 is recomputed
   // (jlo already accounts for it); planes: one halo plane before each chunk
-  const bool zedge = p->z_tiling != 2;
+  const bool zedge = p->z_tiling == 1;
   const AxisTiling tz = zedge ? tile_axis_z(g->nz, mw::kFOwnCols, klo, khi) : tile_axis(g->nz, mw::kFOwnCols, 1, klo, khi);
   const AxisTiling ty = tile_axis(g->ny, mw::kFRows, 0, jlo, jhi);
   const int cols = tz.count * ty.count;

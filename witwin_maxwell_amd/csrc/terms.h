@@ -141,6 +141,18 @@ struct PlaneSet {
   mw_plane_obs p[MW_MAX_PLANES];
 };
 
+// Running DFT of one observed sample x into its column l of (nf, n) re/im sums.
+// The sums are restrict-qualified parameters, so all 2 nf loads issue before the
+// stores instead of each waiting for the previous store.
+__device__ __forceinline__ void dft_column(float x, int nf, int64_t n, int64_t l, const float* __restrict__ wc,
+                                           const float* __restrict__ ws, float* __restrict__ re,
+                                           float* __restrict__ im) {
+  for (int f = 0; f < nf; ++f) {
+    re[f * n + l] += x * wc[f];
+    im[f * n + l] += x * ws[f];
+  }
+}
+
 // All plane observers of one step in one launch: blockIdx.y = plane.
 __global__ __launch_bounds__(256) void accumulate_planes_kernel(PlaneSet set, FieldSet fs, int nf,
                                                                 const float* __restrict__ e_cos,
@@ -163,12 +175,7 @@ __global__ __launch_bounds__(256) void accumulate_planes_kernel(PlaneSet set, Fi
   else if (q.axis == 1) o = ((uint32_t)ia * sy + q.index) * sz + ib;
   else o = ((uint32_t)ia * sy + ib) * sz + q.index;
   const float x = fs.ptr[c][o];
-  const float* wc = c < 3 ? e_cos : h_cos;
-  const float* ws = c < 3 ? e_sin : h_sin;
-  for (int f = 0; f < nf; ++f) {
-    q.re[f * n + l] += x * wc[f];
-    q.im[f * n + l] += x * ws[f];
-  }
+  dft_column(x, nf, n, l, c < 3 ? e_cos : h_cos, c < 3 ? e_sin : h_sin, q.re, q.im);
 }
 
 __global__ __launch_bounds__(64) void accumulate_points_kernel(FieldSet fs, int count, const int32_t* comp,

@@ -92,21 +92,22 @@ class BatchRun:
         self.step_index = 0
 
     # ------------------------------------------------------------ tuning
-    def autotune(self, trial_steps: int = 16, chunks=(0, 8, 16, 32), graphs=(1, 8, 32), cache=True):
-        """Pick (xchunk, graph_steps) by timing ``trial_steps`` steps of each
-        candidate on this batch, then reset every scene."""
+    def autotune(self, trial_steps: int = 16, chunks=(0, 8, 16, 32), graphs=(1, 8, 32), z_tilings=(2, 1),
+                 cache=True):
+        """Pick (xchunk, graph_steps, z_tiling) by timing ``trial_steps`` steps of
+        each candidate on this batch, then reset every scene."""
         sig = _signature(self.runs)
         path = os.environ.get(_CACHE_ENV)
         table = {}
         if cache and path and os.path.exists(path):
             with open(path) as f:
                 table = json.load(f)
-        if sig in table:
+        if sig in table and len(table[sig]) == 3:
             self._apply(*table[sig])
             return tuple(table[sig])
         best, best_t = None, float("inf")
-        for chunk in chunks:
-            self._apply(chunk, 1)
+        for chunk, zt in [(c, z) for z in z_tilings for c in chunks]:
+            self._apply(chunk, 1, zt)
             for g in graphs:
                 steps = max(trial_steps, 2 * g)
                 if steps > self.total_steps:
@@ -122,7 +123,7 @@ class BatchRun:
                 torch.cuda.synchronize()
                 per_step = t0.elapsed_time(t1) / steps
                 if per_step < best_t:
-                    best, best_t = (chunk, g), per_step
+                    best, best_t = (chunk, g, zt), per_step
         self.reset()
         self._apply(*best)
         if cache and path:
@@ -131,8 +132,12 @@ class BatchRun:
                 json.dump(table, f)
         return best
 
-    def _apply(self, chunk, graph_steps):
-        if int(chunk) != self.xchunk:
+    def _apply(self, chunk, graph_steps, z_tiling=None):
+        z_changed = z_tiling is not None and any(r.program.z_tiling != int(z_tiling) for r in self.runs)
+        if z_changed:
+            for r in self.runs:
+                r.program.z_tiling = int(z_tiling)
+        if int(chunk) != self.xchunk or z_changed:
             self.close()
             self.xchunk = int(chunk)
             self._create()

@@ -320,9 +320,9 @@ class DeviceRun:
 
     def set_tiling(self, xchunk: int = 0, shell_chunk: int = 0, z_tiling: int = 0):
         """Single-pass tiling: x planes per block of the interior and PML-shell
-        launches (0 = the library's choice) and how the z ends are tiled (0 or 1:
-        z-edge tiles beside the interior tiles; 2: interior tiles aligned to the z
-        PML slabs, z ends in the per-cell shell)."""
+        launches (0 = the library's choice) and how the z ends are tiled (0 or 2:
+        interior tiles aligned to the z PML slabs, z ends in the per-cell shell;
+        1: z-edge tiles beside the interior tiles)."""
         self.program.xchunk = int(xchunk)
         self.program.shell_chunk = int(shell_chunk)
         self.program.z_tiling = int(z_tiling)
