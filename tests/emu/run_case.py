@@ -1,0 +1,78 @@
+"""One small single-pass case on the CPU emulation of the HIP library, compared
+with the float64 oracle.  Run by tests/test_emulated_kernels.py in a child process
+(AddressSanitizer must be preloaded into the interpreter that loads the library).
+
+usage: python run_case.py LIB"""
+import contextlib
+import ctypes
+import inspect
+import os
+import sys
+import textwrap
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import numpy as np  # noqa: E402
+import torch  # noqa: E402
+
+from witwin_maxwell_amd.fdtd import native as nat  # noqa: E402
+
+lib = nat.load_library(sys.argv[1])
+nat.load_library = lambda path=None: lib
+nat.ptr = lambda t: None if t is None else ctypes.c_void_p(t.data_ptr())
+nat.current_stream = lambda: ctypes.c_void_p(None)
+torch.cuda.device = lambda *a, **k: contextlib.nullcontext()
+torch.cuda.synchronize = lambda *a, **k: None
+
+from witwin_maxwell_amd.fdtd import runtime  # noqa: E402
+
+# DeviceRun refuses non-HIP devices; the emulated library works on host memory
+_src = textwrap.dedent(inspect.getsource(runtime.DeviceRun.__init__))
+_src = _src.replace('if self.device.type != "cuda":', "if False:")
+_ns = {}
+exec(compile(_src, "emulated DeviceRun.__init__", "exec"), runtime.__dict__, _ns)
+runtime.DeviceRun.__init__ = _ns["__init__"]
+
+import witwin_maxwell_amd as mw  # noqa: E402
+from oracle.driver import OracleRun  # noqa: E402
+from witwin_maxwell_amd import configs  # noqa: E402
+from witwin_maxwell_amd.fdtd.cpml import PSI_NAMES  # noqa: E402
+from witwin_maxwell_amd.fdtd.plan import COMPONENTS, compile_plan  # noqa: E402
+
+from conftest import family_err  # noqa: E402
+
+
+def run(name, scene, kw, steps, tiling):
+    scene.device = "cpu"
+    kw = dict(kw)
+    kw.pop("theta", None)
+    kw["time_steps"] = steps
+    plan = compile_plan(scene, **kw)
+    ref = OracleRun(plan).run(steps)
+    dev = runtime.DeviceRun(plan, "cpu", fused=True).set_tiling(*tiling)
+    dev.run()
+    errs = {}
+    for fam in (COMPONENTS[:3], COMPONENTS[3:]):
+        scale = max(ref.peak[c] for c in fam)
+        errs.update(family_err({c: dev.fields[c].double().numpy() for c in fam}, {c: ref.fields[c] for c in fam},
+                               scale=scale))
+    if plan.uses_cpml:
+        for fam in (range(0, 6), range(6, 12)):
+            errs.update(family_err({PSI_NAMES[q]: dev.psi_dense(PSI_NAMES[q]).double().numpy() for q in fam},
+                                   {PSI_NAMES[q]: ref.psi[q] for q in fam}))
+    worst = max(errs.values())
+    print(f"{name}: {plan.shape} tiling {tiling} max err {worst:.2e}", flush=True)
+    assert worst <= 1e-5, errs
+
+
+dx = configs.DX
+# a grid with an interior box, PML shell boxes, soft-source terms and per-sample
+# curl coefficients, under both z tilings
+s, kw = configs.dielectric_boxes(cells=12, pml=5, n_boxes=4, box_range=(3, 6), seed=5)
+s.domain = mw.Domain(((-6 * dx, 6 * dx), (-7 * dx, 7 * dx), (-37 * dx, 37 * dx)))
+run("boxes, aligned z tiling", s, kw, 8, (3, 2, 2))
+run("boxes, z-edge tiles", s, kw, 8, (0, 4, 1))
+print("OK")
