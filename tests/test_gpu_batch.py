@@ -1,4 +1,5 @@
-"""Scene batches (GPU): several scenes of uneven sizes advanced by the batched
+"""Scene batches (GPU): several scenes of uneven sizes (one long enough in z for
+an interior box) advanced by the batched
 launches (direct and graph-replayed) give the same fields and monitors as each
 scene run alone, and those match the float64 oracle."""
 
@@ -29,19 +30,23 @@ def _scenes():
                                injection=mw.TFSF.slab(axis="z", bounds=(-8 * dx, 8 * dx))))
     s3.add_monitor(mw.FluxMonitor("t", axis="z", position=9 * dx, frequencies=(configs.F0,)))
     out.append((s3, {"frequencies": (configs.F0,), "courant": 0.95, "time_steps": 120}))
+    # long in z: an interior box, with z-edge tiles or z shell slabs beside it
+    s4, k4 = configs.dielectric_boxes(cells=16, pml=6, n_boxes=6, box_range=(3, 8), seed=5)
+    s4.domain = mw.Domain(((-8 * dx, 8 * dx), (-9 * dx, 9 * dx), (-44 * dx, 44 * dx)))
+    out.append((s4, dict(k4, time_steps=120)))
     return out
 
 
-@pytest.mark.parametrize("graph_steps,xchunk", [(1, 0), (16, 0), (8, 12)])
-def test_batch_matches_single_scene_runs(graph_steps, xchunk):
+@pytest.mark.parametrize("graph_steps,xchunk,z_tiling", [(1, 0, 0), (16, 0, 1), (8, 12, 2)])
+def test_batch_matches_single_scene_runs(graph_steps, xchunk, z_tiling):
     from oracle.driver import OracleRun
     from witwin_maxwell_amd.fdtd.batch import BatchRun
     from witwin_maxwell_amd.fdtd.runtime import DeviceRun
-    print(f"case: batch of 3 scenes, graph_steps={graph_steps}, xchunk={xchunk}", flush=True)
+    print(f"case: batch of 4 scenes, graph_steps={graph_steps}, xchunk={xchunk}, z_tiling={z_tiling}", flush=True)
     scenes = _scenes()
     plans = [compile_plan(s, **kw) for s, kw in scenes]
-    singles = [DeviceRun(p, "cuda").run() for p in plans]
-    batch = BatchRun([DeviceRun(p, "cuda") for p in plans], xchunk=xchunk)
+    singles = [DeviceRun(p, "cuda").set_tiling(0, 0, z_tiling).run() for p in plans]
+    batch = BatchRun([DeviceRun(p, "cuda").set_tiling(0, 0, z_tiling) for p in plans], xchunk=xchunk)
     batch.run(50, graph_steps=graph_steps)
     batch.run(graph_steps=graph_steps)
     torch.cuda.synchronize()

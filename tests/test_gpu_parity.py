@@ -23,9 +23,10 @@ TOL = 1e-5
 FUSED = [True, False]
 
 
-def _run_both(scene, kw, steps, fused=None):
+def _run_both(scene, kw, steps, fused=None, tiling=None):
     """Run oracle and device side by side; fields are also compared at mid-run,
-    while the pulse is still inside the grid."""
+    while the pulse is still inside the grid.  ``tiling`` = (xchunk, shell_chunk,
+    z_tiling) of the single-pass step."""
     from oracle.driver import OracleRun
     from witwin_maxwell_amd.fdtd.runtime import DeviceRun
     kw = dict(kw)
@@ -34,6 +35,8 @@ def _run_both(scene, kw, steps, fused=None):
     plan = compile_plan(scene, **kw)
     ref = OracleRun(plan)
     dev = DeviceRun(plan, "cuda", fused=fused)
+    if tiling is not None:
+        dev.set_tiling(*tiling)
     mid = steps // 3
     ref.run(mid)
     dev.run(mid)
@@ -165,13 +168,19 @@ def test_conductive_and_gaussian_dipole(fused):
     _check_monitors(ref, dev)
 
 
+TILINGS = [None, (3, 1, 1), (8, 4, 2)]
+
+
+@pytest.mark.parametrize("tiling", TILINGS, ids=["default", "x3-s1-zedge", "x8-s4-aligned"])
 @pytest.mark.parametrize("case", ["vacuum_dft", "boxes_soft_source", "dispersive"])
-def test_lean_interior_kernel(case):
+def test_lean_interior_kernel(case, tiling):
     """Grids whose interior spans more than one 63-column tile inside the PML, so
-    the single-pass step runs its lean interior kernel next to the general one
+    the single-pass step runs its interior kernel next to the PML-shell kernels
     (vacuum: homogeneous coefficients + full-field DFT; boxes: per-sample E curl
-    with the soft source planes crossing lean tiles; dispersive: fused ADE)."""
-    print(f"case: lean interior kernel, {case}", flush=True)
+    with the soft source planes crossing interior tiles; dispersive: fused ADE),
+    under the default tiling, short chunks with z-edge tiles (z CPML and faces in
+    the interior-row layout) and long chunks with PML-aligned z tiles."""
+    print(f"case: lean interior kernel, {case}, tiling {tiling}", flush=True)
     if case == "vacuum_dft":
         scene, kw = configs.vacuum_dipole(cells=20, pml=8)
         dx = configs.DX
@@ -186,7 +195,7 @@ def test_lean_interior_kernel(case):
         scene, kw = configs.dispersive_spheres(cells=(24, 26, 100), pml=8, fill=0.3, radius=(3, 7), seed=2,
                                                injection="soft", pulse_delay=3e-15)
         steps = 1400  # the resolved-plasma dt is ~1/110 period: the pulse needs ~1000 steps to reach the monitor
-    plan, ref, dev = _run_both(scene, kw, steps, True)
+    plan, ref, dev = _run_both(scene, kw, steps, True, tiling)
     assert plan.shape[2] >= 63 + 2 * 8 + 4
     _check_fields(ref, dev)
     _check_monitors(ref, dev)
