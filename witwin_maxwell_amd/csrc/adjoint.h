@@ -38,6 +38,9 @@ struct RevArgs {
   mw_ade ad;    // forward pole constants; polarization/current are adjoint states
   int32_t cpml;
   int32_t ade;
+  // per axis, the coordinates [in_lo, in_hi) whose reverse update involves no face,
+  // no psi and no sample outside the grid (empty: lo >= hi)
+  int32_t in_lo[3], in_hi[3];
 };
 
 __device__ __forceinline__ int axis_size(const mw_grid& g, int axis) {
@@ -191,6 +194,39 @@ __device__ __forceinline__ void reverse_h_at(const RevArgs& a, int i, int j, int
   }
 }
 
+// Transpose of the fused ADE of E sample (c, o) (E -= dt J' inv_eps after the poles
+// advance from E^n, kernels/dispersive.cu:18-83): updates the poles' adjoint state
+// and returns what it adds to lambda_E.
+__device__ __forceinline__ float reverse_ade(const RevArgs& a, int c, uint32_t o, float lam) {
+  const int np = a.ad.count[c];
+  if (np <= 0) return 0.0f;
+  float out = 0.0f;
+  const float dt = a.ad.dt;
+  const float lj = -dt * G(a.ad.inv_eps[c])[o] * lam;
+  for (int p = 0; p < np; ++p) {
+    const mw_pole& pl = a.ad.poles[c][p];
+    const float drive = G(pl.drive)[o];
+    if (pl.kind == 0) {  // Debye: P' = d P + g E, J' = (P' - P)/dt; J^n unread
+      const float ljt = G(pl.current)[o] + lj;
+      const float lp = G(pl.polarization)[o] + ljt * (1.0f / dt);
+      G(pl.polarization)[o] = pl.decay * lp - ljt * (1.0f / dt);
+      G(pl.current)[o] = 0.0f;
+      out += drive * lp;
+    } else if (pl.kind == 1) {  // Drude: J' = d J + g E
+      const float lc = G(pl.current)[o] + lj;
+      G(pl.current)[o] = pl.decay * lc;
+      out += drive * lc;
+    } else {  // Lorentz: J' = d J - r P + g E, P' = P + dt J'
+      const float lpol = G(pl.polarization)[o];
+      const float lnc = G(pl.current)[o] + lj + dt * lpol;
+      G(pl.current)[o] = pl.decay * lnc;
+      G(pl.polarization)[o] = lpol - pl.restoring * lnc;
+      out += drive * lnc;
+    }
+  }
+  return out;
+}
+
 // P2: lambda_E of every E sample at (i, j, k).
 template <bool CPML, bool ADE>
 __device__ __forceinline__ void reverse_e_at(const RevArgs& a, int i, int j, int k) {
@@ -220,35 +256,7 @@ __device__ __forceinline__ void reverse_e_at(const RevArgs& a, int i, int j, int
         }
       }
     }
-    if (ADE) {
-      // E -= dt J' inv_eps, then the pole advances from E^n (kernels/dispersive.cu:18-83)
-      const int np = a.ad.count[c];
-      if (np > 0) {
-        const float dt = a.ad.dt;
-        const float lj = -dt * a.ad.inv_eps[c][o] * lam;
-        for (int p = 0; p < np; ++p) {
-          const mw_pole& pl = a.ad.poles[c][p];
-          const float drive = pl.drive[o];
-          if (pl.kind == 0) {  // Debye: P' = d P + g E, J' = (P' - P)/dt; J^n unread
-            const float ljt = pl.current[o] + lj;
-            const float lp = pl.polarization[o] + ljt * (1.0f / dt);
-            pl.polarization[o] = pl.decay * lp - ljt * (1.0f / dt);
-            pl.current[o] = 0.0f;
-            out += drive * lp;
-          } else if (pl.kind == 1) {  // Drude: J' = d J + g E
-            const float lc = pl.current[o] + lj;
-            pl.current[o] = pl.decay * lc;
-            out += drive * lc;
-          } else {  // Lorentz: J' = d J - r P + g E, P' = P + dt J'
-            const float lpol = pl.polarization[o];
-            const float lnc = pl.current[o] + lj + dt * lpol;
-            pl.current[o] = pl.decay * lnc;
-            pl.polarization[o] = lpol - pl.restoring * lnc;
-            out += drive * lnc;
-          }
-        }
-      }
-    }
+    if (ADE) out += reverse_ade(a, c, o, lam);
     // H samples whose forward difference read this E sample: +r(m = x-1) - r(m = x)
     auto fwd = [&](int h, int t) {
       const int ax = h_axis(h, t);
@@ -265,14 +273,121 @@ __device__ __forceinline__ void reverse_e_at(const RevArgs& a, int i, int j, int
   }
 }
 
+// Interior forms of P1 and P2 for a wavefront whose samples all lie in the
+// in_lo/in_hi box: no faces, no psi, every neighbour inside the grid, so each
+// gather is a plain difference of mu = curl_E lambda_E (P1) or nu = -curl_H
+// lambda_H (P2) samples -- the same sums reverse_h_at / reverse_e_at form, term by
+// term.  The buffers are restrict-qualified parameters so the loads of a sample
+// issue together.
+__device__ __forceinline__ float cval(gptr<const float> __restrict__ arr, float value, uint32_t o) {
+  return arr ? arr[o] : value;
+}
+
+__device__ __forceinline__ void reverse_h_fast(const RevArgs& a, int i, int j, int k, gptr<float> __restrict__ lhx,
+                                               gptr<float> __restrict__ lhy, gptr<float> __restrict__ lhz,
+                                               gptr<const float> __restrict__ lex, gptr<const float> __restrict__ ley,
+                                               gptr<const float> __restrict__ lez, gptr<const float> __restrict__ cex,
+                                               gptr<const float> __restrict__ cey, gptr<const float> __restrict__ cez,
+                                               gptr<const float> __restrict__ dhx, gptr<const float> __restrict__ dhy,
+                                               gptr<const float> __restrict__ dhz) {
+  const int ny = a.g.ny, nz = a.g.nz;
+  const uint32_t sEx = (uint32_t)ny * nz, sEy = (uint32_t)(ny - 1) * nz, sEz = (uint32_t)ny * (nz - 1);
+  const uint32_t ox = i * sEx + (uint32_t)j * nz + k;
+  const uint32_t oy = i * sEy + (uint32_t)j * nz + k;
+  const uint32_t oz = i * sEz + (uint32_t)j * (nz - 1) + k;
+  const float cx = a.cf.curl_value[0], cy = a.cf.curl_value[1], cz = a.cf.curl_value[2];
+  auto mux = [&](uint32_t o) { return cval(cex, cx, o) * lex[o]; };
+  auto muy = [&](uint32_t o) { return cval(cey, cy, o) * ley[o]; };
+  auto muz = [&](uint32_t o) { return cval(cez, cz, o) * lez[o]; };
+  const float mx = mux(ox), mx_k = mux(ox + 1), mx_j = mux(ox + nz);
+  const float my = muy(oy), my_k = muy(oy + 1), my_i = muy(oy + sEy);
+  const float mz = muz(oz), mz_j = muz(oz + (nz - 1)), mz_i = muz(oz + sEz);
+  const float iex = G(a.g.inv_e[0])[i], iex1 = G(a.g.inv_e[0])[i + 1];
+  const float iey = G(a.g.inv_e[1])[j], iey1 = G(a.g.inv_e[1])[j + 1];
+  const float iez = G(a.g.inv_e[2])[k], iez1 = G(a.g.inv_e[2])[k + 1];
+  const uint32_t ohx = ((uint32_t)i * (ny - 1) + j) * (nz - 1) + k;
+  const uint32_t ohy = ((uint32_t)i * ny + j) * (nz - 1) + k;
+  const uint32_t ohz = ((uint32_t)i * (ny - 1) + j) * nz + k;
+  const float vx = cval(dhx, a.cf.decay_value[3], ohx) * lhx[ohx];
+  const float vy = cval(dhy, a.cf.decay_value[4], ohy) * lhy[ohy];
+  const float vz = cval(dhz, a.cf.decay_value[5], ohz) * lhz[ohz];
+  lhx[ohx] = vx + (mz_j * iey1 - mz * iey) + (my * iez - my_k * iez1);   // Ez b, Ey b
+  lhy[ohy] = vy + (mx_k * iez1 - mx * iez) + (mz * iex - mz_i * iex1);   // Ex b, Ez a
+  lhz[ohz] = vz + (mx * iey - mx_j * iey1) + (my_i * iex1 - my * iex);   // Ex a, Ey a
+}
+
+template <bool ADE>
+__device__ __forceinline__ void reverse_e_fast(const RevArgs& a, int i, int j, int k, gptr<float> __restrict__ lex,
+                                               gptr<float> __restrict__ ley, gptr<float> __restrict__ lez,
+                                               gptr<const float> __restrict__ lhx, gptr<const float> __restrict__ lhy,
+                                               gptr<const float> __restrict__ lhz, gptr<const float> __restrict__ chx,
+                                               gptr<const float> __restrict__ chy, gptr<const float> __restrict__ chz,
+                                               gptr<const float> __restrict__ dex, gptr<const float> __restrict__ dey,
+                                               gptr<const float> __restrict__ dez) {
+  const int ny = a.g.ny, nz = a.g.nz;
+  const uint32_t sHx = (uint32_t)(ny - 1) * (nz - 1), sHy = (uint32_t)ny * (nz - 1), sHz = (uint32_t)(ny - 1) * nz;
+  const uint32_t ohx = i * sHx + (uint32_t)j * (nz - 1) + k;
+  const uint32_t ohy = i * sHy + (uint32_t)j * (nz - 1) + k;
+  const uint32_t ohz = i * sHz + (uint32_t)j * nz + k;
+  const float cx = a.cf.curl_value[3], cy = a.cf.curl_value[4], cz = a.cf.curl_value[5];
+  auto nux = [&](uint32_t o) { return -cval(chx, cx, o) * lhx[o]; };
+  auto nuy = [&](uint32_t o) { return -cval(chy, cy, o) * lhy[o]; };
+  auto nuz = [&](uint32_t o) { return -cval(chz, cz, o) * lhz[o]; };
+  const float nx0 = nux(ohx), nx_k = nux(ohx - 1), nx_j = nux(ohx - (nz - 1));
+  const float ny0 = nuy(ohy), ny_k = nuy(ohy - 1), ny_i = nuy(ohy - sHy);
+  const float nz0 = nuz(ohz), nz_j = nuz(ohz - nz), nz_i = nuz(ohz - sHz);
+  const float ihx = G(a.g.inv_h[0])[i], ihx0 = G(a.g.inv_h[0])[i - 1];
+  const float ihy = G(a.g.inv_h[1])[j], ihy0 = G(a.g.inv_h[1])[j - 1];
+  const float ihz = G(a.g.inv_h[2])[k], ihz0 = G(a.g.inv_h[2])[k - 1];
+  const uint32_t ox = ((uint32_t)i * ny + j) * nz + k;
+  const uint32_t oy = ((uint32_t)i * (ny - 1) + j) * nz + k;
+  const uint32_t oz = ((uint32_t)i * ny + j) * (nz - 1) + k;
+  const float lx = lex[ox], ly = ley[oy], lz = lez[oz];
+  float outx = cval(dex, a.cf.decay_value[0], ox) * lx;
+  float outy = cval(dey, a.cf.decay_value[1], oy) * ly;
+  float outz = cval(dez, a.cf.decay_value[2], oz) * lz;
+  if (ADE) {
+    outx += reverse_ade(a, 0, ox, lx);
+    outy += reverse_ade(a, 1, oy, ly);
+    outz += reverse_ade(a, 2, oz, lz);
+  }
+  outx += (ny_k * ihz0 - ny0 * ihz) + (nz0 * ihy - nz_j * ihy0);   // Hy b, Hz b
+  outy += (nx0 * ihz - nx_k * ihz0) + (nz_i * ihx0 - nz0 * ihx);   // Hx b, Hz a
+  outz += (nx_j * ihy0 - nx0 * ihy) + (ny0 * ihx - ny_i * ihx0);   // Hx a, Hy a
+  lex[ox] = outx;
+  ley[oy] = outy;
+  lez[oz] = outz;
+}
+
+// Is sample (i, j, k) inside the interior box?  (i and j are wave-uniform; a
+// wavefront that straddles the box edge along z runs both forms.)
+__device__ __forceinline__ bool rev_interior(const RevArgs& a, int i, int j, int k) {
+  return i >= a.in_lo[0] && i < a.in_hi[0] && j >= a.in_lo[1] && j < a.in_hi[1] && k >= a.in_lo[2] &&
+         k < a.in_hi[2];
+}
+
 template <bool CPML>
 __global__ __launch_bounds__(256) void reverse_h_kernel(RevArgs a) {
-  reverse_h_at<CPML>(a, blockIdx.z, blockIdx.y * 4 + wave_row(), blockIdx.x * 64 + threadIdx.x);
+  const int i = blockIdx.z, j = blockIdx.y * 4 + wave_row(), k = blockIdx.x * 64 + threadIdx.x;
+  if (rev_interior(a, i, j, k)) {
+    reverse_h_fast(a, i, j, k, G(a.a.h[0]), G(a.a.h[1]), G(a.a.h[2]), G(a.a.e[0]), G(a.a.e[1]), G(a.a.e[2]),
+                   G(a.cf.curl[0]), G(a.cf.curl[1]), G(a.cf.curl[2]), G(a.cf.decay[3]), G(a.cf.decay[4]),
+                   G(a.cf.decay[5]));
+  } else {
+    reverse_h_at<CPML>(a, i, j, k);
+  }
 }
 
 template <bool CPML, bool ADE>
 __global__ __launch_bounds__(256) void reverse_e_kernel(RevArgs a) {
-  reverse_e_at<CPML, ADE>(a, blockIdx.z, blockIdx.y * 4 + wave_row(), blockIdx.x * 64 + threadIdx.x);
+  const int i = blockIdx.z, j = blockIdx.y * 4 + wave_row(), k = blockIdx.x * 64 + threadIdx.x;
+  if (rev_interior(a, i, j, k)) {
+    reverse_e_fast<ADE>(a, i, j, k, G(a.a.e[0]), G(a.a.e[1]), G(a.a.e[2]), G(a.a.h[0]), G(a.a.h[1]), G(a.a.h[2]),
+                        G(a.cf.curl[3]), G(a.cf.curl[4]), G(a.cf.curl[5]), G(a.cf.decay[0]), G(a.cf.decay[1]),
+                        G(a.cf.decay[2]));
+  } else {
+    reverse_e_at<CPML, ADE>(a, i, j, k);
+  }
 }
 
 // Batched reverse phases: block b of the launch belongs to the scene s with

@@ -376,7 +376,7 @@ int build_fused_args(const mw_program* p, int n, int chunk_override, mw::FusedAr
 
 // Default x planes per shell block: short marches keep the planes that the
 // blocks resident on one XCD touch at the same time few and contiguous.
-constexpr int kShellChunk = 2;
+constexpr int kShellChunk = 4;
 
 // The cells outside the lean box (and its z-edge tiles, when the z ends are
 // tiled that way) as at most six boxes: x slabs over the whole cross-section, y
@@ -1007,6 +1007,26 @@ int mw_reverse_steps(const mw_program* p, const mw_adjoint* adj, const mw_window
     return MW_OK;
   };
   const dim3 block(64, 4, 1);
+  // interior box of the reverse phases: per axis, coordinates whose update reads
+  // no face, no psi and no sample outside the grid
+  {
+    static const int axis_of[6] = {1, 2, 0, 2, 0, 1};
+    const int n[3] = {g->nx, g->ny, g->nz};
+    for (int ax = 0; ax < 3; ++ax) {
+      int lo = 1, hi = n[ax] - 2;
+      if (a.cpml) {
+        for (int q = 0; q < 12; ++q) {
+          if (axis_of[q % 6] != ax) continue;
+          const int low = p->cpml.psi_low[q], hs = p->cpml.psi_high_start[q], hl = p->cpml.psi_high_len[q];
+          if (low + hl <= 0) continue;
+          lo = std::max(lo, q < 6 ? low : low + 1);
+          if (hl > 0) hi = std::min(hi, q < 6 ? hs - 1 : hs);
+        }
+      }
+      a.in_lo[ax] = lo;
+      a.in_hi[ax] = std::max(lo, hi);
+    }
+  }
   const dim3 grid((g->nz + 63) / 64, (g->ny + 3) / 4, g->nx);
   const mw::WindowArgs wa = w ? window_args(w) : mw::WindowArgs{};
   const int64_t we = w ? window_cells(w, 0) : 0, wh = w ? window_cells(w, 1) : 0;

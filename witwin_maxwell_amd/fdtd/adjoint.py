@@ -243,9 +243,12 @@ def run_outputs(run: DeviceRun):
 class GradientSession:
     """Checkpointed forward + reverse sweep for one plan with a design window."""
 
-    def __init__(self, plan: FDTDPlan, device="cuda", window=None, stride=None, fused=None):
+    def __init__(self, plan: FDTDPlan, device="cuda", window=None, stride=None, fused=None, tune=True):
         self.plan = plan
         self.run = DeviceRun(plan, device, fused=fused)
+        if tune and self.run.fused and plan.time_steps >= 200:
+            # a few trial steps of the two z tilings (the run is reset afterwards)
+            self.run.autotune(trial_steps=8, xchunks=(0, 6), shell_chunks=(4,), z_tilings=(1, 2))
         self.window = window if window is not None else design_window(plan)
         self.stride = int(stride or default_stride(plan.time_steps))
         self.checkpoints = {}
