@@ -243,7 +243,7 @@ def run_outputs(run: DeviceRun):
 class GradientSession:
     """Checkpointed forward + reverse sweep for one plan with a design window."""
 
-    def __init__(self, plan: FDTDPlan, device="cuda", window=None, stride=None, fused=None, tune=True):
+    def __init__(self, plan: FDTDPlan, device="cuda", window=None, stride=None, fused=None, tune=False):
         self.plan = plan
         self.run = DeviceRun(plan, device, fused=fused)
         if tune and self.run.fused and plan.time_steps >= 200:
