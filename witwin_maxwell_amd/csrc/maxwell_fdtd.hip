@@ -407,7 +407,15 @@ void build_shell_args(const mw::FusedArgs& a, int nlean, int chunk, mw::ShellArg
     b.lo[0] = bb[0]; b.hi[0] = bb[1];
     b.lo[1] = bb[2]; b.hi[1] = bb[3];
     b.lo[2] = bb[4]; b.hi[2] = bb[5];
-    b.w_shift = ze <= 16 ? 4 : (ze <= 32 ? 5 : 6);
+    // lanes per row: the width with the fewest idle lanes over the box's z extent
+    // (ties within 5% go to the wider rows)
+    b.w_shift = 6;
+    double best_use = 0.0;
+    for (int sh = 6; sh >= 4; --sh) {
+      const int w = 1 << sh;
+      const double use = (double)ze / (double)(((ze + w - 1) / w) * w);
+      if (use > best_use * 1.05) { best_use = use; b.w_shift = sh; }
+    }
     const int w = 1 << b.w_shift, rows = 256 >> b.w_shift;
     b.chunk = std::min(xe, chunk > 0 ? chunk : kShellChunk);
     b.nbz = (ze + w - 1) / w;
