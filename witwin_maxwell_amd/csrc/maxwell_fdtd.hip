@@ -340,18 +340,18 @@ int build_fused_args(const mw_program* p, int n, int chunk_override, mw::FusedAr
   const bool zedge = p->z_tiling == 1;
   const AxisTiling tz = zedge ? tile_axis_z(g->nz, mw::kFOwnCols, klo, khi) : tile_axis(g->nz, mw::kFOwnCols, 1, klo, khi);
   const AxisTiling ty = tile_axis(g->ny, mw::kFRows, 0, jlo, jhi);
-  const int cols = tz.count * ty.count;
-  const int want_chunks = std::max(1, (4096 + cols - 1) / cols);
-  const int target = std::max(8, std::min(32, (g->nx + want_chunks - 1) / want_chunks));
-  // the chunk near the target that leaves the fewest interior planes to the
-  // general kernel (they are the ones the lean box cannot cover)
+  // x chunk: short marches keep the planes that the blocks resident on one XCD
+  // touch at a time few and contiguous (measured best at 3-6 planes on the dense
+  // workloads); among 3..8 take the one leaving the fewest interior planes to the
+  // shell, nearest to 4 on ties
+  const int target = 4;
   int chunk = target;
   if (chunk_override > 0) {
     chunk = chunk_override;
   } else {
     const int span = ihi - (ilo + 1);
     int best = span >= 0 ? span % target : 0;
-    for (int c = std::max(8, target - 8); c <= std::min(32, target + 8) && span >= 8; ++c) {
+    for (int c = 3; c <= 8 && span >= 3; ++c) {
       const int left = span % c;
       if (left < best || (left == best && std::abs(c - target) < std::abs(chunk - target))) { best = left; chunk = c; }
     }

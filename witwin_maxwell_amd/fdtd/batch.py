@@ -92,7 +92,7 @@ class BatchRun:
         self.step_index = 0
 
     # ------------------------------------------------------------ tuning
-    def autotune(self, trial_steps: int = 16, chunks=(0, 8, 16, 32), graphs=(1, 8, 32), z_tilings=(2, 1),
+    def autotune(self, trial_steps: int = 16, chunks=(0, 3, 6, 8), graphs=(1, 8, 32), z_tilings=(2, 1),
                  cache=True):
         """Pick (xchunk, graph_steps, z_tiling) by timing ``trial_steps`` steps of
         each candidate on this batch, then reset every scene."""
