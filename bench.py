@@ -33,7 +33,7 @@ def algorithmic_bytes(plan, runner):
 
     Per step: every field component read and written once (8 B per sample), each
     non-uniform coefficient array read once, each psi slab sample read and
-    written once, ADE pole state read+written and drive read, full-field DFT
+    written once, ADE pole state read+written where the pole acts, full-field DFT
     accumulators read+written.  Per launch: the same restricted to what that
     kernel must touch (its own family R+W, the other family read once)."""
     import numpy as np
@@ -45,12 +45,14 @@ def algorithmic_bytes(plan, runner):
     coef_h = sum(4 * numel[c] for c in COMPONENTS[3:] for k in ("decay", "curl") if runner.uniform[(c, k)] is None)
     psi_e = sum(8 * t.numel() for n, t in runner.psi.items() if n.startswith("psi_e"))
     psi_h = sum(8 * t.numel() for n, t in runner.psi.items() if n.startswith("psi_h"))
+    # ADE: pole state R+W over the samples a pole acts on (Drude J: 8 B; Debye /
+    # Lorentz P and J: 16 B); drive and 1/eps_inf are per-material constants, so 0.
+    # This build streams dense pole arrays, which shows up as traffic above this.
     ade = 0
-    for c, states in runner.ade_state.items():
-        for s in states:
-            ade += numel[c] * (4 + 8 + (8 if s["polarization"] is not None else 0))
-        if states:
-            ade += 4 * numel[c]
+    for c in COMPONENTS[:3]:
+        for pole in plan.ade[c]["poles"]:
+            nnz = int(np.count_nonzero(np.asarray(pole["drive"])))
+            ade += nnz * (8 if pole["kind"] == "drude" else 16)
     dft = sum(2 * 8 * len(plan.dft_frequencies) * numel[c] for c in COMPONENTS[:3]) if plan.dft_frequencies else 0
     step = 8 * (e_n + h_n) + coef_e + coef_h + psi_e + psi_h + ade + dft
     e_launch = 8 * e_n + 4 * h_n + coef_e + psi_e + ade
@@ -111,14 +113,11 @@ def time_kernel(runner, which, reps):
     return start.elapsed_time(end) * 1e-3 / reps
 
 
-def cpu_baseline(scene, kw, steps=8, warm=2):
-    """Float32 OpenMP transcription (oracle/fdtd_oracle.c) on the host cores."""
+def cpu_baseline(plan, steps=8, warm=2):
+    """Float32 OpenMP transcription (oracle/fdtd_oracle.c) on the host cores, run
+    for warm + steps steps of the same plan."""
     from oracle.driver import OracleRun
-    from witwin_maxwell_amd.fdtd.plan import compile_plan
     import numpy as np
-    kw = dict(kw)
-    kw["time_steps"] = warm + steps
-    plan = compile_plan(scene, **kw)
     run = OracleRun(plan, precision="f32")
     for _ in range(warm):
         run.step()
@@ -265,6 +264,9 @@ def main():
     ap.add_argument("--workload", choices=("forward", "gradient", "batch"), default="forward",
                     help="forward: Yee step of the dielectric-box scene (default); gradient: one design-region "
                          "gradient; batch: batched vs one-scene-per-launch forward runs")
+    ap.add_argument("--scene", choices=("boxes", "vacuum", "dispersive"), default="boxes",
+                    help="forward workload scene: dielectric boxes 281^3 nodes (default), vacuum dipole 85^3 with "
+                         "full-field DFT, dispersive spheres 409x409x281 with TFSF")
     ap.add_argument("--grad-steps", type=int, default=3000)
     ap.add_argument("--batch-count", type=int, default=32)
     ap.add_argument("--batch-steps", type=int, default=1500)
@@ -285,9 +287,20 @@ def main():
     from witwin_maxwell_amd.fdtd.runtime import DeviceRun
 
     total = args.warmup + args.steps
-    scene, kw = configs.dielectric_boxes(cells=args.cells, pml=12, n_boxes=40, steps=max(2000, total))
+    if args.scene == "boxes":
+        scene, kw = configs.dielectric_boxes(cells=args.cells, pml=12, n_boxes=40, steps=max(2000, total))
+        workload = (f"dielectric boxes {args.cells}^3 cells (+2x12 CPML), 40 boxes eps U[1,12], "
+                    f"soft +z plane wave, 2 flux monitors x 8 freq")
+    elif args.scene == "vacuum":
+        scene, kw = configs.vacuum_dipole(cells=64, pml=10, steps=max(500, total))
+        workload = ("vacuum 64^3 cells (+2x10 CPML), centred Ez ideal dipole, full-field DFT at 3 frequencies "
+                    "+ point monitor")
+    else:
+        scene, kw = configs.dispersive_spheres(steps=max(4000, total))
+        workload = ("dispersive spheres 384x384x256 cells (+2x12 CPML), 30% Drude + 2-pole Lorentz, eps U[1,4] "
+                    "background, TFSF slab plane wave, flux at 16 frequencies")
     kw = dict(kw)
-    kw["time_steps"] = max(2000, total)
+    kw["time_steps"] = max(int(kw["time_steps"]), total)
     t0 = time.perf_counter()
     plan = compile_plan(scene, **kw)
     compile_s = time.perf_counter() - t0
@@ -330,7 +343,7 @@ def main():
     achieved = b_dom / t_dom
     traffic = None
     tf = os.path.join(ROOT, "profiles", "traffic_dielectric_boxes_256.json")
-    if os.path.exists(tf):
+    if args.scene == "boxes" and args.cells == 256 and os.path.exists(tf):
         try:
             traffic = json.load(open(tf)).get(dominant)
         except Exception:
@@ -350,12 +363,11 @@ def main():
     cpu = None
     if not args.no_cpu:
         print("[bench] cpu baseline", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(scene, kw, steps=args.cpu_steps)
+        cpu = cpu_baseline(plan, steps=args.cpu_steps)
     line = {"metric": "Yee Gcell-updates/s (% HBM peak)", "value": value, "unit": "Gcell-updates/s", "n_gpus": 1,
             "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
             "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": f"dielectric boxes {args.cells}^3 cells (+2x12 CPML), 40 boxes eps U[1,12], "
-                                   f"soft +z plane wave, 2 flux monitors x 8 freq",
+            "config": {"workload": workload,
                        "nodes": list(plan.shape), "courant": 0.99,
                        "step": "single-pass" if runner.fused else "two-kernel",
                        "tiling": None if tiling is None else dict(zip(("xchunk", "shell_chunk", "z_tiling"), tiling))},
