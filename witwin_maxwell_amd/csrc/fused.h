@@ -185,10 +185,12 @@ __device__ __forceinline__ void lean_plane(const FusedArgs& a, int i, int j, int
     float ex = lean_e<COEF>(a, 0, ox, ec.ex, (hz_c - hz_m) * iey - (hy_c - hy_km) * iez, cex);
     float ey = lean_e<COEF>(a, 1, oy, ec.ey, (hx_c - hx_km) * iez - (hz_c - hz_prev) * iex, cey);
     float ez = lean_e<COEF>(a, 2, oz, ec.ez, (hy_c - hy_prev) * iex - (hx_c - hx_m) * iey, cez);
-    if (ADE) {
-      ex -= a.ad.dt * ade_advance(a.ad, 0, ox, ec.ex) * (a.ad.count[0] ? G(a.ad.inv_eps[0])[ox] : 0.0f);
-      ey -= a.ad.dt * ade_advance(a.ad, 1, oy, ec.ey) * (a.ad.count[1] ? G(a.ad.inv_eps[1])[oy] : 0.0f);
-      ez -= a.ad.dt * ade_advance(a.ad, 2, oz, ec.ez) * (a.ad.count[2] ? G(a.ad.inv_eps[2])[oz] : 0.0f);
+    if (ADE) {  // 1/eps_inf is read only where a pole acted
+      const float jx = ade_advance(a.ad, 0, ox, ec.ex), jy = ade_advance(a.ad, 1, oy, ec.ey);
+      const float jz = ade_advance(a.ad, 2, oz, ec.ez);
+      if (jx != 0.0f) ex -= a.ad.dt * jx * G(a.ad.inv_eps[0])[ox];
+      if (jy != 0.0f) ey -= a.ad.dt * jy * G(a.ad.inv_eps[1])[oy];
+      if (jz != 0.0f) ez -= a.ad.dt * jz * G(a.ad.inv_eps[2])[oz];
     }
     if (e_terms) {
       ex += term_value(a.e_terms, a.n_e_terms, 0, i, j, k, step, a.dt);

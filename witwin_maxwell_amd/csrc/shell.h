@@ -381,7 +381,7 @@ __device__ __forceinline__ void shell_e_cells(
         }
         e_new = exv * cdx + ccx * curl;
       }
-      if (ADE) e_new -= a.ad.dt * jsum * G(a.ad.inv_eps[0])[ox];
+      if (ADE && jsum != 0.0f) e_new -= a.ad.dt * jsum * G(a.ad.inv_eps[0])[ox];
       if (et) e_new += term_value(a.e_terms, a.n_e_terms, 0, i, j, k, step, a.dt);
       if (pec) e_new = 0.0f;
       exo[ox] = e_new;
@@ -404,7 +404,7 @@ __device__ __forceinline__ void shell_e_cells(
         }
         e_new = eyv * cdy + ccy * curl;
       }
-      if (ADE) e_new -= a.ad.dt * jsum * G(a.ad.inv_eps[1])[oy];
+      if (ADE && jsum != 0.0f) e_new -= a.ad.dt * jsum * G(a.ad.inv_eps[1])[oy];
       if (et) e_new += term_value(a.e_terms, a.n_e_terms, 1, i, j, k, step, a.dt);
       if (pec) e_new = 0.0f;
       eyo[oy] = e_new;
@@ -427,7 +427,7 @@ __device__ __forceinline__ void shell_e_cells(
         }
         e_new = ezv * cdz + ccz * curl;
       }
-      if (ADE) e_new -= a.ad.dt * jsum * G(a.ad.inv_eps[2])[oz];
+      if (ADE && jsum != 0.0f) e_new -= a.ad.dt * jsum * G(a.ad.inv_eps[2])[oz];
       if (et) e_new += term_value(a.e_terms, a.n_e_terms, 2, i, j, k, step, a.dt);
       if (pec) e_new = 0.0f;
       ezo[oz] = e_new;

@@ -205,6 +205,9 @@ __device__ __forceinline__ float ade_advance(const mw_ade& a, int c, uint32_t o,
   for (int p = 0; p < n; ++p) {
     const mw_pole& pl = a.poles[c][p];
     const float drive = G(pl.drive)[o];
+    // where a pole does not act its state starts at zero and stays zero, and it
+    // adds nothing: skip its state traffic (exact)
+    if (drive == 0.0f) continue;
     if (pl.kind == 0) {
       const float prev = G(pl.polarization)[o];
       const float next = pl.decay * prev + drive * e_old;

@@ -189,7 +189,7 @@ __device__ __forceinline__ void zedge_march(
           e = ec.ex * coef(a.cf.decay[0], a.cf.decay_value[0], ox) +
               coef(a.cf.curl[0], a.cf.curl_value[0], ox) * (da - (db * kez + pb));
         }
-        if (ADE) e -= a.ad.dt * jsum * (a.ad.count[0] ? G(a.ad.inv_eps[0])[ox] : 0.0f);
+        if (ADE && jsum != 0.0f) e -= a.ad.dt * jsum * G(a.ad.inv_eps[0])[ox];
         if (et) e += term_value(a.e_terms, a.n_e_terms, 0, i, j, k, step, a.dt);
         if (pec) e = 0.0f;
         exo[ox] = e;
@@ -213,7 +213,7 @@ __device__ __forceinline__ void zedge_march(
           e = ec.ey * coef(a.cf.decay[1], a.cf.decay_value[1], oy) +
               coef(a.cf.curl[1], a.cf.curl_value[1], oy) * ((db * kez + pb) - da);
         }
-        if (ADE) e -= a.ad.dt * jsum * (a.ad.count[1] ? G(a.ad.inv_eps[1])[oy] : 0.0f);
+        if (ADE && jsum != 0.0f) e -= a.ad.dt * jsum * G(a.ad.inv_eps[1])[oy];
         if (et) e += term_value(a.e_terms, a.n_e_terms, 1, i, j, k, step, a.dt);
         if (pec) e = 0.0f;
         eyo[oy] = e;
@@ -229,7 +229,7 @@ __device__ __forceinline__ void zedge_march(
         const float jsum = ADE ? ade_advance(a.ad, 2, oz, ec.ez) : 0.0f;
         float e = ec.ez * coef(a.cf.decay[2], a.cf.decay_value[2], oz) +
                   coef(a.cf.curl[2], a.cf.curl_value[2], oz) * ((hy_c - hy_prev) * iex - (hx_c - hx_m) * iey);
-        if (ADE) e -= a.ad.dt * jsum * (a.ad.count[2] ? G(a.ad.inv_eps[2])[oz] : 0.0f);
+        if (ADE && jsum != 0.0f) e -= a.ad.dt * jsum * G(a.ad.inv_eps[2])[oz];
         if (et) e += term_value(a.e_terms, a.n_e_terms, 2, i, j, k, step, a.dt);
         ezo[oz] = e;
         if (DFT) {
