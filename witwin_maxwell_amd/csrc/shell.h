@@ -4,19 +4,21 @@
 // The lean kernel (fused.h) advances the interior box, whose cells are free of
 // PML, faces and boundary logic.  What is left is a thin shell: up to six boxes
 // (x-low/high slabs over the whole y-z cross-section, y-low/high slabs over the
-// lean x range, z-low/high slabs over the lean x-y range).  A tile-per-wavefront
-// fused update would spend most of its lanes outside the grid on the thin z
-// slabs, so the shell uses per-cell kernels instead, with the lane layout chosen
-// per box: W consecutive z samples x 256/W rows per block (W = 64 for the x/y
-// slabs, 16 or 32 for a z slab 12-30 samples wide), marching along x over a
-// chunk of planes with the x-neighbour carried in registers.
+// lean x range and, unless the z ends are z-edge tiles (zedge.h), z-low/high
+// slabs over the lean x-y range).  A tile-per-wavefront fused update would spend
+// most of its lanes outside the grid on the thin z slabs, so the shell uses
+// per-cell kernels instead, with the lane layout chosen per box: W consecutive z
+// samples x 256/W rows per block (W = 16, 32 or 64, whichever leaves the fewest
+// idle lanes over the box's z extent), marching along x over a chunk of planes
+// with the x-neighbour carried in registers.
 //
 //   shell_h: H^{n+1/2} of the shell cells from E^n / H^{n-1/2} (ping-pong psi_h)
 //   lean   : interior H and E (recomputes its own H halo, reads only inputs)
+//   z edges: the z ends of the interior rows, when tiled that way (zedge.h)
 //   shell_e: E^{n+1} of the shell cells from the complete H^{n+1/2}
 //
-// shell_h and lean read only the step's input buffers and write disjoint
-// output cells; shell_e runs after both.  The update equations are those of the
+// shell_h, lean and the z-edge tiles read only the step's input buffers and write
+// disjoint output cells; shell_e runs after them.  The update equations are those of the
 // two-kernel path (yee.h: kernels/magnetic.cu:62-117, kernels/electric.cu:463-553)
 // with the single-pass ordering of ADE, source terms, PEC and DFT.
 #pragma once
