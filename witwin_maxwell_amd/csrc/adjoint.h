@@ -41,6 +41,10 @@ struct RevArgs {
   // per axis, the coordinates [in_lo, in_hi) whose reverse update involves no face,
   // no psi and no sample outside the grid (empty: lo >= hi)
   int32_t in_lo[3], in_hi[3];
+  // the nodes outside that box as at most six boxes for the general launch: box b
+  // covers [lo, hi), with 2^w_shift lanes along z per row, one plane per block
+  int32_t nboxes, total;
+  int32_t box_lo[6][3], box_hi[6][3], box_shift[6], box_nbz[6], box_nby[6], box_first[6];
 };
 
 __device__ __forceinline__ int axis_size(const mw_grid& g, int axis) {
@@ -359,35 +363,62 @@ __device__ __forceinline__ void reverse_e_fast(const RevArgs& a, int i, int j, i
   lez[oz] = outz;
 }
 
-// Is sample (i, j, k) inside the interior box?  (i and j are wave-uniform; a
-// wavefront that straddles the box edge along z runs both forms.)
-__device__ __forceinline__ bool rev_interior(const RevArgs& a, int i, int j, int k) {
-  return i >= a.in_lo[0] && i < a.in_hi[0] && j >= a.in_lo[1] && j < a.in_hi[1] && k >= a.in_lo[2] &&
-         k < a.in_hi[2];
+// Interior launch: the in_lo/in_hi box, a wavefront per 64 z samples of a row,
+// four rows per block, one plane per block (grid: z blocks, y blocks, planes).
+__device__ __forceinline__ bool interior_node(const RevArgs& a, int& i, int& j, int& k) {
+  i = a.in_lo[0] + (int)blockIdx.z;
+  j = a.in_lo[1] + (int)blockIdx.y * 4 + wave_row();
+  k = a.in_lo[2] + (int)blockIdx.x * 64 + (int)threadIdx.x;
+  return i < a.in_hi[0] && j < a.in_hi[1] && k < a.in_hi[2];
 }
 
 template <bool CPML>
 __global__ __launch_bounds__(256) void reverse_h_kernel(RevArgs a) {
-  const int i = blockIdx.z, j = blockIdx.y * 4 + wave_row(), k = blockIdx.x * 64 + threadIdx.x;
-  if (rev_interior(a, i, j, k)) {
-    reverse_h_fast(a, i, j, k, G(a.a.h[0]), G(a.a.h[1]), G(a.a.h[2]), G(a.a.e[0]), G(a.a.e[1]), G(a.a.e[2]),
-                   G(a.cf.curl[0]), G(a.cf.curl[1]), G(a.cf.curl[2]), G(a.cf.decay[3]), G(a.cf.decay[4]),
-                   G(a.cf.decay[5]));
-  } else {
-    reverse_h_at<CPML>(a, i, j, k);
-  }
+  int i, j, k;
+  if (!interior_node(a, i, j, k)) return;
+  reverse_h_fast(a, i, j, k, G(a.a.h[0]), G(a.a.h[1]), G(a.a.h[2]), G(a.a.e[0]), G(a.a.e[1]), G(a.a.e[2]),
+                 G(a.cf.curl[0]), G(a.cf.curl[1]), G(a.cf.curl[2]), G(a.cf.decay[3]), G(a.cf.decay[4]),
+                 G(a.cf.decay[5]));
 }
 
 template <bool CPML, bool ADE>
 __global__ __launch_bounds__(256) void reverse_e_kernel(RevArgs a) {
-  const int i = blockIdx.z, j = blockIdx.y * 4 + wave_row(), k = blockIdx.x * 64 + threadIdx.x;
-  if (rev_interior(a, i, j, k)) {
-    reverse_e_fast<ADE>(a, i, j, k, G(a.a.e[0]), G(a.a.e[1]), G(a.a.e[2]), G(a.a.h[0]), G(a.a.h[1]), G(a.a.h[2]),
-                        G(a.cf.curl[3]), G(a.cf.curl[4]), G(a.cf.curl[5]), G(a.cf.decay[0]), G(a.cf.decay[1]),
-                        G(a.cf.decay[2]));
-  } else {
-    reverse_e_at<CPML, ADE>(a, i, j, k);
-  }
+  int i, j, k;
+  if (!interior_node(a, i, j, k)) return;
+  reverse_e_fast<ADE>(a, i, j, k, G(a.a.e[0]), G(a.a.e[1]), G(a.a.e[2]), G(a.a.h[0]), G(a.a.h[1]), G(a.a.h[2]),
+                      G(a.cf.curl[3]), G(a.cf.curl[4]), G(a.cf.curl[5]), G(a.cf.decay[0]), G(a.cf.decay[1]),
+                      G(a.cf.decay[2]));
+}
+
+// General launch: block -> (box, plane, rows, columns); per-box lane width so a
+// thin z slab does not leave most lanes idle.  Returns false for idle threads.
+__device__ __forceinline__ bool box_node(const RevArgs& a, int& i, int& j, int& k) {
+  const int logical = (int)blockIdx.x;
+  if (logical >= a.total) return false;
+  int b = 0;
+  while (b + 1 < a.nboxes && logical >= a.box_first[b + 1]) ++b;
+  const int t = logical - a.box_first[b];
+  const int sh = a.box_shift[b], w = 1 << sh, rows = 256 >> sh;
+  const int bz = t % a.box_nbz[b], by = (t / a.box_nbz[b]) % a.box_nby[b], bx = t / (a.box_nbz[b] * a.box_nby[b]);
+  const int tid = (int)threadIdx.x + 64 * (int)threadIdx.y;
+  k = a.box_lo[b][2] + bz * w + (tid & (w - 1));
+  j = a.box_lo[b][1] + by * rows + (tid >> sh);
+  i = a.box_lo[b][0] + bx;
+  return k < a.box_hi[b][2] && j < a.box_hi[b][1];
+}
+
+template <bool CPML>
+__global__ __launch_bounds__(256) void reverse_h_box_kernel(RevArgs a) {
+  int i, j, k;
+  if (!box_node(a, i, j, k)) return;
+  reverse_h_at<CPML>(a, i, j, k);
+}
+
+template <bool CPML, bool ADE>
+__global__ __launch_bounds__(256) void reverse_e_box_kernel(RevArgs a) {
+  int i, j, k;
+  if (!box_node(a, i, j, k)) return;
+  reverse_e_at<CPML, ADE>(a, i, j, k);
 }
 
 // Batched reverse phases: block b of the launch belongs to the scene s with

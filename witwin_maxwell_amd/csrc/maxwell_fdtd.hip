@@ -1034,15 +1034,53 @@ int mw_reverse_steps(const mw_program* p, const mw_adjoint* adj, const mw_window
       a.in_lo[ax] = lo;
       a.in_hi[ax] = std::max(lo, hi);
     }
+    bool empty = false;
+    for (int ax = 0; ax < 3; ++ax) empty = empty || a.in_hi[ax] <= a.in_lo[ax];
+    if (empty)
+      for (int ax = 0; ax < 3; ++ax) a.in_lo[ax] = a.in_hi[ax] = 0;
+    // the nodes outside the interior box (all of them when it is empty)
+    const int X0 = empty ? n[0] : a.in_lo[0], X1 = empty ? n[0] : a.in_hi[0];
+    const int Y0 = a.in_lo[1], Y1 = a.in_hi[1], Z0 = a.in_lo[2], Z1 = a.in_hi[2];
+    const int boxes[6][6] = {{0, X0, 0, n[1], 0, n[2]},  {X1, n[0], 0, n[1], 0, n[2]}, {X0, X1, 0, Y0, 0, n[2]},
+                             {X0, X1, Y1, n[1], 0, n[2]}, {X0, X1, Y0, Y1, 0, Z0},     {X0, X1, Y0, Y1, Z1, n[2]}};
+    a.nboxes = 0;
+    a.total = 0;
+    for (const auto& bb : boxes) {
+      const int xe = bb[1] - bb[0], ye = bb[3] - bb[2], ze = bb[5] - bb[4];
+      if (xe <= 0 || ye <= 0 || ze <= 0) continue;
+      const int q = a.nboxes++;
+      for (int ax = 0; ax < 3; ++ax) { a.box_lo[q][ax] = bb[2 * ax]; a.box_hi[q][ax] = bb[2 * ax + 1]; }
+      int shift = 6;
+      double best_use = 0.0;
+      for (int sh = 6; sh >= 4; --sh) {
+        const int w = 1 << sh;
+        const double use = (double)ze / (double)(((ze + w - 1) / w) * w);
+        if (use > best_use * 1.05) { best_use = use; shift = sh; }
+      }
+      a.box_shift[q] = shift;
+      a.box_nbz[q] = (ze + (1 << shift) - 1) >> shift;
+      a.box_nby[q] = (ye + (256 >> shift) - 1) / (256 >> shift);
+      a.box_first[q] = a.total;
+      a.total += a.box_nbz[q] * a.box_nby[q] * xe;
+    }
   }
-  const dim3 grid((g->nz + 63) / 64, (g->ny + 3) / 4, g->nx);
+  const dim3 grid((a.in_hi[2] - a.in_lo[2] + 63) / 64, (a.in_hi[1] - a.in_lo[1] + 3) / 4, a.in_hi[0] - a.in_lo[0]);
+  const bool interior = a.in_hi[0] > a.in_lo[0];
+  const dim3 bgrid(a.total);
   const mw::WindowArgs wa = w ? window_args(w) : mw::WindowArgs{};
   const int64_t we = w ? window_cells(w, 0) : 0, wh = w ? window_cells(w, 1) : 0;
   for (int n = first + count - 1; n >= first; --n) {
     if ((rc = seeds(n, 0))) return rc;
-    if (a.cpml) hipLaunchKernelGGL(mw::reverse_h_kernel<true>, grid, block, 0, s, a);
-    else hipLaunchKernelGGL(mw::reverse_h_kernel<false>, grid, block, 0, s, a);
-    if ((rc = check_launch("reverse_h"))) return rc;
+    if (interior) {
+      if (a.cpml) hipLaunchKernelGGL(mw::reverse_h_kernel<true>, grid, block, 0, s, a);
+      else hipLaunchKernelGGL(mw::reverse_h_kernel<false>, grid, block, 0, s, a);
+      if ((rc = check_launch("reverse_h"))) return rc;
+    }
+    if (a.total > 0) {
+      if (a.cpml) hipLaunchKernelGGL(mw::reverse_h_box_kernel<true>, bgrid, block, 0, s, a);
+      else hipLaunchKernelGGL(mw::reverse_h_box_kernel<false>, bgrid, block, 0, s, a);
+      if ((rc = check_launch("reverse_h (boundary boxes)"))) return rc;
+    }
     if ((rc = seeds(n, 1))) return rc;
     if (w) {
       const int64_t si = n - first;
@@ -1051,14 +1089,26 @@ int mw_reverse_steps(const mw_program* p, const mw_adjoint* adj, const mw_window
                          w->g_decay[2], w->g_curl[0], w->g_curl[1], w->g_curl[2]);
       if ((rc = check_launch("window_gradient"))) return rc;
     }
-    if (a.cpml) {
-      if (a.ade) hipLaunchKernelGGL((mw::reverse_e_kernel<true, true>), grid, block, 0, s, a);
-      else hipLaunchKernelGGL((mw::reverse_e_kernel<true, false>), grid, block, 0, s, a);
-    } else {
-      if (a.ade) hipLaunchKernelGGL((mw::reverse_e_kernel<false, true>), grid, block, 0, s, a);
-      else hipLaunchKernelGGL((mw::reverse_e_kernel<false, false>), grid, block, 0, s, a);
+    if (interior) {
+      if (a.cpml) {
+        if (a.ade) hipLaunchKernelGGL((mw::reverse_e_kernel<true, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((mw::reverse_e_kernel<true, false>), grid, block, 0, s, a);
+      } else {
+        if (a.ade) hipLaunchKernelGGL((mw::reverse_e_kernel<false, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((mw::reverse_e_kernel<false, false>), grid, block, 0, s, a);
+      }
+      if ((rc = check_launch("reverse_e"))) return rc;
     }
-    if ((rc = check_launch("reverse_e"))) return rc;
+    if (a.total > 0) {
+      if (a.cpml) {
+        if (a.ade) hipLaunchKernelGGL((mw::reverse_e_box_kernel<true, true>), bgrid, block, 0, s, a);
+        else hipLaunchKernelGGL((mw::reverse_e_box_kernel<true, false>), bgrid, block, 0, s, a);
+      } else {
+        if (a.ade) hipLaunchKernelGGL((mw::reverse_e_box_kernel<false, true>), bgrid, block, 0, s, a);
+        else hipLaunchKernelGGL((mw::reverse_e_box_kernel<false, false>), bgrid, block, 0, s, a);
+      }
+      if ((rc = check_launch("reverse_e (boundary boxes)"))) return rc;
+    }
   }
   return MW_OK;
 }
