@@ -407,11 +407,14 @@ void build_shell_args(const mw::FusedArgs& a, int nlean, int chunk, mw::ShellArg
     b.lo[0] = bb[0]; b.hi[0] = bb[1];
     b.lo[1] = bb[2]; b.hi[1] = bb[3];
     b.lo[2] = bb[4]; b.hi[2] = bb[5];
-    // lanes per row: the width with the fewest idle lanes over the box's z extent
-    // (ties within 5% go to the wider rows)
+    // lanes per row: 32 or 64, the width with the fewest idle lanes over the box's z
+    // extent (ties within 5% go to the wider rows).  16-lane rows would idle fewer
+    // lanes on the thin z slabs but put four row segments in every wavefront
+    // instead of two: measured slower there (dielectric boxes 281^3 nodes, z slabs of
+    // 13 and 16 planes: 647 -> 626 us per step with 32-lane rows).
     b.w_shift = 6;
     double best_use = 0.0;
-    for (int sh = 6; sh >= 4; --sh) {
+    for (int sh = 6; sh >= 5; --sh) {
       const int w = 1 << sh;
       const double use = (double)ze / (double)(((ze + w - 1) / w) * w);
       if (use > best_use * 1.05) { best_use = use; b.w_shift = sh; }
