@@ -399,6 +399,7 @@ void build_shell_args(const mw::FusedArgs& a, int nlean, int chunk, mw::ShellArg
   // the z-edge tiles (zedge.h), not shell boxes
   const int boxes[6][6] = {{0, X0, 0, ny, 0, nz},  {X1, nx, 0, ny, 0, nz},  {X0, X1, 0, Y0, 0, nz},
                            {X0, X1, Y1, ny, 0, nz}, {X0, X1, Y0, Y1, 0, Z0}, {X0, X1, Y0, Y1, Z1, nz}};
+  const int min_shift = (a.ad.count[0] + a.ad.count[1] + a.ad.count[2]) > 0 ? 4 : 5;
   for (int bi = 0; bi < (a.zedge ? 4 : 6); ++bi) {
     const int* bb = boxes[bi];
     const int xe = bb[1] - bb[0], ye = bb[3] - bb[2], ze = bb[5] - bb[4];
@@ -407,14 +408,16 @@ void build_shell_args(const mw::FusedArgs& a, int nlean, int chunk, mw::ShellArg
     b.lo[0] = bb[0]; b.hi[0] = bb[1];
     b.lo[1] = bb[2]; b.hi[1] = bb[3];
     b.lo[2] = bb[4]; b.hi[2] = bb[5];
-    // lanes per row: 32 or 64, the width with the fewest idle lanes over the box's z
-    // extent (ties within 5% go to the wider rows).  16-lane rows would idle fewer
-    // lanes on the thin z slabs but put four row segments in every wavefront
-    // instead of two: measured slower there (dielectric boxes 281^3 nodes, z slabs of
-    // 13 and 16 planes: 647 -> 626 us per step with 32-lane rows).
+    // lanes per row: the width with the fewest idle lanes over the box's z extent
+    // (ties within 5% go to the wider rows), from 16, 32 or 64 with dispersive poles
+    // and from 32 or 64 without.  16-lane rows idle fewer lanes on the thin z slabs
+    // but put four row segments in every wavefront instead of two; per step with
+    // 16- vs 32-lane rows on those slabs (13-16 planes): dielectric boxes 281^3 nodes
+    // 644 vs 624 us, vacuum 85^3 58 vs 53 us, design region 225x225x145 252 vs 245
+    // us; with pole state in the E pass (dispersive 409x409x281) 2679 vs 2758 us.
     b.w_shift = 6;
     double best_use = 0.0;
-    for (int sh = 6; sh >= 5; --sh) {
+    for (int sh = 6; sh >= min_shift; --sh) {
       const int w = 1 << sh;
       const double use = (double)ze / (double)(((ze + w - 1) / w) * w);
       if (use > best_use * 1.05) { best_use = use; b.w_shift = sh; }

@@ -8,10 +8,11 @@
 // slabs over the lean x-y range).  A tile-per-wavefront fused update would spend
 // most of its lanes outside the grid on the thin z slabs, so the shell uses
 // per-cell kernels instead, with the lane layout chosen per box: W consecutive z
-// samples x 256/W rows per block (the kernels take W = 16, 32 or 64; the forward
-// step uses 32 or 64, whichever leaves the fewer idle lanes over the box's z
-// extent), marching along x over a chunk of planes with the x-neighbour carried in
-// registers.
+// samples x 256/W rows per block (W = 16, 32 or 64, whichever leaves the fewest
+// idle lanes over the box's z extent; 16 only with dispersive poles, whose E pass is
+// heavy enough per cell that idle lanes cost more than the extra row segments per
+// wavefront), marching along x over a chunk of planes with the x-neighbour carried
+// in registers.
 //
 //   shell_h: H^{n+1/2} of the shell cells from E^n / H^{n-1/2} (ping-pong psi_h)
 //   lean   : interior H and E (recomputes its own H halo, reads only inputs)
