@@ -103,6 +103,12 @@ typedef struct mw_ade {
   float dt;
   const float* inv_eps[3];
   mw_pole poles[3][MW_MAX_POLES];
+  /* Optional (NULL: every pole's drive is read at every sample): per-sample byte of
+   * the component's shape, bit p set where pole p's drive is non-zero.  A pole whose
+   * bit is clear is skipped there -- its state stays zero and it adds nothing -- so
+   * the drive arrays are read only where some pole acts.  No reference counterpart
+   * (the reference streams every pole array at every sample). */
+  const uint8_t* active[3];
 } mw_ade;
 
 /* H <- H*decay - curl*(curl E) with CPML psi for the components in comp_mask

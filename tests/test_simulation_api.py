@@ -73,3 +73,25 @@ def test_result_accessors():
         res.monitor("nope")
     with pytest.raises(ValueError):
         res.tensor("Ex", frequency=1.5e14)
+
+
+def test_pole_activity_mask_matches_drives():
+    """The step kernels skip pole p where bit p of the activity mask is clear; the
+    mask must be exactly the set of samples where that pole's drive is non-zero."""
+    from witwin_maxwell_amd import configs
+    from witwin_maxwell_amd.fdtd.materials import pole_activity
+    from witwin_maxwell_amd.fdtd.plan import compile_plan
+    scene, kw = configs.dispersive_spheres(cells=(24, 24, 20), pml=4, steps=10, radius=(2, 5), device="cpu")
+    plan = compile_plan(scene, **kw)
+    seen_mixed = False
+    for c in ("Ex", "Ey", "Ez"):
+        poles = plan.ade[c]["poles"]
+        assert poles, "the dispersive scene puts poles on every E component"
+        mask = pole_activity(poles)
+        assert mask.dtype == np.uint8 and mask.shape == tuple(plan.field_shape(c))
+        for p, pole in enumerate(poles):
+            np.testing.assert_array_equal((mask >> p) & 1, (np.asarray(pole["drive"]) != 0).astype(np.uint8))
+        assert (mask >> len(poles)).max() == 0
+        seen_mixed |= bool((mask == 0).any() and (mask != 0).any())
+    assert seen_mixed, "some samples carry no pole and some do"
+    assert pole_activity([]) is None

@@ -202,7 +202,11 @@ __device__ __forceinline__ float edge_diff(bool low_end, int mode, float first, 
 __device__ __forceinline__ float ade_advance(const mw_ade& a, int c, uint32_t o, float e_old) {
   float jsum = 0.0f;
   const int n = a.count[c];
+  // bit p: pole p acts on this sample (without the mask every pole is read)
+  const uint32_t act = a.active[c] ? (uint32_t)G(a.active[c])[o] : 0xffu;
+  if (act == 0u) return 0.0f;
   for (int p = 0; p < n; ++p) {
+    if (((act >> p) & 1u) == 0u) continue;
     const mw_pole& pl = a.poles[c][p];
     const float drive = G(pl.drive)[o];
     // where a pole does not act its state starts at zero and stays zero, and it

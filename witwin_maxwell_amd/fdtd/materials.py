@@ -236,3 +236,15 @@ def material_characteristic_frequency(scene) -> float:
         for p in m.lorentz_poles:
             best = max(best, p.resonance_frequency, p.gamma)
     return best
+
+
+def pole_activity(poles):
+    """Per-sample activity mask of the poles on one E component: bit p set where
+    pole p's drive is non-zero.  Where a pole's drive is zero its state starts at
+    zero and stays zero and it adds no current, so the step kernels skip it there
+    and read the drive arrays only where some pole acts.  None without poles."""
+    mask = None
+    for p, pole in enumerate(poles):
+        bit = (np.asarray(pole["drive"]) != 0).astype(np.uint8) << np.uint8(p)
+        mask = bit if mask is None else (mask | bit)
+    return mask

@@ -56,7 +56,7 @@ class MwPole(ctypes.Structure):
 
 class MwAde(ctypes.Structure):
     _fields_ = [("count", ctypes.c_int32 * 3), ("dt", ctypes.c_float), ("inv_eps", vp * 3),
-                ("poles", (MwPole * MW_MAX_POLES) * 3)]
+                ("poles", (MwPole * MW_MAX_POLES) * 3), ("active", vp * 3)]
 
 
 class MwTerm(ctypes.Structure):

@@ -16,6 +16,7 @@ import torch
 
 from . import native as nat
 from .cpml import PSI_NAMES, PSI_SPECS, compact_shape
+from .materials import pole_activity
 from .plan import COMPONENTS, FDTDPlan
 
 _KIND = {"debye": 0, "drude": 1, "lorentz": 2}
@@ -119,6 +120,7 @@ class DeviceRun:
             if poles:
                 has_ade = True
                 ad.inv_eps[ci] = self._dev(plan.ade[c]["inv_eps"])
+                ad.active[ci] = self._dev(pole_activity(poles), dtype=np.uint8)
             states = []
             for pi, pole in enumerate(poles):
                 pl = ad.poles[ci][pi]
