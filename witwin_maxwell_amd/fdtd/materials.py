@@ -245,6 +245,7 @@ def pole_activity(poles):
     and read the drive arrays only where some pole acts.  None without poles."""
     mask = None
     for p, pole in enumerate(poles):
-        bit = (np.asarray(pole["drive"]) != 0).astype(np.uint8) << np.uint8(p)
+        # the device holds the drives in float32: test the values it will read
+        bit = (np.asarray(pole["drive"], dtype=np.float32) != 0).astype(np.uint8) << np.uint8(p)
         mask = bit if mask is None else (mask | bit)
     return mask
