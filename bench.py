@@ -344,9 +344,15 @@ def main():
         t_h = time_kernel(runner, "H", reps)
         dominant, t_dom, b_dom = ("update_e_kernel", t_e, e_bytes) if t_e >= t_h else ("update_h_kernel", t_h, h_bytes)
     achieved = b_dom / t_dom
+    # HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE runs of the same
+    # workload and kernels (only where such a run exists for exactly this workload)
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "traffic_dielectric_boxes_256.json")
-    if args.scene == "boxes" and args.cells == 256 and os.path.exists(tf):
+    tf = None
+    if runner.fused and args.scene == "boxes" and args.cells == 256:
+        tf = os.path.join(ROOT, "profiles", "traffic_dielectric_boxes_256.json")
+    elif runner.fused and args.scene == "dispersive":
+        tf = os.path.join(ROOT, "profiles", "traffic_dispersive_spheres_409.json")
+    if tf and os.path.exists(tf):
         try:
             traffic = json.load(open(tf)).get(dominant)
         except Exception:
