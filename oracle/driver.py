@@ -96,7 +96,7 @@ def _ptr(arr, real):
 class OracleRun:
     """Holds the state of one plan being stepped on the CPU."""
 
-    def __init__(self, plan, precision="f64"):
+    def __init__(self, plan, precision="f64", record_monitors=None):
         self.plan = plan
         self.lib = load(precision)
         dt_np = self.lib._dtype
@@ -153,13 +153,20 @@ class OracleRun:
         self.e_terms = [(t, arr(t["patch"]), np.ascontiguousarray(t["delay"], dtype=np.float64)) for t in plan.e_terms]
         self.h_terms = [(t, arr(t["patch"]), np.ascontiguousarray(t["delay"], dtype=np.float64)) for t in plan.h_terms]
 
-        # TFSF auxiliary line (float64 restatement, witwin_maxwell_amd.fdtd.tfsf)
+        # TFSF auxiliary 1-D line (float64; advanced by _line_h / _line_e below)
         self.aux = plan.aux
         if self.aux is not None:
             self.aux_e = np.zeros(self.aux["n_e"])
             self.aux_h = np.zeros(self.aux["n_e"] - 1)
 
         self.obs = plan.observers
+        # the oracle's own monitor sampling and payload assembly (oracle/monitors.py)
+        self.monitors = None
+        if record_monitors is None:
+            record_monitors = precision == "f64"
+        if self.obs is not None and record_monitors:
+            from .monitors import OracleObservers
+            self.monitors = OracleObservers(plan)
         if self.obs is not None:
             nf = len(self.obs.frequencies)
             self.point_re = np.zeros((nf, len(self.obs.points)))
@@ -215,9 +222,8 @@ class OracleRun:
         lib.or_update_h(ctypes.byref(self.grid), self._eptr, self._hptr, self._psiptr)
         self._shifted(self.h_terms, t)
         if self.aux is not None:
-            from witwin_maxwell_amd.fdtd.tfsf import advance_line_h
             self._line_terms(plan.line_h_terms, self.aux_e)
-            advance_line_h(self.aux, self.aux_e, self.aux_h)
+            self._line_h()
         for c in ("Ex", "Ey", "Ez"):
             e = self.fields[c]
             for p in self.ade[c]["poles"]:
@@ -225,9 +231,8 @@ class OracleRun:
                                    p["restoring"], plan.dt, _ptr(p["pol"], real), _ptr(p["cur"], real))
         lib.or_update_e(ctypes.byref(self.grid), self._eptr, self._hptr, self._psiptr)
         if self.aux is not None:
-            from witwin_maxwell_amd.fdtd.tfsf import advance_line_e
             self._line_terms(plan.line_e_terms, self.aux_h)
-            advance_line_e(self.aux, self.aux_e, self.aux_h, n)
+            self._line_e(n)
         self._shifted(self.e_terms, t + 0.5 * plan.dt)
         for ti, (comp, off, patch) in enumerate(self.point_terms):
             f = self.fields[comp]
@@ -247,6 +252,8 @@ class OracleRun:
                                   ws.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), _ptr(re, real), _ptr(im, real))
         if self.obs is not None:
             self._observe(n)
+            if self.monitors is not None:
+                self.monitors.record(n, self.fields)
         if self.track_peak:
             for c in _COMP:
                 self.peak[c] = max(self.peak[c], float(np.max(np.abs(self.fields[c]))))
@@ -256,6 +263,22 @@ class OracleRun:
             self.trajectory.append((e_before, {c: self.fields[c].copy() for c in _COMP[3:]},
                                     [p.copy() for p in self.psi[:6]]))
         self.step_index += 1
+
+    def _line_h(self):
+        """Auxiliary line H: h[m] <- decay_h[m] h[m] - curl_h[m] (e[m+1] - e[m])
+        (kernels/sources.cu:803-814)."""
+        a, e, h = self.aux, self.aux_e, self.aux_h
+        h *= a["decay_h"]
+        h -= a["curl_h"] * np.diff(e)
+
+    def _line_e(self, n):
+        """Auxiliary line E: hard source at sample 0, zero at the far end, the 1-D
+        curl update in between (kernels/sources.cu:816-839)."""
+        a, e, h = self.aux, self.aux_e, self.aux_h
+        inner = slice(1, e.size - 1)
+        e[inner] = a["decay_e"][inner] * e[inner] - a["curl_e"][inner] * np.diff(h)
+        e[0] = a["source"][n]
+        e[-1] = 0.0
 
     def _observe(self, n):
         ec, es, hc, hs = (w[n] for w in self.plan.obs_weights)
@@ -278,9 +301,10 @@ class OracleRun:
         return self
 
     def monitor_results(self):
-        from witwin_maxwell_amd.fdtd.spectral import assemble_monitor_results
-        return assemble_monitor_results(self.obs, self.point_re, self.point_im, self.plane_re, self.plane_im,
-                                        self.plan.obs_window_sum)
+        """Monitor payloads assembled by the oracle's own code (oracle/monitors.py):
+        per monitor, the component data (point values or plane arrays with their
+        coordinates) and, for flux planes, the flux."""
+        return self.monitors.results() if self.monitors is not None else {}
 
 
 def run_plan(plan, steps=None, precision="f64"):

@@ -75,4 +75,21 @@ s, kw = configs.dielectric_boxes(cells=12, pml=5, n_boxes=4, box_range=(3, 6), s
 s.domain = mw.Domain(((-6 * dx, 6 * dx), (-7 * dx, 7 * dx), (-37 * dx, 37 * dx)))
 run("boxes, aligned z tiling", s, kw, 8, (3, 2, 2))
 run("boxes, z-edge tiles", s, kw, 8, (0, 4, 1))
+
+# every pole kind (Debye, Drude, two Lorentz) with the per-sample activity mask,
+# in the interior box, the z-edge tiles and the shell, under both z tilings
+two_pi = 2 * np.pi
+d = mw.Scene(domain=mw.Domain(((-6 * dx, 6 * dx), (-7 * dx, 7 * dx), (-37 * dx, 37 * dx))),
+             grid=mw.GridSpec.uniform(dx), boundary=mw.BoundarySpec.pml(5), device="cpu")
+d.add_structure(mw.Structure(mw.Box((0, 0, -20 * dx), (8 * dx, 9 * dx, 20 * dx)),
+                             mw.Material(eps_r=2.0, debye_poles=(mw.DebyePole(2.0, 3e-15),))))
+d.add_structure(mw.Structure(mw.Sphere((1 * dx, -1 * dx, 12 * dx), 5 * dx),
+                             mw.Material(eps_r=2.0, drude_poles=(mw.DrudePole(1.2e15 / two_pi, 1e14 / two_pi),),
+                                         lorentz_poles=(mw.LorentzPole(1.5, 4e15 / two_pi, 2e14 / two_pi),
+                                                        mw.LorentzPole(0.8, 6e15 / two_pi, 2e14 / two_pi)))))
+d.add_source(mw.PointDipole((0.2 * dx, 0.3 * dx, -0.4 * dx), (0.5, 0.3, 1.0), profile="ideal",
+                            source_time=mw.GaussianPulse(configs.F0, configs.F0 / 4)))
+dkw = {"frequencies": (configs.F0,), "courant": 0.9}
+run("dispersive, aligned z tiling", d, dkw, 12, (3, 2, 2))
+run("dispersive, z-edge tiles", d, dkw, 12, (0, 4, 1))
 print("OK")

@@ -234,8 +234,8 @@ def test_dft_accumulation_matches_direct_sum(oracle_lib):
         w = 2 * np.pi * f
         e_direct = np.sum(np.asarray(ey) * np.exp(1j * w * n * plan.dt)) * 2 / plan.time_steps
         h_direct = np.sum(np.asarray(hz) * np.exp(1j * w * (n - 0.5) * plan.dt)) * 2 / plan.time_steps
-        assert abs(res["Ey"][fi] - e_direct) <= 1e-12 * abs(e_direct)
-        assert abs(res["Hz"][fi] - h_direct) <= 1e-12 * abs(h_direct)
+        assert abs(res["components"]["Ey"][fi] - e_direct) <= 1e-12 * abs(e_direct)
+        assert abs(res["components"]["Hz"][fi] - h_direct) <= 1e-12 * abs(h_direct)
         full = run.dft["Ey"][0][fi][iy] + 1j * run.dft["Ey"][1][fi][iy]
         assert abs(full * 2 / plan.time_steps - e_direct) <= 1e-12 * abs(e_direct)
 

@@ -263,8 +263,8 @@ class Simulation:
             raise NotImplementedError("port excitations are outside this FDTD runtime")
         if c.shutoff > 0:
             raise NotImplementedError("auto-shutoff is outside this FDTD runtime")
-        if c.spectral_sampler.normalize_source:
-            raise NotImplementedError("source-spectrum normalisation is outside this FDTD runtime")
+        if c.spectral_sampler.normalize_source and len(self.scene.sources) != 1:
+            raise ValueError("normalize_source=True requires exactly one source.")
         if c.enable_plot:
             raise NotImplementedError("live plotting is outside this FDTD runtime")
 
@@ -328,11 +328,78 @@ class Simulation:
         else:
             for c in ("Ex", "Ey", "Ez"):
                 fields[c] = run.fields[c].detach().clone()
+        if self.config.spectral_sampler.normalize_source:
+            _normalize_by_source(plan, self.scene.sources[0], monitors, fields, self.frequencies[0])
         monitors = _tensorize(monitors, device)
         stats = {"time_steps": steps, "dt": plan.dt, "shape": plan.shape, "fused": run.fused,
                  "full_field_dft": bool(plan.dft_frequencies)}
         return Result(method="fdtd", scene=self.scene, prepared_scene=plan.prepared, frequency=self.frequencies[0],
                       frequencies=self.frequencies, fields=fields, monitors=monitors, solver_stats=stats)
+
+
+def source_spectrum(plan, source, frequencies, weights, window_sum, default_frequency):
+    """Spectrum of the source's waveform envelope at ``frequencies``: the running
+    DFT of s(n dt) with amplitude 1 and phase 0, taken with the monitors' own
+    weights and scaled by 2 / sum(w) like the monitor sums
+    (fdtd/runtime/spectral.py:584-621, sources.py:524-546)."""
+    from .fdtd.excitation import primary_source_time
+    from .fdtd.plan import signal_value
+    st = dict(primary_source_time(source, default_frequency))
+    st["amplitude"] = 1.0
+    st["phase"] = 0.0
+    sig = np.array([signal_value(st, n * plan.dt) for n in range(plan.time_steps)])
+    wc, ws = weights
+    scale = np.where(window_sum > 0, 2.0 / np.where(window_sum > 0, window_sum, 1.0), 0.0)
+    spec = (sig @ wc + 1j * (sig @ ws)) * scale
+    # near-zero values are floored at 1e-10 of the peak (observers.py:985-1011)
+    peak = float(np.max(np.abs(spec))) if spec.size else 0.0
+    floor = 1e-10 * peak if peak > 0 else 1e-30
+    spec = np.where(np.abs(spec) > floor, spec, floor + 0j)
+    index = {float(f): i for i, f in enumerate(frequencies)}
+    return spec, index
+
+
+def _divide(value, spec):
+    if isinstance(value, torch.Tensor):
+        s = torch.as_tensor(spec, dtype=torch.complex128, device=value.device)
+        return value / s.reshape((-1,) + (1,) * (value.ndim - 1)) if s.ndim and value.ndim else value / s
+    v = np.asarray(value)
+    return v / (spec.reshape((-1,) + (1,) * (v.ndim - 1)) if np.ndim(spec) and v.ndim else spec)
+
+
+def _normalize_by_source(plan, source, monitors, fields, default_frequency):
+    """Divide monitor spectra by the source spectrum and fluxes by its squared
+    magnitude (observers.py:1044-1084; full-field DFT postprocess.py:60-78)."""
+    if plan.observers is not None and monitors:
+        spec, index = source_spectrum(plan, source, plan.observers.frequencies, plan.obs_weights[:2],
+                                      plan.obs_window_sum, default_frequency)
+        for name, entry in monitors.items():
+            freqs = entry.get("frequencies", (entry.get("frequency"),))
+            s = spec[[index[float(f)] for f in freqs]]
+            s = s[0] if len(freqs) == 1 else s
+            comps = entry.get("components", {})
+            for comp in list(comps):
+                v = comps[comp]
+                if isinstance(v, dict):
+                    v["data"] = _divide(v["data"], s)
+                else:
+                    comps[comp] = _divide(v, s)
+            for comp in ("Ex", "Ey", "Ez", "Hx", "Hy", "Hz"):
+                if comp in entry:
+                    entry[comp] = _divide(entry[comp], s)
+            if "data" in entry and entry.get("component") in comps:
+                v = comps[entry["component"]]
+                entry["data"] = v["data"] if isinstance(v, dict) else v
+            for key in ("flux", "power"):
+                if key in entry:
+                    entry[key] = entry[key] / (np.abs(s) ** 2 if not isinstance(entry[key], torch.Tensor)
+                                               else torch.as_tensor(np.abs(s) ** 2, device=entry[key].device))
+    if plan.dft_frequencies and fields:
+        spec, _ = source_spectrum(plan, source, plan.dft_frequencies, plan.dft_weights, plan.dft_window_sum,
+                                  default_frequency)
+        s = spec if len(plan.dft_frequencies) > 1 else spec[0]
+        for c in list(fields):
+            fields[c] = _divide(fields[c], s)
 
 
 def run(simulation: Simulation) -> Result:
