@@ -28,6 +28,10 @@ HALF_SPAN = 0.6
 PML = 18
 
 
+def _np(x):
+    return x.detach().cpu().numpy() if hasattr(x, "detach") else np.asarray(x)
+
+
 def _incident_power(frequency, dx, aperture_fraction=0.5):
     from witwin_maxwell_amd.fdtd.excitation import _physical_interior, soft_plane_wave_index
     from witwin_maxwell_amd.scene import prepare_scene
@@ -49,10 +53,10 @@ def _incident_power(frequency, dx, aperture_fraction=0.5):
                                 run_time=mw.TimeConfig.auto(steady_cycles=12, transient_cycles=30),
                                 spectral_sampler=mw.SpectralSampler(normalize_source=True)).run()
     comps = result.monitor("probe")["components"]
-    ex = comps["Ex"]["data"].detach().cpu().numpy()
-    hy = comps["Hy"]["data"].detach().cpu().numpy()
-    xc, yc = (np.asarray(c) for c in comps["Ex"]["coords"])
-    assert np.allclose(xc, np.asarray(comps["Hy"]["coords"][0])) and np.allclose(yc, np.asarray(comps["Hy"]["coords"][1]))
+    ex = _np(comps["Ex"]["data"])
+    hy = _np(comps["Hy"]["data"])
+    xc, yc = (_np(c) for c in comps["Ex"]["coords"])
+    assert np.allclose(xc, _np(comps["Hy"]["coords"][0])) and np.allclose(yc, _np(comps["Hy"]["coords"][1]))
     lo = {a: prep.nodes(a)[_physical_interior(prep, a)[0]] for a in "xy"}
     hi = {a: prep.nodes(a)[_physical_interior(prep, a)[1]] for a in "xy"}
     r = prep.domain_range
@@ -65,7 +69,7 @@ def _incident_power(frequency, dx, aperture_fraction=0.5):
     s_z = 0.5 * np.real(ex[win] * np.conj(hy[win]))
     measured = float(s_z.sum()) * dx * dx
     analytic = float(mx.sum() * my.sum()) * dx * dx / aperture
-    flux = float(result.monitor("flux")["flux"])
+    flux = float(_np(result.monitor("flux")["flux"]).reshape(-1)[0])
     physical = (hi["x"] - lo["x"]) * (hi["y"] - lo["y"]) / aperture
     return measured / analytic, flux / physical
 
