@@ -57,11 +57,16 @@ def load(precision="f64"):
     i64, i32, dbl = ctypes.c_int64, ctypes.c_int32, ctypes.c_double
     lib.or_ade_advance.argtypes = [ctypes.c_int, i64, P, P, dbl, dbl, dbl, P, P]
     lib.or_ade_apply.argtypes = [i64, P, P, dbl, P]
+    IP = ctypes.POINTER(i64)
+    lib.or_ade_advance_idx.argtypes = [ctypes.c_int, i64, IP, P, P, dbl, dbl, dbl, P, P]
+    lib.or_ade_apply_idx.argtypes = [i64, IP, P, P, dbl, P]
     lib.or_add_patch.argtypes = [P, i32, i32, P, i32, i32, i32, i32, i32, i32, dbl]
     lib.or_add_shifted.argtypes = [P, i32, i32, P, ctypes.POINTER(dbl), i32, i32, i32, i32, i32, i32, dbl, i32,
                                    dbl, dbl, dbl, dbl, dbl]
     lib.or_accumulate.argtypes = [i64, P, i32, ctypes.POINTER(dbl), ctypes.POINTER(dbl), P, P]
     lib.or_reverse_step.argtypes = [ctypes.c_void_p, PP, PP, PP, PP, ctypes.c_void_p, PP, PP, PP, PP, PP]
+    lib.or_max_abs.argtypes = [i64, P]
+    lib.or_max_abs.restype = dbl
     lib._real = real
     lib._dtype = np.float64 if precision == "f64" else np.float32
     _LIBS[precision] = lib
@@ -77,7 +82,7 @@ def _grid_struct(real):
     fields = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
               ("bc_lo", ctypes.c_int32 * 3), ("bc_hi", ctypes.c_int32 * 3), ("cpml", ctypes.c_int32),
               ("pad", ctypes.c_int32), ("inv_e", P * 3), ("inv_h", P * 3), ("decay", P * 6), ("curl", P * 6),
-              ("kinv_e", P * 3), ("b_e", P * 3), ("c_e", P * 3), ("kinv_h", P * 3), ("b_h", P * 3), ("c_h", P * 3)]
+              ("decay_value", real * 6), ("curl_value", real * 6), ("kinv_e", P * 3), ("b_e", P * 3), ("c_e", P * 3), ("kinv_h", P * 3), ("b_h", P * 3), ("c_h", P * 3)]
     return type("OrGrid", (ctypes.Structure,), {"_fields_": fields})
 
 
@@ -87,6 +92,21 @@ def _ade_adj_struct(real):
               ("restoring", (ctypes.c_double * 4) * 3), ("drive", (P * 4) * 3), ("apol", (P * 4) * 3),
               ("acur", (P * 4) * 3), ("inv_eps", P * 3), ("dt", ctypes.c_double)]
     return type("OrAdeAdj", (ctypes.Structure,), {"_fields_": fields})
+
+
+def _uniform(a):
+    """The value of an array that holds one value everywhere, else None."""
+    a = np.asarray(a)
+    if a.size == 0:
+        return None
+    v = a.flat[0]
+    if all(st == 0 for st in a.strides) or bool(np.all(a == v)):
+        return float(v)
+    return None
+
+
+def _iptr(arr):
+    return arr.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
 
 
 def _ptr(arr, real):
@@ -122,8 +142,13 @@ class OracleRun:
             g.inv_h[a] = _ptr(arr(plan.inv_h[a]), real)
         for ci, c in enumerate(_COMP):
             dec, curl = plan.coef[c]
-            g.decay[ci] = _ptr(arr(np.broadcast_to(dec, plan.field_shape(c))), real)
-            g.curl[ci] = _ptr(arr(np.broadcast_to(curl, plan.field_shape(c))), real)
+            for a, ptrs, values in ((dec, g.decay, g.decay_value), (curl, g.curl, g.curl_value)):
+                u = _uniform(a)
+                if u is None:
+                    ptrs[ci] = _ptr(arr(np.broadcast_to(a, plan.field_shape(c))), real)
+                else:  # one value over the whole component: passed as a scalar
+                    ptrs[ci] = None
+                    values[ci] = u
         g.cpml = 1 if plan.uses_cpml else 0
         if plan.uses_cpml:
             for a, ax in enumerate("xyz"):
@@ -144,8 +169,11 @@ class OracleRun:
         for c in ("Ex", "Ey", "Ez"):
             entries = []
             for pole in plan.ade[c]["poles"]:
+                drive = arr(pole["drive"])
+                # the samples where the pole acts (its state is zero elsewhere)
+                idx = np.ascontiguousarray(np.flatnonzero(drive), dtype=np.int64)
                 entries.append({"kind": _KIND[pole["kind"]], "decay": pole["decay"], "restoring": pole["restoring"],
-                                "drive": arr(pole["drive"]), "pol": np.zeros(plan.field_shape(c), dt_np),
+                                "drive": drive, "idx": idx, "pol": np.zeros(plan.field_shape(c), dt_np),
                                 "cur": np.zeros(plan.field_shape(c), dt_np)})
             self.ade[c] = {"inv_eps": arr(plan.ade[c]["inv_eps"]), "poles": entries}
 
@@ -227,8 +255,9 @@ class OracleRun:
         for c in ("Ex", "Ey", "Ez"):
             e = self.fields[c]
             for p in self.ade[c]["poles"]:
-                lib.or_ade_advance(p["kind"], e.size, _ptr(e, real), _ptr(p["drive"], real), p["decay"],
-                                   p["restoring"], plan.dt, _ptr(p["pol"], real), _ptr(p["cur"], real))
+                lib.or_ade_advance_idx(p["kind"], p["idx"].size, _iptr(p["idx"]), _ptr(e, real),
+                                       _ptr(p["drive"], real), p["decay"], p["restoring"], plan.dt,
+                                       _ptr(p["pol"], real), _ptr(p["cur"], real))
         lib.or_update_e(ctypes.byref(self.grid), self._eptr, self._hptr, self._psiptr)
         if self.aux is not None:
             self._line_terms(plan.line_e_terms, self.aux_h)
@@ -241,8 +270,8 @@ class OracleRun:
         for c in ("Ex", "Ey", "Ez"):
             e = self.fields[c]
             for p in self.ade[c]["poles"]:
-                lib.or_ade_apply(e.size, _ptr(p["cur"], real), _ptr(self.ade[c]["inv_eps"], real), plan.dt,
-                                 _ptr(e, real))
+                lib.or_ade_apply_idx(p["idx"].size, _iptr(p["idx"]), _ptr(p["cur"], real),
+                                     _ptr(self.ade[c]["inv_eps"], real), plan.dt, _ptr(e, real))
         if self.dft is not None:
             wc = np.ascontiguousarray(plan.dft_weights[0][n])
             ws = np.ascontiguousarray(plan.dft_weights[1][n])
@@ -256,7 +285,8 @@ class OracleRun:
                 self.monitors.record(n, self.fields)
         if self.track_peak:
             for c in _COMP:
-                self.peak[c] = max(self.peak[c], float(np.max(np.abs(self.fields[c]))))
+                f = self.fields[c]
+                self.peak[c] = max(self.peak[c], lib.or_max_abs(f.size, _ptr(f, real)))
         if self.trajectory is not None:
             # E^n, H^{n+1/2} and the E-side psi^{n+1}: what the coefficient
             # gradients of step n need on the way back

@@ -46,8 +46,10 @@ typedef struct {
   int32_t pad;
   const real_t* inv_e[3];
   const real_t* inv_h[3];
-  const real_t* decay[6];
+  const real_t* decay[6]; /* NULL: the component's coefficient is decay_value[c] everywhere */
   const real_t* curl[6];
+  real_t decay_value[6];
+  real_t curl_value[6];
   const real_t* kinv_e[3];
   const real_t* b_e[3];
   const real_t* c_e[3];
@@ -55,6 +57,23 @@ typedef struct {
   const real_t* b_h[3];
   const real_t* c_h[3];
 } or_grid;
+
+/* OpenMP only for large arrays: small grids (the pinning tests) run serially,
+ * free of fork/join overhead */
+#define PARALLEL(count) ((count) >= (1 << 18))
+
+#define DECAY(c, l) (g->decay[c] ? g->decay[c][l] : g->decay_value[c])
+#define CURL(c, l) (g->curl[c] ? g->curl[c][l] : g->curl_value[c])
+
+/* CPML recursion psi' = b psi + c d.  Outside the absorbing layers b = 1 and
+ * c = 0: psi starts at zero and stays exactly zero there, so it is neither read
+ * nor written (the same numbers as the dense update, without its traffic). */
+static inline real_t psi_step(real_t* p, real_t b, real_t c, real_t d) {
+  if (b == (real_t)1 && c == (real_t)0) return 0;
+  const real_t v = b * *p + c * d;
+  *p = v;
+  return v;
+}
 
 static inline int64_t off3(int64_t i, int64_t j, int64_t k, int64_t ny, int64_t nz) {
   return (i * ny + j) * nz + k;
@@ -65,7 +84,7 @@ void or_update_h(const or_grid* g, real_t* const* e, real_t* const* h, real_t* c
   const int64_t nx = g->nx, ny = g->ny, nz = g->nz;
   const real_t *ex = e[0], *ey = e[1], *ez = e[2];
   /* Hx (nx, ny-1, nz-1): a = dEz/dy, b = dEy/dz */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (PARALLEL(nx * ny * nz))
   for (int64_t i = 0; i < nx; ++i)
     for (int64_t j = 0; j < ny - 1; ++j)
       for (int64_t k = 0; k < nz - 1; ++k) {
@@ -74,18 +93,16 @@ void or_update_h(const or_grid* g, real_t* const* e, real_t* const* h, real_t* c
         real_t db = (ey[off3(i, j, k + 1, ny - 1, nz)] - ey[off3(i, j, k, ny - 1, nz)]) * g->inv_h[2][k];
         real_t curl;
         if (g->cpml) {
-          real_t pa = g->b_h[1][j] * psi[6][l] + g->c_h[1][j] * da;
-          real_t pb = g->b_h[2][k] * psi[7][l] + g->c_h[2][k] * db;
-          psi[6][l] = pa;
-          psi[7][l] = pb;
+          real_t pa = psi_step(&psi[6][l], g->b_h[1][j], g->c_h[1][j], da);
+          real_t pb = psi_step(&psi[7][l], g->b_h[2][k], g->c_h[2][k], db);
           curl = (da * g->kinv_h[1][j] + pa) - (db * g->kinv_h[2][k] + pb);
         } else {
           curl = da - db;
         }
-        h[0][l] = h[0][l] * g->decay[3][l] - g->curl[3][l] * curl;
+        h[0][l] = h[0][l] * DECAY(3, l) - CURL(3, l) * curl;
       }
   /* Hy (nx-1, ny, nz-1): a = dEz/dx, b = dEx/dz, curl = b - a */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (PARALLEL(nx * ny * nz))
   for (int64_t i = 0; i < nx - 1; ++i)
     for (int64_t j = 0; j < ny; ++j)
       for (int64_t k = 0; k < nz - 1; ++k) {
@@ -94,18 +111,16 @@ void or_update_h(const or_grid* g, real_t* const* e, real_t* const* h, real_t* c
         real_t db = (ex[off3(i, j, k + 1, ny, nz)] - ex[off3(i, j, k, ny, nz)]) * g->inv_h[2][k];
         real_t curl;
         if (g->cpml) {
-          real_t pa = g->b_h[0][i] * psi[8][l] + g->c_h[0][i] * da;
-          real_t pb = g->b_h[2][k] * psi[9][l] + g->c_h[2][k] * db;
-          psi[8][l] = pa;
-          psi[9][l] = pb;
+          real_t pa = psi_step(&psi[8][l], g->b_h[0][i], g->c_h[0][i], da);
+          real_t pb = psi_step(&psi[9][l], g->b_h[2][k], g->c_h[2][k], db);
           curl = (db * g->kinv_h[2][k] + pb) - (da * g->kinv_h[0][i] + pa);
         } else {
           curl = db - da;
         }
-        h[1][l] = h[1][l] * g->decay[4][l] - g->curl[4][l] * curl;
+        h[1][l] = h[1][l] * DECAY(4, l) - CURL(4, l) * curl;
       }
   /* Hz (nx-1, ny-1, nz): a = dEy/dx, b = dEx/dy */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (PARALLEL(nx * ny * nz))
   for (int64_t i = 0; i < nx - 1; ++i)
     for (int64_t j = 0; j < ny - 1; ++j)
       for (int64_t k = 0; k < nz; ++k) {
@@ -114,15 +129,13 @@ void or_update_h(const or_grid* g, real_t* const* e, real_t* const* h, real_t* c
         real_t db = (ex[off3(i, j + 1, k, ny, nz)] - ex[off3(i, j, k, ny, nz)]) * g->inv_h[1][j];
         real_t curl;
         if (g->cpml) {
-          real_t pa = g->b_h[0][i] * psi[10][l] + g->c_h[0][i] * da;
-          real_t pb = g->b_h[1][j] * psi[11][l] + g->c_h[1][j] * db;
-          psi[10][l] = pa;
-          psi[11][l] = pb;
+          real_t pa = psi_step(&psi[10][l], g->b_h[0][i], g->c_h[0][i], da);
+          real_t pb = psi_step(&psi[11][l], g->b_h[1][j], g->c_h[1][j], db);
           curl = (da * g->kinv_h[0][i] + pa) - (db * g->kinv_h[1][j] + pb);
         } else {
           curl = da - db;
         }
-        h[2][l] = h[2][l] * g->decay[5][l] - g->curl[5][l] * curl;
+        h[2][l] = h[2][l] * DECAY(5, l) - CURL(5, l) * curl;
       }
 }
 
@@ -158,7 +171,7 @@ void or_update_e(const or_grid* g, real_t* const* e, real_t* const* h, real_t* c
   const real_t *hx = h[0], *hy = h[1], *hz = h[2];
   const int *lo = g->bc_lo, *hi = g->bc_hi;
   /* Ex (nx-1, ny, nz): a = dHz/dy (Hz: nx-1, ny-1, nz), b = dHy/dz (Hy: nx-1, ny, nz-1) */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (PARALLEL(nx * ny * nz))
   for (int64_t i = 0; i < nx - 1; ++i)
     for (int64_t j = 0; j < ny; ++j)
       for (int64_t k = 0; k < nz; ++k) {
@@ -175,18 +188,16 @@ void or_update_e(const or_grid* g, real_t* const* e, real_t* const* h, real_t* c
                           hy[off3(i, j, nz - 2, ny, nz - 1)]);
         real_t curl;
         if (g->cpml) {
-          real_t pa = g->b_e[1][j] * psi[0][l] + g->c_e[1][j] * da;
-          real_t pb = g->b_e[2][k] * psi[1][l] + g->c_e[2][k] * db;
-          psi[0][l] = pa;
-          psi[1][l] = pb;
+          real_t pa = psi_step(&psi[0][l], g->b_e[1][j], g->c_e[1][j], da);
+          real_t pb = psi_step(&psi[1][l], g->b_e[2][k], g->c_e[2][k], db);
           curl = (da * g->kinv_e[1][j] + pa) - (db * g->kinv_e[2][k] + pb);
         } else {
           curl = da - db;
         }
-        e[0][l] = e[0][l] * g->decay[0][l] + g->curl[0][l] * curl;
+        e[0][l] = e[0][l] * DECAY(0, l) + CURL(0, l) * curl;
       }
   /* Ey (nx, ny-1, nz): a = dHz/dx (Hz: nx-1, ny-1, nz), b = dHx/dz (Hx: nx, ny-1, nz-1), curl = b - a */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (PARALLEL(nx * ny * nz))
   for (int64_t i = 0; i < nx; ++i)
     for (int64_t j = 0; j < ny - 1; ++j)
       for (int64_t k = 0; k < nz; ++k) {
@@ -203,18 +214,16 @@ void or_update_e(const or_grid* g, real_t* const* e, real_t* const* h, real_t* c
                           hx[off3(i, j, nz - 2, ny - 1, nz - 1)]);
         real_t curl;
         if (g->cpml) {
-          real_t pa = g->b_e[0][i] * psi[2][l] + g->c_e[0][i] * da;
-          real_t pb = g->b_e[2][k] * psi[3][l] + g->c_e[2][k] * db;
-          psi[2][l] = pa;
-          psi[3][l] = pb;
+          real_t pa = psi_step(&psi[2][l], g->b_e[0][i], g->c_e[0][i], da);
+          real_t pb = psi_step(&psi[3][l], g->b_e[2][k], g->c_e[2][k], db);
           curl = (db * g->kinv_e[2][k] + pb) - (da * g->kinv_e[0][i] + pa);
         } else {
           curl = db - da;
         }
-        e[1][l] = e[1][l] * g->decay[1][l] + g->curl[1][l] * curl;
+        e[1][l] = e[1][l] * DECAY(1, l) + CURL(1, l) * curl;
       }
   /* Ez (nx, ny, nz-1): a = dHy/dx (Hy: nx-1, ny, nz-1), b = dHx/dy (Hx: nx, ny-1, nz-1) */
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (PARALLEL(nx * ny * nz))
   for (int64_t i = 0; i < nx; ++i)
     for (int64_t j = 0; j < ny; ++j)
       for (int64_t k = 0; k < nz - 1; ++k) {
@@ -231,22 +240,20 @@ void or_update_e(const or_grid* g, real_t* const* e, real_t* const* h, real_t* c
                           hx[off3(i, ny - 2, k, ny - 1, nz - 1)]);
         real_t curl;
         if (g->cpml) {
-          real_t pa = g->b_e[0][i] * psi[4][l] + g->c_e[0][i] * da;
-          real_t pb = g->b_e[1][j] * psi[5][l] + g->c_e[1][j] * db;
-          psi[4][l] = pa;
-          psi[5][l] = pb;
+          real_t pa = psi_step(&psi[4][l], g->b_e[0][i], g->c_e[0][i], da);
+          real_t pb = psi_step(&psi[5][l], g->b_e[1][j], g->c_e[1][j], db);
           curl = (da * g->kinv_e[0][i] + pa) - (db * g->kinv_e[1][j] + pb);
         } else {
           curl = da - db;
         }
-        e[2][l] = e[2][l] * g->decay[2][l] + g->curl[2][l] * curl;
+        e[2][l] = e[2][l] * DECAY(2, l) + CURL(2, l) * curl;
       }
 }
 
 /* dispersive.cu:18-70; kind 0 = Debye, 1 = Drude, 2 = Lorentz */
 void or_ade_advance(int kind, int64_t n, const real_t* e, const real_t* drive, double decay, double restoring,
                     double dt, real_t* pol, real_t* cur) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (PARALLEL(n))
   for (int64_t l = 0; l < n; ++l) {
     if (kind == 0) {
       real_t prev = pol[l];
@@ -264,9 +271,43 @@ void or_ade_advance(int kind, int64_t n, const real_t* e, const real_t* drive, d
   }
 }
 
+/* or_ade_advance over the samples idx[0..count) only: where a pole's drive is
+ * zero its state starts at zero and stays zero, so skipping those samples gives
+ * the dense routine's numbers */
+void or_ade_advance_idx(int kind, int64_t count, const int64_t* idx, const real_t* e, const real_t* drive,
+                        double decay, double restoring, double dt, real_t* pol, real_t* cur) {
+#pragma omp parallel for schedule(static) if (PARALLEL(count))
+  for (int64_t q = 0; q < count; ++q) {
+    const int64_t l = idx[q];
+    if (kind == 0) {
+      real_t prev = pol[l];
+      real_t next = (real_t)decay * prev + drive[l] * e[l];
+      pol[l] = next;
+      cur[l] = (next - prev) * (real_t)(1.0 / dt);
+    } else if (kind == 1) {
+      cur[l] = (real_t)decay * cur[l] + drive[l] * e[l];
+    } else {
+      real_t pc = cur[l], pp = pol[l];
+      real_t nc = (real_t)decay * pc - (real_t)restoring * pp + drive[l] * e[l];
+      cur[l] = nc;
+      pol[l] = pp + (real_t)dt * nc;
+    }
+  }
+}
+
+/* or_ade_apply over idx[0..count): elsewhere the pole's current is zero */
+void or_ade_apply_idx(int64_t count, const int64_t* idx, const real_t* cur, const real_t* inv_eps, double dt,
+                      real_t* e) {
+#pragma omp parallel for schedule(static) if (PARALLEL(count))
+  for (int64_t q = 0; q < count; ++q) {
+    const int64_t l = idx[q];
+    e[l] -= (real_t)dt * cur[l] * inv_eps[l];
+  }
+}
+
 /* dispersive.cu:72-83 */
 void or_ade_apply(int64_t n, const real_t* cur, const real_t* inv_eps, double dt, real_t* e) {
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (PARALLEL(n))
   for (int64_t l = 0; l < n; ++l) e[l] -= (real_t)dt * cur[l] * inv_eps[l];
 }
 
@@ -314,7 +355,7 @@ void or_accumulate(int64_t n, const real_t* v, int32_t nf, const double* wcos, c
   for (int32_t f = 0; f < nf; ++f) {
     const real_t c = (real_t)wcos[f], s = (real_t)wsin[f];
     if (c == 0 && s == 0) continue;
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) if (PARALLEL(n))
     for (int64_t l = 0; l < n; ++l) {
       re[f * n + l] += v[l] * c;
       im[f * n + l] += v[l] * s;
@@ -323,6 +364,17 @@ void or_accumulate(int64_t n, const real_t* v, int32_t nf, const double* wcos, c
 }
 
 int32_t or_real_bytes(void) { return (int32_t)sizeof(real_t); }
+
+/* max |v| over n samples (the running field peak the parity tests normalise by) */
+double or_max_abs(int64_t n, const real_t* v) {
+  double m = 0.0;
+#pragma omp parallel for schedule(static) if (PARALLEL(n)) reduction(max : m)
+  for (int64_t l = 0; l < n; ++l) {
+    const double a = fabs((double)v[l]);
+    if (a > m) m = a;
+  }
+  return m;
+}
 
 /* ------------------------------------------------------------------ reverse
  * Exact transpose of one forward step in the driver's order (H update, ADE
@@ -407,8 +459,8 @@ void or_reverse_step(const or_grid* g, real_t* const* ae, real_t* const* ah, rea
           } else if (inactive) {
             out = lam;
           } else {
-            out = g->decay[c][l] * lam;
-            const real_t mu = g->curl[c][l] * lam;
+            out = DECAY(c, l) * lam;
+            const real_t mu = CURL(c, l) * lam;
             const real_t wa = term_t(g, cp, 1, ax_a, ca, sa, mu, &apsi[2 * c][l]) * g->inv_e[ax_a][ca];
             const real_t wb = term_t(g, cp, 1, ax_b, cb, sb, mu, &apsi[2 * c + 1][l]) * g->inv_e[ax_b][cb];
             /* H lines read by the two differences: a = d(H_pa)/d(ax_a), b = d(H_pb)/d(ax_b) */
@@ -518,14 +570,14 @@ void or_reverse_step(const or_grid* g, real_t* const* ae, real_t* const* ah, rea
       for (int64_t k = 0; k < nz - 1; ++k) {
         const int64_t l = off3(i, j, k, ny - 1, nz - 1);
         const real_t lam = hx[l];
-        const real_t nu = -g->curl[3][l] * lam;
+        const real_t nu = -CURL(3, l) * lam;
         const real_t wa = term_t(g, cp, 0, 1, j, 1.0f, nu, &apsi[6][l]) * g->inv_h[1][j];
         const real_t wb = term_t(g, cp, 0, 2, k, -1.0f, nu, &apsi[7][l]) * g->inv_h[2][k];
         tz[off3(i, j + 1, k, ny, nz - 1)] += wa;
         tz[off3(i, j, k, ny, nz - 1)] -= wa;
         ty[off3(i, j, k + 1, ny - 1, nz)] += wb;
         ty[off3(i, j, k, ny - 1, nz)] -= wb;
-        hx[l] = g->decay[3][l] * lam;
+        hx[l] = DECAY(3, l) * lam;
       }
   /* Hy (nx-1, ny, nz-1): a = dEz/dx (-), b = dEx/dz (+) */
   for (int64_t i = 0; i < nx - 1; ++i)
@@ -533,14 +585,14 @@ void or_reverse_step(const or_grid* g, real_t* const* ae, real_t* const* ah, rea
       for (int64_t k = 0; k < nz - 1; ++k) {
         const int64_t l = off3(i, j, k, ny, nz - 1);
         const real_t lam = hy[l];
-        const real_t nu = -g->curl[4][l] * lam;
+        const real_t nu = -CURL(4, l) * lam;
         const real_t wa = term_t(g, cp, 0, 0, i, -1.0f, nu, &apsi[8][l]) * g->inv_h[0][i];
         const real_t wb = term_t(g, cp, 0, 2, k, 1.0f, nu, &apsi[9][l]) * g->inv_h[2][k];
         tz[off3(i + 1, j, k, ny, nz - 1)] += wa;
         tz[off3(i, j, k, ny, nz - 1)] -= wa;
         tx[off3(i, j, k + 1, ny, nz)] += wb;
         tx[off3(i, j, k, ny, nz)] -= wb;
-        hy[l] = g->decay[4][l] * lam;
+        hy[l] = DECAY(4, l) * lam;
       }
   /* Hz (nx-1, ny-1, nz): a = dEy/dx (+), b = dEx/dy (-) */
   for (int64_t i = 0; i < nx - 1; ++i)
@@ -548,14 +600,14 @@ void or_reverse_step(const or_grid* g, real_t* const* ae, real_t* const* ah, rea
       for (int64_t k = 0; k < nz; ++k) {
         const int64_t l = off3(i, j, k, ny - 1, nz);
         const real_t lam = hz[l];
-        const real_t nu = -g->curl[5][l] * lam;
+        const real_t nu = -CURL(5, l) * lam;
         const real_t wa = term_t(g, cp, 0, 0, i, 1.0f, nu, &apsi[10][l]) * g->inv_h[0][i];
         const real_t wb = term_t(g, cp, 0, 1, j, -1.0f, nu, &apsi[11][l]) * g->inv_h[1][j];
         ty[off3(i + 1, j, k, ny - 1, nz)] += wa;
         ty[off3(i, j, k, ny - 1, nz)] -= wa;
         tx[off3(i, j + 1, k, ny, nz)] += wb;
         tx[off3(i, j, k, ny, nz)] -= wb;
-        hz[l] = g->decay[5][l] * lam;
+        hz[l] = DECAY(5, l) * lam;
       }
   for (int c = 0; c < 3; ++c) {
     const int64_t n = (int64_t)(c == 0 ? nx - 1 : nx) * (c == 1 ? ny - 1 : ny) * (c == 2 ? nz - 1 : nz);

@@ -63,6 +63,31 @@ def _state_arrays(run):
     return arrs
 
 
+def _zero_unreachable(plan, run):
+    """Zero the state the forward dynamics never populate: psi outside the
+    absorbing layers (b = 1, c = 0 there, so psi stays zero) and pole state where
+    the pole's drive is zero.  The oracle's step skips that state, so the
+    transpose identity is taken over the reachable states."""
+    axis_of = "yzxzxyyzxzxy"
+    for q, psi in enumerate(run.psi):
+        ax = "xyz".index(axis_of[q])
+        if not plan.uses_cpml:
+            psi[...] = 0.0
+            continue
+        prof = plan.cpml["xyz"[ax]]
+        b, c = (prof["b_e"], prof["c_e"]) if q < 6 else (prof["b_h"], prof["c_h"])
+        n = psi.shape[ax]
+        live = ~((np.asarray(b)[:n] == 1.0) & (np.asarray(c)[:n] == 0.0))
+        shape = [1, 1, 1]
+        shape[ax] = n
+        psi *= live.reshape(shape)
+    for c in ("Ex", "Ey", "Ez"):
+        for p in run.ade[c]["poles"]:
+            acts = p["drive"] != 0
+            p["pol"] *= acts
+            p["cur"] *= acts
+
+
 def _adj_arrays(adj):
     arrs = list(adj.ae) + list(adj.ah) + list(adj.apsi)
     for c in ("Ex", "Ey", "Ez"):
@@ -82,6 +107,7 @@ def test_reverse_step_is_exact_transpose(oracle_lib, bname):
     xs = _state_arrays(run)
     for a in xs:
         a[...] = rng.standard_normal(a.shape)
+    _zero_unreachable(plan, run)
     x0 = [a.copy() for a in xs]
     run.step()
     fx = [a.copy() for a in xs]

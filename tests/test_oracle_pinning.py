@@ -311,3 +311,36 @@ def test_cpml_face_reflection(oracle_lib, grading, high_bound):
         out[direction] = trace[int(peak + 5 * sigma_steps):].max() / trace.max()
     assert out[-1] < 1e-3, out
     assert out[1] < high_bound, out
+
+
+def test_sparse_pole_update_equals_dense(oracle_lib):
+    """The oracle advances and applies each pole over the samples where its
+    drive is non-zero; over many steps that gives exactly the dense routines'
+    numbers (a pole with zero drive keeps a zero state and adds nothing)."""
+    import ctypes
+    lib = oracle_lib.load("f64")
+    rng = np.random.default_rng(11)
+    n = 4000
+    P = ctypes.POINTER(ctypes.c_double)
+    IP = ctypes.POINTER(ctypes.c_int64)
+    ptr = lambda a: a.ctypes.data_as(P)
+    for kind in (0, 1, 2):
+        drive = rng.standard_normal(n) * (rng.random(n) < 0.3)
+        idx = np.flatnonzero(drive).astype(np.int64)
+        inv_eps = 1.0 + rng.random(n)
+        dense = {k: np.zeros(n) for k in ("pol", "cur", "e")}
+        sparse = {k: np.zeros(n) for k in ("pol", "cur", "e")}
+        for step in range(20):
+            e_in = rng.standard_normal(n)
+            for st in (dense, sparse):
+                st["e"][:] = e_in
+            lib.or_ade_advance(kind, n, ptr(dense["e"]), ptr(drive), 0.9, 0.05, 1e-2, ptr(dense["pol"]),
+                               ptr(dense["cur"]))
+            lib.or_ade_advance_idx(kind, idx.size, idx.ctypes.data_as(IP), ptr(sparse["e"]), ptr(drive), 0.9, 0.05,
+                                   1e-2, ptr(sparse["pol"]), ptr(sparse["cur"]))
+            lib.or_ade_apply(n, ptr(dense["cur"]), ptr(inv_eps), 1e-2, ptr(dense["e"]))
+            lib.or_ade_apply_idx(idx.size, idx.ctypes.data_as(IP), ptr(sparse["cur"]), ptr(inv_eps), 1e-2,
+                                 ptr(sparse["e"]))
+            for k in ("pol", "cur", "e"):
+                assert np.array_equal(dense[k], sparse[k]), (kind, step, k)
+        assert np.any(dense["cur"] != 0)

@@ -12,7 +12,7 @@ Follows
     (compiler/materials.py:1310-1470).
 
 All host arithmetic is float64; the device receives float32 copies.  Structure
-occupancy is the hard-voxel node rasteriser of ``geometry.rasterize_nodes`` (the
+occupancy is the hard-voxel node rasteriser of ``geometry.rasterize_window`` (the
 reference's occupancy lives in the absent ``witwin.core``).
 """
 
@@ -23,12 +23,25 @@ import torch
 import torch.nn.functional as F
 
 from ..constants import EPSILON_0, MU_0
-from ..geometry import Box, rasterize_nodes
+from ..geometry import Box, rasterize_window
 
 E_COMPONENTS = ("Ex", "Ey", "Ez")
 H_COMPONENTS = ("Hx", "Hy", "Hz")
 
 
+def _threaded(fn):
+    """Run an averaging rule through torch's multi-threaded CPU kernels when the
+    input is a large numpy array (same float64 arithmetic, same result)."""
+    def wrapper(node, component):
+        if isinstance(node, np.ndarray) and node.size >= (1 << 20):
+            return fn(torch.from_numpy(np.ascontiguousarray(node)), component).numpy()
+        return fn(node, component)
+    wrapper.__doc__ = fn.__doc__
+    wrapper.__name__ = fn.__name__
+    return wrapper
+
+
+@_threaded
 def average_node_to_component(node, component):
     """Two-node average onto the E edge (materials.py:399-407)."""
     if component == "Ex":
@@ -40,6 +53,7 @@ def average_node_to_component(node, component):
     raise ValueError(component)
 
 
+@_threaded
 def average_node_to_magnetic_component(node, component):
     """Four-node average onto the H face (materials.py:420-456)."""
     if component == "Hx":
@@ -71,23 +85,27 @@ class NodeMaterialModel:
         x, y, z = prepared.x_nodes64, prepared.y_nodes64, prepared.z_nodes64
         pole_index = {}
         for structure in prepared.scene.structures:
-            mask = rasterize_nodes(structure.geometry, x, y, z)
+            hit = rasterize_window(structure.geometry, x, y, z)
+            if hit is None:
+                continue
+            win, mask = hit
             if not mask.any():
                 continue
             m = structure.material
-            self.eps_r[mask] = m.eps_r
-            self.mu_r[mask] = m.mu_r
-            self.sigma_e[mask] = m.sigma_e
+            # every assignment touches the structure's bounding window only
+            self.eps_r[win][mask] = m.eps_r
+            self.mu_r[win][mask] = m.mu_r
+            self.sigma_e[win][mask] = m.sigma_e
             # a later structure replaces whatever poles an earlier one put on these nodes
             for _, _, weight in self.poles:
-                weight[mask] = 0.0
+                weight[win][mask] = 0.0
             for kind, plist in (("debye", m.debye_poles), ("drude", m.drude_poles), ("lorentz", m.lorentz_poles)):
                 for pole in plist:
                     key = (kind, pole)
                     if key not in pole_index:
                         pole_index[key] = len(self.poles)
                         self.poles.append((kind, pole, np.zeros(shape)))
-                    self.poles[pole_index[key]][2][mask] = 1.0
+                    self.poles[pole_index[key]][2][win][mask] = 1.0
         self.poles = [p for p in self.poles if p[2].any()]
 
     @property
@@ -181,17 +199,37 @@ def electric_coefficients(eps_rel_edge, sigma_edge, dt: float):
     return (1.0 - half) / (1.0 + half), (dt / eps) / (1.0 + half)
 
 
+def _uniform_node(arr):
+    """The value of a node array that holds one value everywhere, else None."""
+    a = np.asarray(arr)
+    v = a.flat[0]
+    return float(v) if np.all(a == v) else None
+
+
 def update_coefficients(prepared, model: NodeMaterialModel, eps_node: np.ndarray, dt: float):
-    """Per-component (decay, curl) in float64 (materials.py:942-1008, :1577-1594)."""
+    """Per-component (decay, curl) in float64 (materials.py:942-1008, :1577-1594).
+    A coefficient that is uniform over its component (lossless media, uniform mu)
+    is returned as a zero-stride broadcast of its value: the same numbers, with no
+    full-size array behind them."""
     coef = {}
     eps_edges = {}
+    lossless = _uniform_node(model.sigma_e) == 0.0
     for comp in E_COMPONENTS:
         eps_rel = average_node_to_component(eps_node, comp)
         eps_edges[comp] = eps_rel * EPSILON_0
-        coef[comp] = electric_coefficients(eps_rel, average_node_to_component(model.sigma_e, comp), dt)
+        if lossless:
+            # sigma = 0: decay = (1 - 0)/(1 + 0) = 1 and curl = (dt/eps)/1 exactly
+            coef[comp] = (np.broadcast_to(np.float64(1.0), eps_rel.shape), dt / eps_edges[comp])
+        else:
+            coef[comp] = electric_coefficients(eps_rel, average_node_to_component(model.sigma_e, comp), dt)
+    mu_value = _uniform_node(model.mu_r)
     for comp in H_COMPONENTS:
-        mu = average_node_to_magnetic_component(model.mu_r, comp) * MU_0
-        coef[comp] = (np.ones_like(mu), dt / mu)
+        shape = component_shape(comp, model.mu_r.shape)
+        if mu_value is not None:
+            coef[comp] = (np.broadcast_to(np.float64(1.0), shape), np.broadcast_to(np.float64(dt / (mu_value * MU_0)), shape))
+        else:
+            mu = average_node_to_magnetic_component(model.mu_r, comp) * MU_0
+            coef[comp] = (np.ones_like(mu), dt / mu)
     return coef, eps_edges
 
 

@@ -88,23 +88,31 @@ class Structure:
         object.__setattr__(self, "name", name)
 
 
-def rasterize_nodes(geometry, x, y, z):
-    """Boolean node-occupancy of ``geometry`` on the tensor-product grid (x, y, z).
-
-    Only the bounding window of the shape is evaluated, so a small body inside a
-    large grid costs O(window)."""
-    out = np.zeros((x.size, y.size, z.size), dtype=bool)
+def rasterize_window(geometry, x, y, z):
+    """Node occupancy of ``geometry`` restricted to its bounding window on the
+    tensor-product grid (x, y, z): ``(slices, mask)`` with ``mask`` the boolean
+    occupancy of the window ``slices``, or ``None`` when the window is empty.
+    A small body inside a large grid costs O(window)."""
     (bx0, bx1), (by0, by1), (bz0, bz1) = geometry.bounds
     i0, i1 = np.searchsorted(x, bx0, "left"), np.searchsorted(x, bx1, "right")
     j0, j1 = np.searchsorted(y, by0, "left"), np.searchsorted(y, by1, "right")
     k0, k1 = np.searchsorted(z, bz0, "left"), np.searchsorted(z, bz1, "right")
     if i1 <= i0 or j1 <= j0 or k1 <= k0:
-        return out
+        return None
     xs = x[i0:i1][:, None, None]
     ys = y[j0:j1][None, :, None]
     zs = z[k0:k1][None, None, :]
-    out[i0:i1, j0:j1, k0:k1] = geometry.contains(xs, ys, zs)
+    mask = np.broadcast_to(geometry.contains(xs, ys, zs), (i1 - i0, j1 - j0, k1 - k0))
+    return (slice(i0, i1), slice(j0, j1), slice(k0, k1)), mask
+
+
+def rasterize_nodes(geometry, x, y, z):
+    """Boolean node-occupancy of ``geometry`` on the whole tensor-product grid."""
+    out = np.zeros((x.size, y.size, z.size), dtype=bool)
+    hit = rasterize_window(geometry, x, y, z)
+    if hit is not None:
+        out[hit[0]] = hit[1]
     return out
 
 
-__all__ = ["Box", "Sphere", "Structure", "rasterize_nodes"]
+__all__ = ["Box", "Sphere", "Structure", "rasterize_nodes", "rasterize_window"]
