@@ -27,7 +27,11 @@ def _f32(a, device):
 
 
 def _uniform_value(arr):
-    a = np.asarray(arr, dtype=np.float32)
+    """The float32 value of a coefficient array that holds one value, else None."""
+    a = np.asarray(arr)
+    if a.size and all(s == 0 for s in a.strides):  # a broadcast scalar
+        return float(np.float32(a.flat[0]))
+    a = a.astype(np.float32)
     if a.size and a.min() == a.max():
         return float(a.flat[0])
     return None
@@ -69,6 +73,7 @@ class DeviceRun:
             g.inv_h[a] = self._dev(plan.inv_h[a])
         cf = prog.coeffs
         self.uniform = {}
+        self.coef_tensors = {}  # (component, "decay"|"curl") -> device array, where not uniform
         for ci, c in enumerate(COMPONENTS):
             dec, curl = plan.coef[c]
             for name, arr, tgt_arr, tgt_val in (("decay", dec, cf.decay, cf.decay_value),
@@ -80,6 +85,7 @@ class DeviceRun:
                     tgt_val[ci] = u
                 else:
                     tgt_arr[ci] = self._dev(np.broadcast_to(arr, plan.field_shape(c)))
+                    self.coef_tensors[(c, name)] = self.keep[-1]
                     tgt_val[ci] = 0.0
         cp = prog.cpml
         self.psi = {}
@@ -375,6 +381,16 @@ class DeviceRun:
             with open(path, "w") as f:
                 json.dump(table, f)
         return best
+
+    def set_coefficients(self, component, name, box, values):
+        """Overwrite the device ``name`` ("decay"/"curl") coefficients of
+        ``component`` on the index box ``box`` (a tuple of slices) with ``values``
+        (e.g. a perturbed design window).  The component's coefficient must be
+        stored as an array (not uniform)."""
+        t = self.coef_tensors.get((component, name))
+        if t is None:
+            raise ValueError(f"{component} {name} is uniform on this run: no array to overwrite")
+        t[box] = torch.as_tensor(np.asarray(values), dtype=torch.float32, device=self.device)
 
     def reset(self):
         for fd in self._fields:
