@@ -1,33 +1,65 @@
 """Benchmark: Yee Gcell-updates/s of the forward FDTD step on one MI355X.
 
-Workload (the dielectric-box configuration): 256^3 physical cells (281^3 Yee
-nodes with CPML 12 on all faces), 40 seeded dielectric boxes (eps ~ U[1,12],
+Default workload (the dielectric-box configuration): 256^3 physical cells (281^3
+Yee nodes with CPML 12 on all faces), 40 seeded dielectric boxes (eps ~ U[1,12],
 edge 8-64 cells), +z soft plane wave with a Gaussian pulse, two z-plane DFT flux
 monitors at 8 frequencies over f0 +- 20%, Courant 0.99.  A "step" is one FDTD
 time step of that scene (H+CPML, soft-source terms, E+CPML, monitor DFT).
 
-Prints ONE JSON line.  ``value`` is node-updates per second over the timed steps
-with all inputs resident in HBM; ``roofline`` prices the dominant kernel's
-algorithmic bytes per launch against 8 TB/s; ``cpu_baseline`` times the float32
+Prints ONE JSON line.  ``value`` is node-updates per second over exactly
+``--steps`` timed steps with all inputs resident in HBM; ``protocol`` adds the
+two-point slope (T(2S) - T(S)) / S as the median of several rounds, twice (A/A),
+so the spread of the two medians shows the noise floor; ``roofline`` prices the
+step's algorithmic bytes against 8 TB/s; ``cpu_baseline`` times the float32
 OpenMP transcription of the same update equations (oracle/fdtd_oracle.c) on a
 bounded sample of the same scene on the host cores.
+
+Other workloads: ``--scene vacuum|dispersive`` (forward), ``--workload gradient``
+(one checkpointed design-region gradient), ``--workload batch`` (32 scenes
+batched vs one scene per launch).  ``--summarize stats DIR`` and ``--summarize
+traffic FETCH_DIR WRITE_DIR`` turn rocprofv3 CSV output into the per-kernel
+tables and HBM bytes per dispatch kept under profiles/.
 """
 
 from __future__ import annotations
 
 import argparse
+import csv
 import ctypes
+import glob
 import json
 import os
+import platform
 import sys
 import time
+from collections import defaultdict
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK = 8.0e12
+COPY_PROBE_BYTES = 256 * 1024 * 1024  # --copy-probe copies this many bytes per dispatch
 
 
+# ------------------------------------------------------------------ host facts
+def host_info():
+    model = platform.processor() or ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = os.cpu_count() or 1
+    return {"cpu_model": model, "nproc": avail, "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
+
+
+# ------------------------------------------------------------ algorithmic bytes
 def algorithmic_bytes(plan, runner):
     """Compulsory float32 bytes per step and per E/H launch.
 
@@ -46,12 +78,13 @@ def algorithmic_bytes(plan, runner):
     psi_e = sum(8 * t.numel() for n, t in runner.psi.items() if n.startswith("psi_e"))
     psi_h = sum(8 * t.numel() for n, t in runner.psi.items() if n.startswith("psi_h"))
     # ADE: pole state R+W over the samples a pole acts on (Drude J: 8 B; Debye /
-    # Lorentz P and J: 16 B); drive and 1/eps_inf are per-material constants, so 0.
-    # This build streams dense pole arrays, which shows up as traffic above this.
+    # Lorentz P and J: 16 B), counted on the float32 drives the device reads;
+    # drive and 1/eps_inf are per-material constants, so 0.  Dense pole arrays
+    # show up as traffic above this.
     ade = 0
     for c in COMPONENTS[:3]:
         for pole in plan.ade[c]["poles"]:
-            nnz = int(np.count_nonzero(np.asarray(pole["drive"])))
+            nnz = int(np.count_nonzero(np.asarray(pole["drive"], dtype=np.float32)))
             ade += nnz * (8 if pole["kind"] == "drude" else 16)
     dft = sum(2 * 8 * len(plan.dft_frequencies) * numel[c] for c in COMPONENTS[:3]) if plan.dft_frequencies else 0
     step = 8 * (e_n + h_n) + coef_e + coef_h + psi_e + psi_h + ade + dft
@@ -60,11 +93,60 @@ def algorithmic_bytes(plan, runner):
     return step, e_launch, h_launch
 
 
+def state_bytes(runner):
+    """Bytes of one scene's whole state (fields, psi slabs, pole state)."""
+    n = sum(t.numel() for t in runner.fields.values()) + sum(t.numel() for t in runner.psi.values())
+    for states in runner.ade_state.values():
+        for s in states:
+            n += sum(v.numel() for v in s.values() if v is not None)
+    return 4 * n
+
+
+# ------------------------------------------------------------------- timing
+def _events():
+    import torch
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def timed_run(runner, steps):
+    """Seconds for ``runner.run(steps)`` between HIP events on the launch stream."""
+    import torch
+    start, end = _events()
+    torch.cuda.synchronize()
+    start.record()
+    runner.run(steps)
+    end.record()
+    end.synchronize()
+    return start.elapsed_time(end) * 1e-3
+
+
+def slope_protocol(runner, span, rounds):
+    """Two-point slope (T(2S) - T(S)) / S per round (run from a reset state each
+    round), median over ``rounds``; done twice, the second set is the A/A run
+    whose median shows the noise floor."""
+    import numpy as np
+    medians, all_slopes = [], []
+    for _ in range(2):
+        slopes = []
+        for _ in range(rounds):
+            runner.reset()
+            t1 = timed_run(runner, span)
+            runner.reset()
+            t2 = timed_run(runner, 2 * span)
+            slopes.append((t2 - t1) / span)
+        all_slopes.append(slopes)
+        medians.append(float(np.median(slopes)))
+    mid = 0.5 * (medians[0] + medians[1])
+    return {"method": "two-point slope (T(2S)-T(S))/S, median of rounds, A/A", "S": span, "rounds": rounds,
+            "slope_ms_per_step": mid * 1e3, "median_a_ms": medians[0] * 1e3, "median_b_ms": medians[1] * 1e3,
+            "aa_spread": abs(medians[0] - medians[1]) / mid if mid > 0 else None,
+            "round_ms_a": [s * 1e3 for s in all_slopes[0]], "round_ms_b": [s * 1e3 for s in all_slopes[1]]}
+
+
 def time_fused(runner, reps):
     """Average duration of one single-pass step launch: the program is copied with
-    its monitors detached, so each step is exactly one kernel on torch's current
-    stream, bracketed by HIP events."""
-    import torch
+    its monitors detached, so each step is exactly the step kernels on torch's
+    current stream, bracketed by HIP events."""
     from witwin_maxwell_amd.fdtd import native as nat
     prog = nat.MwProgram()
     ctypes.memmove(ctypes.byref(prog), ctypes.byref(runner.program), ctypes.sizeof(prog))
@@ -76,7 +158,8 @@ def time_fused(runner, reps):
         raise RuntimeError(f"no steps left in the program for the launch timing (at step {first})")
     stream = nat.current_stream()
     nat.check(runner.lib.mw_run_steps(ctypes.byref(prog), first, 2, stream), runner.lib)
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start, end = _events()
+    import torch
     torch.cuda.synchronize()
     start.record()
     nat.check(runner.lib.mw_run_steps(ctypes.byref(prog), first + 2, reps, stream), runner.lib)
@@ -105,7 +188,7 @@ def time_kernel(runner, which, reps):
 
     for _ in range(3):
         launch()
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start, end = _events()
     torch.cuda.synchronize()
     start.record()
     for _ in range(reps):
@@ -115,11 +198,16 @@ def time_kernel(runner, which, reps):
     return start.elapsed_time(end) * 1e-3 / reps
 
 
+# --------------------------------------------------------------- CPU baseline
+def _threads():
+    return int(os.environ.get("OMP_NUM_THREADS", host_info()["nproc"]))
+
+
 def cpu_baseline(plan, steps=8, warm=2):
     """Float32 OpenMP transcription (oracle/fdtd_oracle.c) on the host cores, run
     for warm + steps steps of the same plan."""
-    from oracle.driver import OracleRun
     import numpy as np
+    from oracle.driver import OracleRun
     run = OracleRun(plan, precision="f32")
     for _ in range(warm):
         run.step()
@@ -128,12 +216,34 @@ def cpu_baseline(plan, steps=8, warm=2):
         run.step()
     dt = time.perf_counter() - t0
     nodes = int(np.prod(plan.shape))
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    threads = _threads()
     return {"value": nodes * steps / dt / 1e9, "unit": "Gcell-updates/s", "cores": threads, "kind": "port",
+            "cpu_model": host_info()["cpu_model"],
             "sample": f"same {plan.shape[0]}x{plan.shape[1]}x{plan.shape[2]}-node scene, {steps} steps after "
                       f"{warm} warm-up steps, float32 OpenMP ({threads} threads), {dt:.2f} s"}
 
 
+def cpu_gradient_rates(plan, steps=4):
+    """Seconds per forward step and per reverse step of the float32 OpenMP
+    transcription on the host cores (the reverse step without the window
+    gradient, which is O(window))."""
+    from oracle.driver import OracleAdjoint, OracleRun
+    run = OracleRun(plan, precision="f32", record_monitors=False)
+    run.step()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        run.step()
+    t_f = (time.perf_counter() - t0) / steps
+    adj = OracleAdjoint(run)
+    adj.reverse_step(0, gradients=False)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        adj.reverse_step(0, gradients=False)
+    t_r = (time.perf_counter() - t0) / steps
+    return t_f, t_r
+
+
+# ----------------------------------------------------------------- workloads
 def _flux_objective(monitors):
     e = monitors["objective"]
     return e["flux"] if "flux" in e else sum((e[c].abs() ** 2).sum() for c in ("Ex", "Ey") if c in e)
@@ -143,40 +253,72 @@ def bench_gradient(args):
     """Design-region gradient: checkpointed forward + native reverse sweep of the
     design-region configuration (225x225x145 nodes, 64x64x32 design voxels,
     3000 steps, checkpoints every 100), loss = |E|^2 of the central 40x40 of the
-    objective plane at f0."""
+    objective plane at f0.
+
+    Roofline: algorithmic bytes of the whole gradient (forward steps, the segment
+    replay steps, reverse steps at the forward step's bytes -- adjoint fields and
+    psi R+W, coefficients read -- and each checkpoint's snapshot and restore)
+    over its wall time."""
     import numpy as np
     import torch
     from witwin_maxwell_amd import configs
     from witwin_maxwell_amd.fdtd.adjoint import differentiable_run
-    scene, kw = configs.design_region(steps=args.grad_steps)
-    theta = kw.pop("theta").clone().requires_grad_(True)
-    region = scene.material_regions[0]
     from witwin_maxwell_amd.scene import MaterialRegion
-    scene.material_regions[0] = MaterialRegion(region.name, region.geometry, torch.sigmoid(theta),
-                                               eps_bounds=region.eps_bounds)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    plan, mons, session = differentiable_run(scene, stride=100, **kw)
-    e = mons["objective"]
-    loss = 0.0
-    for c in ("Ex", "Ey"):
-        d = e[c]
-        a0, b0 = (d.shape[-2] - 40) // 2, (d.shape[-1] - 40) // 2
-        loss = loss + (d[..., a0:a0 + 40, b0:b0 + 40].abs() ** 2).sum()
-    torch.cuda.synchronize()
-    t_fwd = time.perf_counter() - t0
-    loss.backward()
-    torch.cuda.synchronize()
-    total = time.perf_counter() - t0
+    scene, kw = configs.design_region(steps=args.grad_steps)
+    theta0 = kw.pop("theta")
+    region = scene.material_regions[0]
+    stride = 100
+
+    def one_gradient():
+        theta = theta0.clone().requires_grad_(True)
+        scene.material_regions[0] = MaterialRegion(region.name, region.geometry, torch.sigmoid(theta),
+                                                   eps_bounds=region.eps_bounds)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        plan, mons, session = differentiable_run(scene, stride=stride, **kw)
+        e = mons["objective"]
+        loss = 0.0
+        for c in ("Ex", "Ey"):
+            d = e[c]
+            a0, b0 = (d.shape[-2] - 40) // 2, (d.shape[-1] - 40) // 2
+            loss = loss + (d[..., a0:a0 + 40, b0:b0 + 40].abs() ** 2).sum()
+        torch.cuda.synchronize()
+        t_fwd = time.perf_counter() - t0
+        loss.backward()
+        torch.cuda.synchronize()
+        return time.perf_counter() - t0, t_fwd, plan, session, theta
+
+    one_gradient()  # warm-up (library load, first launches)
+    total, t_fwd, plan, session, theta = one_gradient()
+    T = plan.time_steps
+    run = session.run
     nodes = int(np.prod(plan.shape))
+    step_bytes, _, _ = algorithmic_bytes(plan, run)
+    n_ckpt = len(range(stride, T, stride))
+    alg = T * 3 * step_bytes + n_ckpt * 2 * 2 * state_bytes(run)
+    roofline = {"bound": "hbm", "achieved": alg / total / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": alg / total / HBM_PEAK, "traffic": None,
+                "kernel": "whole gradient (forward, replay and reverse step kernels)",
+                "alg_bytes_per_gradient": alg, "step_alg_bytes_per_node": step_bytes / nodes}
+    cpu = None
+    if not args.no_cpu:
+        print("[bench] cpu baseline (forward and reverse steps of the float32 port)", file=sys.stderr, flush=True)
+        t_f, t_r = cpu_gradient_rates(plan, steps=args.cpu_grad_steps)
+        per_grad = T * (2 * t_f + t_r)
+        cpu = {"value": 1.0 / per_grad, "unit": "gradients/s", "cores": _threads(), "kind": "port",
+               "cpu_model": host_info()["cpu_model"],
+               "sample": f"{args.cpu_grad_steps} forward steps ({t_f:.3f} s each) and {args.cpu_grad_steps} reverse "
+                         f"steps ({t_r:.3f} s each) of the same {plan.shape[0]}x{plan.shape[1]}x{plan.shape[2]}-node "
+                         f"scene, float32 OpenMP; gradient = {T} x (2 forward + 1 reverse) steps"}
     line = {"metric": "design gradients/s", "value": 1.0 / total, "unit": "gradients/s", "n_gpus": 1,
-            "steps": plan.time_steps, "warmup": 0, "ms_per_step": total * 1e3 / plan.time_steps,
+            "steps": T, "warmup": 1, "ms_per_step": total * 1e3 / T,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": "design region 200x200x120 cells (+2x12 CPML), 64x64x32 design voxels, Ez dipole, "
                                    "|E|^2 of a 40x40 objective plane at f0, checkpoints every 100 steps",
                        "nodes": list(plan.shape)},
+            "roofline": roofline, "cpu_baseline": cpu, "host": host_info(),
             "detail": {"forward_s": t_fwd, "backward_s": total - t_fwd,
-                       "forward_gnodes_per_s": nodes * plan.time_steps / t_fwd / 1e9,
+                       "forward_gnodes_per_s": nodes * T / t_fwd / 1e9,
                        "grad_norm": float(theta.grad.norm())}}
     print(json.dumps(line), flush=True)
 
@@ -203,8 +345,10 @@ def bench_batch(args):
     # one scene per launch sequence
     lat = []
     singles = []
+    step_bytes = 0
     for p in plans:
         r = DeviceRun(p, "cuda")
+        step_bytes += algorithmic_bytes(p, r)[0]
         r.run(min(8, T))  # warm-up
         r.reset()
         torch.cuda.synchronize()
@@ -234,12 +378,25 @@ def bench_batch(args):
                 err = max(err, float(np.max(np.abs(a - b)) / max(np.max(np.abs(a)), 1e-30)))
     batch.close()
     n = len(plans)
+    roofline = {"bound": "hbm", "achieved": step_bytes * T / t_batch / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": step_bytes * T / t_batch / HBM_PEAK, "traffic": None,
+                "kernel": "batched step (all scenes' step kernels)", "alg_bytes_per_step": step_bytes}
+    cpu = None
+    if not args.no_cpu:
+        mid = plans[int(np.argsort([int(np.prod(p.shape)) for p in plans])[len(plans) // 2])]
+        base = cpu_baseline(mid, steps=args.cpu_steps)
+        rate = base["value"] * 1e9  # node-updates/s of the port on a median-size scene
+        cpu = {"value": rate / (nodes * T), "unit": "scene-runs/s", "cores": base["cores"], "kind": "port",
+               "cpu_model": base["cpu_model"],
+               "sample": f"median-size scene {mid.shape}: {base['sample']}; scene-runs/s = node-updates/s over "
+                         f"the batch's {nodes} nodes x {T} steps"}
     line = {"metric": "batched forward scene-runs/s vs one scene per launch", "value": n / t_batch,
             "unit": "scene-runs/s", "n_gpus": 1, "steps": T, "warmup": 0, "ms_per_step": t_batch * 1e3 / T,
             "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": f"{n} scenes 64^3..192x192x128 cells (+2x10 CPML), 32x32x16 design region, "
                                    f"dipole, single-frequency flux plane, {T} steps, forward only",
                        "total_nodes": nodes, "xchunk_graph_steps_z_tiling": list(choice)},
+            "roofline": roofline, "cpu_baseline": cpu, "host": host_info(),
             "detail": {"batched_s": t_batch, "plain_s": t_plain, "speedup": t_plain / t_batch,
                        "batched_gnode_updates_per_s": nodes * T / t_batch / 1e9,
                        "plain_gnode_updates_per_s": nodes * T / t_plain / 1e9,
@@ -249,6 +406,102 @@ def bench_batch(args):
     print(json.dumps(line), flush=True)
 
 
+# ----------------------------------------------------- rocprofv3 summaries
+def _find(directory, suffix):
+    hits = sorted(glob.glob(os.path.join(directory, "**", f"*{suffix}"), recursive=True))
+    if not hits:
+        raise FileNotFoundError(f"no *{suffix} under {directory}")
+    return hits
+
+
+def kernel_stats(directory):
+    """Per-kernel calls, average and total duration from a ``rocprofv3
+    --kernel-trace --stats --output-format csv`` run."""
+    rows = []
+    for path in _find(directory, "kernel_stats.csv"):
+        with open(path) as fh:
+            rows.extend(csv.DictReader(fh))
+    out = {}
+    for r in rows:
+        name = r.get("Name") or r.get("KernelName") or r.get("Kernel_Name")
+        calls = int(float(r.get("Calls", 0)))
+        avg = float(r.get("AverageNs", r.get("Average", 0)))
+        tot = float(r.get("TotalDurationNs", r.get("TotalDuration", 0)))
+        out[name] = {"calls": calls, "avg_us": avg / 1e3, "total_ms": tot / 1e6}
+    return out
+
+
+def _short(name):
+    """Kernel family of a rocprofv3 kernel name (template arguments kept for the
+    step kernels, whose variants are timed separately)."""
+    if "__amd_rocclr_copyBuffer" in name:
+        return "copy"
+    for key in ("lean_step_kernel", "lean_batch_kernel", "zedge_step_kernel", "zedge_batch_kernel"):
+        if key in name:
+            return name[name.index(key):].split("(")[0]
+    return name.split("(")[0][:80]
+
+
+def counter_per_dispatch(directory, counter):
+    """Average counter value per dispatch of each kernel family; for the copy
+    probe, the largest dispatch (the probe's 256 MiB copy, not the small
+    device-to-device copies a run also makes)."""
+    vals = defaultdict(list)
+    for path in _find(directory, "counter_collection.csv"):
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                if r.get("Counter_Name") != counter:
+                    continue
+                vals[_short(r.get("Kernel_Name", ""))].append(float(r["Counter_Value"]))
+    return {k: (max(v) if k == "copy" else sum(v) / len(v)) for k, v in vals.items()}
+
+
+def traffic_summary(fetch_dir, write_dir):
+    """HBM bytes per dispatch: FETCH_SIZE + WRITE_SIZE (KiB) of each kernel, each
+    scaled by bytes / counter reading of the copy probe, which reads and writes
+    COPY_PROBE_BYTES."""
+    fetch = counter_per_dispatch(fetch_dir, "FETCH_SIZE")
+    write = counter_per_dispatch(write_dir, "WRITE_SIZE")
+    if not fetch.get("copy") or not write.get("copy"):
+        raise RuntimeError("the runs hold no copy-probe dispatch (run bench.py with --copy-probe)")
+    f_scale = COPY_PROBE_BYTES / (fetch["copy"] * 1024.0)
+    w_scale = COPY_PROBE_BYTES / (write["copy"] * 1024.0)
+    out = {"_calibration": {"fetch_scale": f_scale, "write_scale": w_scale}}
+    for k in fetch:
+        if k == "copy":
+            continue
+        f = fetch[k] * 1024.0 * f_scale
+        w = write.get(k, 0.0) * 1024.0 * w_scale
+        out[k] = {"fetch_bytes": f, "write_bytes": w, "total_bytes": f + w}
+    return out
+
+
+def summarize(argv):
+    if argv[0] == "stats":
+        for k, v in sorted(kernel_stats(argv[1]).items(), key=lambda kv: -kv[1]["total_ms"]):
+            print(f"{v['calls']:8d} calls  avg {v['avg_us']:10.2f} us  total {v['total_ms']:10.2f} ms  {k}")
+    elif argv[0] == "traffic":
+        print(json.dumps(traffic_summary(argv[1], argv[2]), indent=1))
+    else:
+        raise SystemExit("--summarize stats DIR | traffic FETCH_DIR WRITE_DIR")
+
+
+def _traffic_for(tiling, path, kernel):
+    """Measured HBM bytes of the step from a committed counter summary, only when
+    it was measured at the tiling this run uses."""
+    if not path or not os.path.exists(path):
+        return None
+    try:
+        data = json.load(open(path))
+    except (OSError, ValueError):
+        return None
+    measured = data.get("_tiling")
+    if tiling is None or measured is None or list(measured) != list(tiling):
+        return None
+    return data.get(kernel)
+
+
+# --------------------------------------------------------------------- main
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -257,6 +510,9 @@ def main():
     ap.add_argument("--cells", type=int, default=256)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--cpu-grad-steps", type=int, default=4)
+    ap.add_argument("--slope-steps", type=int, default=100, help="S of the two-point slope (0: skip the protocol)")
+    ap.add_argument("--slope-rounds", type=int, default=5)
     ap.add_argument("--two-kernel", action="store_true", help="use the two-launch (H, E) step instead of the single pass")
     ap.add_argument("--no-tune", action="store_true", help="keep the library's default single-pass tiling")
     ap.add_argument("--tiling", default="",
@@ -272,7 +528,11 @@ def main():
     ap.add_argument("--grad-steps", type=int, default=3000)
     ap.add_argument("--batch-count", type=int, default=32)
     ap.add_argument("--batch-steps", type=int, default=1500)
+    ap.add_argument("--summarize", nargs="+", metavar="ARG",
+                    help="stats DIR | traffic FETCH_DIR WRITE_DIR: summarise rocprofv3 CSV output and exit")
     args = ap.parse_args()
+    if args.summarize:
+        return summarize(args.summarize)
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU; --gpus {args.gpus} is not supported "
               f"(the FDTD path runs on a single device).", file=sys.stderr)
@@ -293,18 +553,25 @@ def main():
         scene, kw = configs.dielectric_boxes(cells=args.cells, pml=12, n_boxes=40, steps=max(2000, total))
         workload = (f"dielectric boxes {args.cells}^3 cells (+2x12 CPML), 40 boxes eps U[1,12], "
                     f"soft +z plane wave, 2 flux monitors x 8 freq")
+        cells = args.cells ** 3
+        traffic_file = "traffic_dielectric_boxes_256.json" if args.cells == 256 else None
     elif args.scene == "vacuum":
         scene, kw = configs.vacuum_dipole(cells=64, pml=10, steps=max(500, total))
         workload = ("vacuum 64^3 cells (+2x10 CPML), centred Ez ideal dipole, full-field DFT at 3 frequencies "
                     "+ point monitor")
+        cells = 64 ** 3
+        traffic_file = None
     else:
         scene, kw = configs.dispersive_spheres(steps=max(4000, total))
         workload = ("dispersive spheres 384x384x256 cells (+2x12 CPML), 30% Drude + 2-pole Lorentz, eps U[1,4] "
                     "background, TFSF slab plane wave, flux at 16 frequencies")
+        cells = 384 * 384 * 256
+        traffic_file = "traffic_dispersive_spheres_409.json"
     kw = dict(kw)
-    # room for the event-timed single launches after the timed run
+    # room for the event-timed single launches after the timed run and for the
+    # slope rounds (each from a reset state)
     reps = max(20, min(200, args.steps // 5))
-    kw["time_steps"] = max(int(kw["time_steps"]), total + reps + 2)
+    kw["time_steps"] = max(int(kw["time_steps"]), total + reps + 2, 2 * args.slope_steps + 2)
     t0 = time.perf_counter()
     plan = compile_plan(scene, **kw)
     compile_s = time.perf_counter() - t0
@@ -313,6 +580,8 @@ def main():
     tiling = None
     if runner.fused and args.tiling:
         tiling = tuple(int(v) for v in args.tiling.split(","))
+        if len(tiling) == 2:
+            tiling = tiling + (0,)
         runner.set_tiling(*tiling)
     elif runner.fused and not args.no_tune:
         # launch geometry of the single-pass step, timed on this run and reset
@@ -322,13 +591,8 @@ def main():
           flush=True)
     runner.run(args.warmup)
     torch.cuda.synchronize()
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record()
-    runner.run(args.steps)
-    end.record()
-    end.synchronize()
+    elapsed = timed_run(runner, args.steps)
     torch.cuda.synchronize()
-    elapsed = start.elapsed_time(end) * 1e-3
     ms_per_step = elapsed * 1e3 / args.steps
     value = nodes * args.steps / elapsed / 1e9
     print(f"[bench] {args.steps} steps in {elapsed:.3f}s ({ms_per_step:.4f} ms/step)", file=sys.stderr, flush=True)
@@ -344,24 +608,19 @@ def main():
         t_h = time_kernel(runner, "H", reps)
         dominant, t_dom, b_dom = ("update_e_kernel", t_e, e_bytes) if t_e >= t_h else ("update_h_kernel", t_h, h_bytes)
     achieved = b_dom / t_dom
-    # HBM bytes per launch from the committed FETCH_SIZE/WRITE_SIZE runs of the same
-    # workload and kernels (only where such a run exists for exactly this workload)
     traffic = None
-    tf = None
-    if runner.fused and args.scene == "boxes" and args.cells == 256:
-        tf = os.path.join(ROOT, "profiles", "traffic_dielectric_boxes_256.json")
-    elif runner.fused and args.scene == "dispersive":
-        tf = os.path.join(ROOT, "profiles", "traffic_dispersive_spheres_409.json")
-    if tf and os.path.exists(tf):
-        try:
-            traffic = json.load(open(tf)).get(dominant)
-        except Exception:
-            traffic = None
+    if runner.fused and traffic_file:
+        traffic = _traffic_for(tiling, os.path.join(ROOT, "profiles", traffic_file), dominant)
     roofline = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": traffic, "kernel": dominant,
                 "kernel_avg_us": t_dom * 1e6, "alg_bytes_per_launch": b_dom,
                 "step_alg_bytes_per_node": step_bytes / nodes,
                 "step_frac": step_bytes * args.steps / elapsed / HBM_PEAK}
+    protocol = None
+    if args.slope_steps > 0:
+        protocol = slope_protocol(runner, args.slope_steps, args.slope_rounds)
+        print(f"[bench] slope {protocol['slope_ms_per_step']:.4f} ms/step, A/A spread {protocol['aa_spread']:.3%}",
+              file=sys.stderr, flush=True)
     if args.copy_probe:
         src = torch.empty(64 * 1024 * 1024, dtype=torch.float32, device="cuda:0").fill_(1.0)
         dst = torch.empty_like(src)
@@ -380,7 +639,8 @@ def main():
                        "nodes": list(plan.shape), "courant": 0.99,
                        "step": "single-pass" if runner.fused else "two-kernel",
                        "tiling": None if tiling is None else dict(zip(("xchunk", "shell_chunk", "z_tiling"), tiling))},
-            "roofline": roofline, "cpu_baseline": cpu}
+            "physical_gcell_updates_per_s": cells * args.steps / elapsed / 1e9,
+            "roofline": roofline, "protocol": protocol, "cpu_baseline": cpu, "host": host_info()}
     print(json.dumps(line), flush=True)
 
 
