@@ -13,7 +13,8 @@
   flux plane 6 cells past the TFSF face sees it;
 * design-region gradient (225x225x145 nodes, 64x64x32 design voxels, 3000
   steps, checkpoints every 100): the adjoint gradient against central finite
-  differences (h = 1e-3 in theta) of device forward runs on 16 seeded voxels;
+  differences of device forward runs, along a random direction over every voxel
+  and on 16 seeded voxels (h = 1e-3 in theta);
 * scene batch (32 scenes 64^3 .. 192x192x128 cells): every scene of the batch
   against the same scene run alone, and the smallest and largest against the
   oracle.
@@ -102,18 +103,24 @@ def _objective(monitors):
 
 def test_design_region_gradient_finite_differences():
     """Adjoint gradient of the full design-region configuration against central
-    differences of device forward runs.  Each difference perturbs one design
-    voxel's theta by +-h, recomputes the design window's E coefficients with the
-    float64 formulas and writes them into the device run, whose forward run then
-    gives the perturbed loss.  The voxels are drawn (seeded) from those whose
-    adjoint gradient is within 1/10 of the largest: a float32 forward run
-    resolves a loss change of ~1e-6 of the loss, and a voxel far from the fields
-    moves it by less."""
+    differences of device forward runs.  Each difference perturbs theta, recomputes
+    the design window's E coefficients with the float64 formulas and writes them
+    into the device run, whose forward run then gives the perturbed loss.
+
+    * Directional: theta +- h d for a seeded random sign vector d over all
+      131072 voxels, h = 1e-2: (L+ - L-) / 2h against grad . d, within 1e-3.
+      The difference moves the loss by ~2e-3 of itself, far above the float32
+      resolution of a forward run (~1e-7 of the loss), so it resolves the
+      gradient to ~1e-4.
+    * Per voxel: 16 seeded voxels among those whose gradient is within 1/10 of
+      the largest, h = 1e-3.  One voxel moves the loss by only ~5e-7 of itself,
+      a few float32 resolutions, so each difference carries ~10% noise: the
+      check is the sign and 0.2 of the largest gradient."""
     import witwin_maxwell_amd as mw
     from witwin_maxwell_amd.fdtd import materials as mat
     from witwin_maxwell_amd.fdtd.adjoint import differentiable_run, window_coefficients
     from witwin_maxwell_amd.scene import prepare_scene
-    print("case: design region 200x200x120 cells, 3000 steps, stride 100, FD on 16 voxels", flush=True)
+    print("case: design region 200x200x120 cells, 3000 steps, stride 100, directional FD + FD on 16 voxels", flush=True)
     scene, kw = configs.design_region()
     theta = kw.pop("theta").clone().requires_grad_(True)
     region = scene.material_regions[0]
@@ -130,7 +137,6 @@ def test_design_region_gradient_finite_differences():
     box = tuple(slice(l, h) for l, h in zip(lo, hi))
     prepared = prepare_scene(scene)
     model = mat.NodeMaterialModel(prepared)
-    base_loss = None
 
     def forward_loss(th):
         eps_node = mat.node_permittivity(prepared, model, [torch.sigmoid(th)])
@@ -152,6 +158,15 @@ def test_design_region_gradient_finite_differences():
     base_loss = forward_loss(th0)
     assert abs(base_loss - float(loss)) <= 1e-5 * abs(base_loss), (base_loss, float(loss))
     rng = np.random.default_rng(0)
+    # directional derivative over every voxel
+    d = torch.as_tensor(rng.choice([-1.0, 1.0], size=grad.shape), dtype=th0.dtype)
+    h_dir = 1e-2
+    fd_dir = (forward_loss(th0 + h_dir * d) - forward_loss(th0 - h_dir * d)) / (2 * h_dir)
+    ad_dir = float((grad * d.numpy()).sum())
+    err_dir = abs(fd_dir - ad_dir) / abs(ad_dir)
+    print(f"directional: adjoint {ad_dir:.6e}, finite difference {fd_dir:.6e}, relative error {err_dir:.2e}",
+          flush=True)
+    # single voxels
     strong = np.flatnonzero(np.abs(grad).ravel() >= 0.1 * np.abs(grad).max())
     voxels = rng.choice(strong, size=16, replace=strong.size < 16)
     h = 1e-3
@@ -166,7 +181,9 @@ def test_design_region_gradient_finite_differences():
     fd, ad = np.array(fd), np.array(ad)
     err = np.max(np.abs(fd - ad)) / np.max(np.abs(ad))
     print("adjoint:", ad, "\nfinite differences:", fd, f"\nmax error / max |gradient| = {err:.2e}", flush=True)
-    assert err <= 1e-3, err
+    assert err_dir <= 1e-3, (fd_dir, ad_dir)
+    assert np.all(np.sign(fd) == np.sign(ad)), (fd, ad)
+    assert err <= 0.2, err
 
 
 def test_scene_batch_at_stated_sizes():
