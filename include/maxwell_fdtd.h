@@ -329,6 +329,22 @@ int mw_batch_create(const mw_program* const* programs, int32_t count, int32_t xc
 int mw_batch_run(mw_batch* batch, int32_t first, int32_t steps, int32_t graph_steps, void* stream);
 int mw_batch_destroy(mw_batch* batch);
 
+/* No-allocation batch run.  No counterpart in the reference: for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Advances steps [first, first+steps)
+ * of every scene like mw_batch_create + mw_batch_run (direct launches, no graph)
+ * + mw_batch_destroy, with the same kernels, but allocates nothing: the batch's
+ * descriptor tables and step counter live in a static device array of
+ * MW_NOALLOC_SCRATCH_BYTES on each GPU the library runs on (held until the
+ * process ends), rewritten by every call.  Tables that do not fit (about 150
+ * scenes) are refused with MW_ERR_ARGUMENT.  The call returns once the tables
+ * are on the device; the steps run on `stream`.  Calls that overlap on one GPU
+ * (another stream, another thread before the first call's steps finished) share
+ * that array and race. */
+#define MW_NOALLOC_SCRATCH_BYTES (1 << 20)
+int mw_batch_run_noalloc(const mw_program* const* programs, int32_t count, int32_t xchunk, int32_t first,
+                         int32_t steps, void* stream);
+
 /* Adjoint of a scene batch: one adjoint state and design window per scene
  * (the single-scene mw_reverse_steps / mw_capture_steps, batched: every launch
  * covers all scenes).  `forward` must be the batch the scenes advanced in; its

@@ -5,6 +5,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -679,6 +680,155 @@ int batch_step(const mw_batch* b, hipStream_t s, bool observe = true) {
   return check_launch("batch step_counter");
 }
 
+// Per-scene descriptor tables of a batch, gathered into a sink: VectorSink
+// (host vectors, uploaded to buffers the batch handle owns) or the fixed-capacity
+// sinks of the no-allocation entry point.  Fills the handle's counts and flags.
+struct VectorSink {
+  std::vector<mw::FusedArgs> args;
+  std::vector<mw::ShellArgs> shells;
+  std::vector<int> shell_prefix{0}, lean_prefix{0}, zedge_prefix{0};
+  std::vector<mw::BatchTerm> terms;
+  std::vector<mw_aux_line> aux;
+  std::vector<mw::BatchPlane> planes;
+  std::vector<mw::BatchPoint> points;
+  bool arg(const mw::FusedArgs& a) { args.push_back(a); return true; }
+  bool shell(const mw::ShellArgs& a) { shells.push_back(a); return true; }
+  bool prefixes(int shell_blocks, int lean_tiles, int zedge_tiles) {
+    shell_prefix.push_back(shell_prefix.back() + shell_blocks);
+    lean_prefix.push_back(lean_prefix.back() + lean_tiles);
+    zedge_prefix.push_back(zedge_prefix.back() + zedge_tiles);
+    return true;
+  }
+  bool term(const mw::BatchTerm& t) { terms.push_back(t); return true; }
+  bool line(const mw_aux_line& l) { aux.push_back(l); return true; }
+  bool plane(const mw::BatchPlane& q) { planes.push_back(q); return true; }
+  bool point(const mw::BatchPoint& q) { points.push_back(q); return true; }
+};
+
+template <class Sink>
+int collect_batch(const mw_program* const* programs, int count, int xchunk, Sink& sink, mw_batch* b) {
+  if (xchunk < 0 || xchunk > 256) return fail(MW_ERR_ARGUMENT, "batch: xchunk %d out of range", xchunk);
+  int rc;
+  const char* full = "batch: the scenes' tables exceed the no-allocation scratch (MW_NOALLOC_SCRATCH_BYTES)";
+  int shell_total = 0, lean_total = 0, zedge_total = 0;
+  b->count = count;
+  for (int si = 0; si < count; ++si) {
+    const mw_program* p = programs[si];
+    if (!p) return fail(MW_ERR_ARGUMENT, "batch: program %d is NULL", si);
+    const mw_grid* g = &p->grid;
+    if ((rc = check_grid(g)) || (rc = check_fields(&p->fields, true, true)) ||
+        (rc = check_fields(&p->fields_alt, true, true)) || (rc = check_cpml(g, &p->cpml, true)) ||
+        (rc = check_cpml(g, &p->cpml, false)) || (rc = check_ade(g, p->has_ade ? &p->ade : nullptr)) ||
+        (rc = check_terms(g, p->h_terms, p->n_h_terms, "h_terms", p->total_steps)) ||
+        (rc = check_terms(g, p->e_terms, p->n_e_terms, "e_terms", p->total_steps)))
+      return rc;
+    if (!p->fused || has_periodic(g)) return fail(MW_ERR_ARGUMENT, "batch: scene %d is not a single-pass program", si);
+    if ((p->n_h_terms > 0 && !p->h_terms_dev) || (p->n_e_terms > 0 && !p->e_terms_dev))
+      return fail(MW_ERR_ARGUMENT, "batch: scene %d has no device term descriptors", si);
+    const bool cpml = p->cpml.enabled != 0;
+    const bool ade = p->has_ade && (p->ade.count[0] + p->ade.count[1] + p->ade.count[2]) > 0;
+    const bool dft = p->nf_dft > 0;
+    const int mode = coef_mode(p->coeffs);
+    if (si == 0) {
+      b->cpml = cpml; b->ade = ade; b->dft = dft; b->mode = mode; b->total_steps = p->total_steps;
+    } else if (cpml != b->cpml || ade != b->ade || dft != b->dft) {
+      return fail(MW_ERR_ARGUMENT, "batch: scene %d differs in CPML/ADE/full-field-DFT use", si);
+    }
+    if (mode != b->mode) b->mode = 0;
+    b->total_steps = std::min(b->total_steps, p->total_steps);
+    for (int par = 0; par < 2; ++par) {
+      mw::FusedArgs a;
+      int nlean = 0;
+      build_fused_args(p, par, xchunk > 0 ? xchunk : p->xchunk, &a, &nlean);
+      a.dft_cos = p->dft_cos;  // the batch kernels add step * nf
+      a.dft_sin = p->dft_sin;
+      mw::ShellArgs sh;
+      build_shell_args(a, nlean, p->shell_chunk, &sh);
+      if (!sink.arg(a) || !sink.shell(sh)) return fail(MW_ERR_ARGUMENT, "%s", full);
+      if (par == 0) {
+        const int zt = mw::zedge_tiles(a);
+        if (!sink.prefixes(sh.total, nlean, zt)) return fail(MW_ERR_ARGUMENT, "%s", full);
+        shell_total += sh.total;
+        lean_total += nlean;
+        zedge_total += zt;
+      }
+    }
+    for (int f = 0; f < 2; ++f) {
+      const mw_term* t = f ? p->e_terms : p->h_terms;
+      for (int i = 0; i < (f ? p->n_e_terms : p->n_h_terms); ++i) {
+        if (t[i].kind != MW_TERM_SHIFTED) continue;
+        if (!t[i].values) return fail(MW_ERR_ARGUMENT, "batch: scene %d term %d has no value buffer", si, i);
+        if (!sink.term(mw::BatchTerm{t[i], p->dt})) return fail(MW_ERR_ARGUMENT, "%s", full);
+        b->n_terms++;
+        b->max_term_cells = std::max<int64_t>(b->max_term_cells, (int64_t)t[i].shape[0] * t[i].shape[1] * t[i].shape[2]);
+      }
+    }
+    if (p->aux.n_e > 0) {
+      const mw_aux_line& l = p->aux;
+      if (!l.e || !l.h || !l.decay_e || !l.curl_e || !l.decay_h || !l.curl_h || !l.source)
+        return fail(MW_ERR_ARGUMENT, "batch: scene %d auxiliary line arrays missing", si);
+      if (!sink.line(l)) return fail(MW_ERR_ARGUMENT, "%s", full);
+      b->n_aux++;
+      b->max_aux = std::max(b->max_aux, l.n_e);
+    }
+    if (p->n_planes < 0 || p->n_planes > MW_MAX_PLANES) return fail(MW_ERR_ARGUMENT, "batch: scene %d plane count", si);
+    if ((p->n_planes > 0 || p->n_points > 0) &&
+        (p->nf_obs <= 0 || !p->obs_e_cos || !p->obs_e_sin || !p->obs_h_cos || !p->obs_h_sin))
+      return fail(MW_ERR_ARGUMENT, "batch: scene %d observer weight tables missing", si);
+    auto comp_ptr = [](const mw_fields& f, int c) -> const float* { return c < 3 ? f.e[c] : f.h[c - 3]; };
+    for (int q = 0; q < p->n_planes; ++q) {
+      const mw_plane_obs& o = p->planes[q];
+      if (o.component < 0 || o.component > 5 || o.axis < 0 || o.axis > 2 || !o.re || !o.im)
+        return fail(MW_ERR_ARGUMENT, "batch: scene %d plane %d descriptor invalid", si, q);
+      mw::BatchPlane bp{};
+      bp.component = o.component; bp.axis = o.axis; bp.index = o.index; bp.nf = p->nf_obs;
+      component_shape(g, o.component, bp.shape);
+      if (o.index < 0 || o.index >= bp.shape[o.axis])
+        return fail(MW_ERR_ARGUMENT, "batch: scene %d plane %d index outside axis", si, q);
+      bp.field[0] = comp_ptr(p->fields_alt, o.component);  // state after an even step
+      bp.field[1] = comp_ptr(p->fields, o.component);
+      bp.re = o.re; bp.im = o.im;
+      bp.w_cos = o.component < 3 ? p->obs_e_cos : p->obs_h_cos;
+      bp.w_sin = o.component < 3 ? p->obs_e_sin : p->obs_h_sin;
+      const int* sh = bp.shape;
+      const int64_t cells = o.axis == 0 ? (int64_t)sh[1] * sh[2] : (o.axis == 1 ? (int64_t)sh[0] * sh[2] : (int64_t)sh[0] * sh[1]);
+      b->max_plane_cells = std::max(b->max_plane_cells, cells);
+      if (!sink.plane(bp)) return fail(MW_ERR_ARGUMENT, "%s", full);
+      b->n_planes++;
+    }
+    if (p->n_points > 0) {
+      if (!p->point_component || !p->point_linear || !p->point_re || !p->point_im)
+        return fail(MW_ERR_ARGUMENT, "batch: scene %d point observer arrays missing", si);
+      for (int q = 0; q < p->n_points; ++q) {
+        int32_t comp = -1, lin = -1;
+        if (hipMemcpy(&comp, p->point_component + q, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(&lin, p->point_linear + q, sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
+          return fail(MW_ERR_RUNTIME, "batch: cannot read scene %d point tables", si);
+        if (comp < 0 || comp > 5) return fail(MW_ERR_ARGUMENT, "batch: scene %d point %d component", si, q);
+        int sh[3];
+        component_shape(g, comp, sh);
+        if (lin < 0 || (int64_t)lin >= (int64_t)sh[0] * sh[1] * sh[2])
+          return fail(MW_ERR_ARGUMENT, "batch: scene %d point %d index outside the field", si, q);
+        mw::BatchPoint bq{};
+        bq.linear = lin; bq.nf = p->nf_obs; bq.stride = p->n_points;
+        bq.field[0] = comp_ptr(p->fields_alt, comp);
+        bq.field[1] = comp_ptr(p->fields, comp);
+        bq.re = p->point_re + q; bq.im = p->point_im + q;
+        bq.w_cos = comp < 3 ? p->obs_e_cos : p->obs_h_cos;
+        bq.w_sin = comp < 3 ? p->obs_e_sin : p->obs_h_sin;
+        if (!sink.point(bq)) return fail(MW_ERR_ARGUMENT, "%s", full);
+        b->n_points++;
+      }
+    }
+  }
+  b->total_shell = shell_total;
+  b->total_lean = lean_total;
+  b->total_zedge = zedge_total;
+  if (b->n_terms > 65535 || b->n_aux > 65535 || b->n_planes > 65535)
+    return fail(MW_ERR_ARGUMENT, "batch: too many terms, lines or planes for one launch");
+  return MW_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1122,137 +1272,17 @@ int mw_reverse_steps(const mw_program* p, const mw_adjoint* adj, const mw_window
 int mw_batch_create(const mw_program* const* programs, int32_t count, int32_t xchunk, mw_batch** out) {
   if (!programs || !out || count <= 0) return fail(MW_ERR_ARGUMENT, "batch: need programs, a count > 0 and an output");
   *out = nullptr;
-  if (xchunk < 0 || xchunk > 256) return fail(MW_ERR_ARGUMENT, "batch: xchunk %d out of range", xchunk);
   mw_batch* b = new mw_batch();
   auto bail = [&](int rc) { batch_free(b); return rc; };
   int rc;
-  std::vector<mw::FusedArgs> args;
-  std::vector<mw::ShellArgs> shells;
-  std::vector<int> prefix(1, 0), lean_prefix(1, 0), zedge_prefix(1, 0);  // shell blocks, lean / z-edge tiles
-  std::vector<mw::BatchTerm> terms;
-  std::vector<mw_aux_line> aux;
-  std::vector<mw::BatchPlane> planes;
-  std::vector<mw::BatchPoint> points;
-  b->count = count;
-  for (int si = 0; si < count; ++si) {
-    const mw_program* p = programs[si];
-    if (!p) return bail(fail(MW_ERR_ARGUMENT, "batch: program %d is NULL", si));
-    const mw_grid* g = &p->grid;
-    if ((rc = check_grid(g)) || (rc = check_fields(&p->fields, true, true)) ||
-        (rc = check_fields(&p->fields_alt, true, true)) || (rc = check_cpml(g, &p->cpml, true)) ||
-        (rc = check_cpml(g, &p->cpml, false)) || (rc = check_ade(g, p->has_ade ? &p->ade : nullptr)) ||
-        (rc = check_terms(g, p->h_terms, p->n_h_terms, "h_terms", p->total_steps)) ||
-        (rc = check_terms(g, p->e_terms, p->n_e_terms, "e_terms", p->total_steps)))
-      return bail(rc);
-    if (!p->fused || has_periodic(g)) return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d is not a single-pass program", si));
-    if ((p->n_h_terms > 0 && !p->h_terms_dev) || (p->n_e_terms > 0 && !p->e_terms_dev))
-      return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d has no device term descriptors", si));
-    const bool cpml = p->cpml.enabled != 0;
-    const bool ade = p->has_ade && (p->ade.count[0] + p->ade.count[1] + p->ade.count[2]) > 0;
-    const bool dft = p->nf_dft > 0;
-    const int mode = coef_mode(p->coeffs);
-    if (si == 0) {
-      b->cpml = cpml; b->ade = ade; b->dft = dft; b->mode = mode; b->total_steps = p->total_steps;
-    } else if (cpml != b->cpml || ade != b->ade || dft != b->dft) {
-      return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d differs in CPML/ADE/full-field-DFT use", si));
-    }
-    if (mode != b->mode) b->mode = 0;
-    b->total_steps = std::min(b->total_steps, p->total_steps);
-    for (int par = 0; par < 2; ++par) {
-      mw::FusedArgs a;
-      int nlean = 0;
-      build_fused_args(p, par, xchunk > 0 ? xchunk : p->xchunk, &a, &nlean);
-      a.dft_cos = p->dft_cos;  // the batch kernels add step * nf
-      a.dft_sin = p->dft_sin;
-      args.push_back(a);
-      mw::ShellArgs sh;
-      build_shell_args(a, nlean, p->shell_chunk, &sh);
-      shells.push_back(sh);
-      if (par == 0) {
-        prefix.push_back(prefix.back() + sh.total);
-        lean_prefix.push_back(lean_prefix.back() + nlean);
-        zedge_prefix.push_back(zedge_prefix.back() + mw::zedge_tiles(a));
-      }
-    }
-    for (int f = 0; f < 2; ++f) {
-      const mw_term* t = f ? p->e_terms : p->h_terms;
-      for (int i = 0; i < (f ? p->n_e_terms : p->n_h_terms); ++i) {
-        if (t[i].kind != MW_TERM_SHIFTED) continue;
-        if (!t[i].values) return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d term %d has no value buffer", si, i));
-        terms.push_back(mw::BatchTerm{t[i], p->dt});
-        b->max_term_cells = std::max<int64_t>(b->max_term_cells, (int64_t)t[i].shape[0] * t[i].shape[1] * t[i].shape[2]);
-      }
-    }
-    if (p->aux.n_e > 0) {
-      const mw_aux_line& l = p->aux;
-      if (!l.e || !l.h || !l.decay_e || !l.curl_e || !l.decay_h || !l.curl_h || !l.source)
-        return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d auxiliary line arrays missing", si));
-      aux.push_back(l);
-      b->max_aux = std::max(b->max_aux, l.n_e);
-    }
-    if (p->n_planes < 0 || p->n_planes > MW_MAX_PLANES) return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d plane count", si));
-    if ((p->n_planes > 0 || p->n_points > 0) &&
-        (p->nf_obs <= 0 || !p->obs_e_cos || !p->obs_e_sin || !p->obs_h_cos || !p->obs_h_sin))
-      return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d observer weight tables missing", si));
-    auto comp_ptr = [](const mw_fields& f, int c) -> const float* { return c < 3 ? f.e[c] : f.h[c - 3]; };
-    for (int q = 0; q < p->n_planes; ++q) {
-      const mw_plane_obs& o = p->planes[q];
-      if (o.component < 0 || o.component > 5 || o.axis < 0 || o.axis > 2 || !o.re || !o.im)
-        return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d plane %d descriptor invalid", si, q));
-      mw::BatchPlane bp{};
-      bp.component = o.component; bp.axis = o.axis; bp.index = o.index; bp.nf = p->nf_obs;
-      component_shape(g, o.component, bp.shape);
-      if (o.index < 0 || o.index >= bp.shape[o.axis])
-        return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d plane %d index outside axis", si, q));
-      bp.field[0] = comp_ptr(p->fields_alt, o.component);  // state after an even step
-      bp.field[1] = comp_ptr(p->fields, o.component);
-      bp.re = o.re; bp.im = o.im;
-      bp.w_cos = o.component < 3 ? p->obs_e_cos : p->obs_h_cos;
-      bp.w_sin = o.component < 3 ? p->obs_e_sin : p->obs_h_sin;
-      const int* sh = bp.shape;
-      const int64_t cells = o.axis == 0 ? (int64_t)sh[1] * sh[2] : (o.axis == 1 ? (int64_t)sh[0] * sh[2] : (int64_t)sh[0] * sh[1]);
-      b->max_plane_cells = std::max(b->max_plane_cells, cells);
-      planes.push_back(bp);
-    }
-    if (p->n_points > 0) {
-      if (!p->point_component || !p->point_linear || !p->point_re || !p->point_im)
-        return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d point observer arrays missing", si));
-      std::vector<int32_t> comp(p->n_points), lin(p->n_points);
-      if (hipMemcpy(comp.data(), p->point_component, p->n_points * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess ||
-          hipMemcpy(lin.data(), p->point_linear, p->n_points * sizeof(int32_t), hipMemcpyDeviceToHost) != hipSuccess)
-        return bail(fail(MW_ERR_RUNTIME, "batch: cannot read scene %d point tables", si));
-      for (int q = 0; q < p->n_points; ++q) {
-        if (comp[q] < 0 || comp[q] > 5) return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d point %d component", si, q));
-        int sh[3];
-        component_shape(g, comp[q], sh);
-        if (lin[q] < 0 || (int64_t)lin[q] >= (int64_t)sh[0] * sh[1] * sh[2])
-          return bail(fail(MW_ERR_ARGUMENT, "batch: scene %d point %d index outside the field", si, q));
-        mw::BatchPoint bq{};
-        bq.linear = lin[q]; bq.nf = p->nf_obs; bq.stride = p->n_points;
-        bq.field[0] = comp_ptr(p->fields_alt, comp[q]);
-        bq.field[1] = comp_ptr(p->fields, comp[q]);
-        bq.re = p->point_re + q; bq.im = p->point_im + q;
-        bq.w_cos = comp[q] < 3 ? p->obs_e_cos : p->obs_h_cos;
-        bq.w_sin = comp[q] < 3 ? p->obs_e_sin : p->obs_h_sin;
-        points.push_back(bq);
-      }
-    }
-  }
-  b->total_shell = prefix.back();
-  b->total_lean = lean_prefix.back();
-  b->total_zedge = zedge_prefix.back();
-  b->n_terms = (int)terms.size();
-  b->n_aux = (int)aux.size();
-  b->n_planes = (int)planes.size();
-  b->n_points = (int)points.size();
-  if (b->n_terms > 65535 || b->n_aux > 65535 || b->n_planes > 65535)
-    return bail(fail(MW_ERR_ARGUMENT, "batch: too many terms, lines or planes for one launch"));
-  if ((rc = upload(args, &b->d_args, "arguments")) || (rc = upload(shells, &b->d_shell, "shell arguments")) ||
-      (rc = upload(prefix, &b->d_shell_prefix, "shell block table")) ||
-      (rc = upload(zedge_prefix, &b->d_zedge_prefix, "z-edge tile table")) ||
-      (rc = upload(lean_prefix, &b->d_lean_prefix, "lean tile table")) || (rc = upload(terms, &b->d_terms, "terms")) ||
-      (rc = upload(aux, &b->d_aux, "auxiliary lines")) || (rc = upload(planes, &b->d_planes, "planes")) ||
-      (rc = upload(points, &b->d_points, "points")))
+  VectorSink v;
+  if ((rc = collect_batch(programs, count, xchunk, v, b))) return bail(rc);
+  if ((rc = upload(v.args, &b->d_args, "arguments")) || (rc = upload(v.shells, &b->d_shell, "shell arguments")) ||
+      (rc = upload(v.shell_prefix, &b->d_shell_prefix, "shell block table")) ||
+      (rc = upload(v.zedge_prefix, &b->d_zedge_prefix, "z-edge tile table")) ||
+      (rc = upload(v.lean_prefix, &b->d_lean_prefix, "lean tile table")) || (rc = upload(v.terms, &b->d_terms, "terms")) ||
+      (rc = upload(v.aux, &b->d_aux, "auxiliary lines")) || (rc = upload(v.planes, &b->d_planes, "planes")) ||
+      (rc = upload(v.points, &b->d_points, "points")))
     return bail(rc);
   if (hipMalloc((void**)&b->d_counter, sizeof(int)) != hipSuccess) return bail(fail(MW_ERR_RUNTIME, "batch: counter"));
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess ||
@@ -1260,6 +1290,158 @@ int mw_batch_create(const mw_program* const* programs, int32_t count, int32_t xc
       hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) != hipSuccess)
     return bail(fail(MW_ERR_RUNTIME, "batch: cannot create its stream/events"));
   *out = b;
+  return MW_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// No-allocation batch run.  The descriptor tables and the step counter of a
+// batch live in `scratch`, a static device array (each GPU holds its own copy
+// until the process ends); its address on the current device comes from
+// hipGetSymbolAddress once per device and is used wherever host code needs the
+// array.  The tables are assembled in a static host staging buffer of the same
+// layout and copied in before the steps.
+
+static __device__ __align__(256) unsigned char scratch[MW_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+alignas(256) unsigned char g_stage[MW_NOALLOC_SCRATCH_BYTES];
+std::mutex g_noalloc_mutex;
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices];
+
+// First pass: table sizes only.
+struct CountSink {
+  int64_t args = 0, shells = 0, prefixes_n = 1, terms = 0, lines = 0, planes = 0, points = 0;
+  bool arg(const mw::FusedArgs&) { ++args; return true; }
+  bool shell(const mw::ShellArgs&) { ++shells; return true; }
+  bool prefixes(int, int, int) { ++prefixes_n; return true; }
+  bool term(const mw::BatchTerm&) { ++terms; return true; }
+  bool line(const mw_aux_line&) { ++lines; return true; }
+  bool plane(const mw::BatchPlane&) { ++planes; return true; }
+  bool point(const mw::BatchPoint&) { ++points; return true; }
+};
+
+// Byte offsets of each table in the scratch (256-byte aligned).
+struct ScratchLayout {
+  size_t counter, args, shells, shell_prefix, lean_prefix, zedge_prefix, terms, aux, planes, points, total;
+};
+
+ScratchLayout scratch_layout(const CountSink& c) {
+  ScratchLayout L{};
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) & ~size_t(255);
+    return o;
+  };
+  L.counter = take(sizeof(int));
+  L.args = take(c.args * sizeof(mw::FusedArgs));
+  L.shells = take(c.shells * sizeof(mw::ShellArgs));
+  L.shell_prefix = take(c.prefixes_n * sizeof(int));
+  L.lean_prefix = take(c.prefixes_n * sizeof(int));
+  L.zedge_prefix = take(c.prefixes_n * sizeof(int));
+  L.terms = take(c.terms * sizeof(mw::BatchTerm));
+  L.aux = take(c.lines * sizeof(mw_aux_line));
+  L.planes = take(c.planes * sizeof(mw::BatchPlane));
+  L.points = take(c.points * sizeof(mw::BatchPoint));
+  L.total = off;
+  return L;
+}
+
+// Second pass: writes each table into the host staging buffer at its offset,
+// refusing anything beyond the counts of the first pass.
+struct StageSink {
+  unsigned char* base;
+  ScratchLayout L;
+  CountSink cap;
+  CountSink n;
+  template <typename T>
+  bool put(size_t off, int64_t& i, int64_t limit, const T& v) {
+    if (i >= limit) return false;
+    memcpy(base + off + (size_t)i * sizeof(T), &v, sizeof(T));
+    ++i;
+    return true;
+  }
+  bool arg(const mw::FusedArgs& a) { return put(L.args, n.args, cap.args, a); }
+  bool shell(const mw::ShellArgs& a) { return put(L.shells, n.shells, cap.shells, a); }
+  bool prefixes(int shell_blocks, int lean_tiles, int zedge_tiles) {
+    if (n.prefixes_n >= cap.prefixes_n) return false;
+    int* sp = (int*)(base + L.shell_prefix);
+    int* lp = (int*)(base + L.lean_prefix);
+    int* zp = (int*)(base + L.zedge_prefix);
+    const int64_t i = n.prefixes_n++;
+    sp[i] = sp[i - 1] + shell_blocks;
+    lp[i] = lp[i - 1] + lean_tiles;
+    zp[i] = zp[i - 1] + zedge_tiles;
+    return true;
+  }
+  bool term(const mw::BatchTerm& t) { return put(L.terms, n.terms, cap.terms, t); }
+  bool line(const mw_aux_line& l) { return put(L.aux, n.lines, cap.lines, l); }
+  bool plane(const mw::BatchPlane& q) { return put(L.planes, n.planes, cap.planes, q); }
+  bool point(const mw::BatchPoint& q) { return put(L.points, n.points, cap.points, q); }
+};
+
+}  // namespace
+
+extern "C" {
+
+int mw_batch_run_noalloc(const mw_program* const* programs, int32_t count, int32_t xchunk, int32_t first,
+                         int32_t steps, void* stream) {
+  if (!programs || count <= 0) return fail(MW_ERR_ARGUMENT, "no-allocation batch: need programs and a count > 0");
+  if (first < 0 || steps < 0) return fail(MW_ERR_ARGUMENT, "no-allocation batch: negative first step or step count");
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail(MW_ERR_RUNTIME, "no-allocation batch: no usable current device (%d)", dev);
+  std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+      return fail(MW_ERR_RUNTIME, "no-allocation batch: cannot resolve the scratch array on device %d: %s", dev,
+                  hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  unsigned char* dptr = (unsigned char*)g_scratch_ptr[dev];
+  int rc;
+  mw_batch sized;
+  CountSink counts;
+  if ((rc = collect_batch(programs, count, xchunk, counts, &sized))) return rc;
+  const ScratchLayout L = scratch_layout(counts);
+  if (L.total > (size_t)MW_NOALLOC_SCRATCH_BYTES)
+    return fail(MW_ERR_ARGUMENT, "no-allocation batch: the %d scenes' tables need %zu bytes, the scratch holds %d",
+                count, L.total, MW_NOALLOC_SCRATCH_BYTES);
+  mw_batch b;
+  StageSink stage{g_stage, L, counts, CountSink{}};
+  memset(g_stage + L.shell_prefix, 0, sizeof(int));
+  memset(g_stage + L.lean_prefix, 0, sizeof(int));
+  memset(g_stage + L.zedge_prefix, 0, sizeof(int));
+  if ((rc = collect_batch(programs, count, xchunk, stage, &b))) return rc;
+  if (first + (int64_t)steps > b.total_steps)
+    return fail(MW_ERR_ARGUMENT, "no-allocation batch: steps [%d, %d) outside the %d steps every scene holds", first,
+                first + steps, b.total_steps);
+  hipStream_t s = (hipStream_t)stream;
+  if (hipMemcpyAsync(dptr, g_stage, L.total, hipMemcpyHostToDevice, s) != hipSuccess)
+    return fail(MW_ERR_RUNTIME, "no-allocation batch: cannot copy the tables into the scratch");
+  // the staging buffer is the next call's: wait until this copy has read it
+  if (hipStreamSynchronize(s) != hipSuccess)
+    return fail(MW_ERR_RUNTIME, "no-allocation batch: the table copy did not complete");
+  b.d_counter = (int*)(dptr + L.counter);
+  b.d_args = (mw::FusedArgs*)(dptr + L.args);
+  b.d_shell = (mw::ShellArgs*)(dptr + L.shells);
+  b.d_shell_prefix = (int*)(dptr + L.shell_prefix);
+  b.d_lean_prefix = (int*)(dptr + L.lean_prefix);
+  b.d_zedge_prefix = (int*)(dptr + L.zedge_prefix);
+  b.d_terms = (mw::BatchTerm*)(dptr + L.terms);
+  b.d_aux = (mw_aux_line*)(dptr + L.aux);
+  b.d_planes = (mw::BatchPlane*)(dptr + L.planes);
+  b.d_points = (mw::BatchPoint*)(dptr + L.points);
+  hipLaunchKernelGGL(mw::step_counter_kernel, dim3(1), dim3(64), 0, s, b.d_counter, 1, first);
+  if ((rc = check_launch("no-allocation batch counter"))) return rc;
+  for (int i = 0; i < steps; ++i)
+    if ((rc = batch_step(&b, s))) return rc;
   return MW_OK;
 }
 

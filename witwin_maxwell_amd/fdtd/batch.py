@@ -144,6 +144,29 @@ class BatchRun:
         self.graph_steps = int(graph_steps)
 
 
+def run_batch_noalloc(runs, steps=None, xchunk: int = 0):
+    """Advance every scene of ``runs`` (single-pass ``DeviceRun``s at the same step
+    index) by ``steps`` steps through ``mw_batch_run_noalloc``: the batched
+    kernels with their tables in the library's static device scratch, so the
+    call allocates no device memory.  Returns the runs."""
+    runs = list(runs)
+    if not runs:
+        raise ValueError("a batch needs at least one scene")
+    first = runs[0].step_index
+    if any(r.step_index != first for r in runs) or any(not r.fused for r in runs):
+        raise ValueError("the scenes must be single-pass runs at the same step")
+    total = min(r.plan.time_steps for r in runs)
+    steps = total - first if steps is None else int(steps)
+    lib = runs[0].lib
+    progs = (ctypes.POINTER(nat.MwProgram) * len(runs))(*[ctypes.pointer(r.program) for r in runs])
+    with torch.cuda.device(runs[0].device):
+        rc = lib.mw_batch_run_noalloc(progs, len(runs), int(xchunk), first, steps, nat.current_stream())
+    nat.check(rc, lib)
+    for r in runs:
+        r.step_index = first + steps
+    return runs
+
+
 # ---------------------------------------------------------------- gradients
 class BatchGradientSession:
     """Checkpointed forward + reverse sweep of many scenes at once: the forward

@@ -111,7 +111,7 @@ EXPORTS = ("mw_update_magnetic", "mw_update_electric", "mw_add_source_patch", "m
            "mw_accumulate_dft", "mw_accumulate_plane", "mw_accumulate_points", "mw_run_steps", "mw_capture_steps",
            "mw_reverse_steps", "mw_batch_create", "mw_batch_run", "mw_batch_destroy",
            "mw_adjoint_batch_create", "mw_adjoint_batch_capture", "mw_adjoint_batch_reverse",
-           "mw_adjoint_batch_destroy", "mw_last_error",
+           "mw_adjoint_batch_destroy", "mw_batch_run_noalloc", "mw_last_error",
            "mw_abi_version", "mw_abi_struct_sizes")
 
 _LIB = None
@@ -148,6 +148,7 @@ def load_library(path=None):
     lib.mw_batch_create.argtypes = [P(P(MwProgram)), i32, i32, P(vp)]
     lib.mw_batch_run.argtypes = [vp, i32, i32, i32, vp]
     lib.mw_batch_destroy.argtypes = [vp]
+    lib.mw_batch_run_noalloc.argtypes = [P(P(MwProgram)), i32, i32, i32, i32, vp]
     lib.mw_adjoint_batch_create.argtypes = [vp, P(P(MwProgram)), P(P(MwAdjoint)), P(P(MwWindow)), i32, P(vp)]
     lib.mw_adjoint_batch_capture.argtypes = [vp, i32, i32, vp]
     lib.mw_adjoint_batch_reverse.argtypes = [vp, i32, i32, vp]
