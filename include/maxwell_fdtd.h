@@ -155,6 +155,29 @@ int mw_accumulate_plane(const float* field, int32_t nx, int32_t ny, int32_t nz, 
 int mw_accumulate_points(const float* field, const int32_t* linear_index, int32_t count, int32_t nf,
                          const float* wcos, const float* wsin, float* re, float* im, void* stream);
 
+/* One pole's current on one E component over `count` samples: kind 0 Debye
+ * (P' = decay P + drive E, J = (P' - P)/dt), 1 Drude (J' = decay J + drive E;
+ * polarization unused, may be NULL), 2 Lorentz (J' = decay J - restoring P +
+ * drive E, P' = P + dt J').  Replaces update{Debye,Drude,Lorentz}Current3D
+ * (kernels/dispersive.cu:18-70); the step kernels fuse it into the E update. */
+int mw_update_pole_current(int32_t kind, const float* e, const float* drive, float* polarization, float* current,
+                           int64_t count, float decay, float restoring, float dt, void* stream);
+
+/* e -= dt * current * inv_eps over `count` samples.  Replaces
+ * applyPolarizationCurrent3D (kernels/dispersive.cu:72-83). */
+int mw_apply_polarization_current(float* e, const float* current, const float* inv_eps, int64_t count, float dt,
+                                  void* stream);
+
+/* TFSF auxiliary line, one half step each: h[m] = decay[m] h[m] - curl[m] (e[m+1] -
+ * e[m]) for m < n_h; and e[n_e-1] = 0, e[source_index] = source_value, e[m] =
+ * decay[m] e[m] - curl[m] (h[m] - h[m-1]) for the other 0 < m < n_e - 1.  Replace
+ * updateAuxiliary{Magnetic,Electric}1D (kernels/sources.cu:803-839); mw_run_steps
+ * advances the line itself. */
+int mw_update_aux_magnetic(float* h, const float* e, const float* decay, const float* curl, int32_t n_h,
+                           void* stream);
+int mw_update_aux_electric(float* e, const float* h, const float* decay, const float* curl, int32_t n_e,
+                           int32_t source_index, float source_value, void* stream);
+
 /* ---- multi-step program: the whole per-step sequence with device-side step
  * tables, launched from native code (reference hot loop
  * fdtd/runtime/stepping.py:2289-2357 and its graph runners :1818-2024). ---- */

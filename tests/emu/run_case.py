@@ -92,4 +92,21 @@ d.add_source(mw.PointDipole((0.2 * dx, 0.3 * dx, -0.4 * dx), (0.5, 0.3, 1.0), pr
 dkw = {"frequencies": (configs.F0,), "courant": 0.9}
 run("dispersive, aligned z tiling", d, dkw, 12, (3, 2, 2))
 run("dispersive, z-edge tiles", d, dkw, 12, (0, 4, 1))
+# periodic x faces (H read across the seam by the shell's E pass), PMC y faces,
+# CPML z, with an interior box and a dielectric crossing the seam
+pz = mw.Scene(domain=mw.Domain(((-8 * dx, 8 * dx), (-7 * dx, 7 * dx), (-37 * dx, 37 * dx))),
+              grid=mw.GridSpec.uniform(dx),
+              boundary=mw.BoundarySpec.faces(x="periodic", y="pmc", z="pml", num_layers=5), device="cpu")
+pz.add_structure(mw.Structure(mw.Box((7 * dx, 0, 3 * dx), (6 * dx, 5 * dx, 8 * dx)), mw.Material(eps_r=3.0)))
+pz.add_source(mw.PointDipole((6.6 * dx, 0.3 * dx, -0.4 * dx), (0.5, 0.3, 1.0), profile="ideal",
+                             source_time=mw.GaussianPulse(configs.F0, configs.F0 / 4)))
+run("periodic x, PMC y, aligned z tiling", pz, {"frequencies": (configs.F0,), "courant": 0.9}, 10, (3, 2, 2))
+run("periodic x, PMC y, z-edge tiles", pz, {"frequencies": (configs.F0,), "courant": 0.9}, 10, (0, 4, 1))
+# periodic z faces: the z ends stay shell cells even when z-edge tiles are asked for
+pk = mw.Scene(domain=mw.Domain(((-6 * dx, 6 * dx), (-7 * dx, 7 * dx), (-37 * dx, 37 * dx))),
+              grid=mw.GridSpec.uniform(dx), boundary=mw.BoundarySpec.faces(default="pml", z="periodic",
+                                                                            num_layers=5), device="cpu")
+pk.add_source(mw.PointDipole((0.2 * dx, 0.3 * dx, 36.6 * dx), (0.5, 0.3, 1.0), profile="ideal",
+                             source_time=mw.GaussianPulse(configs.F0, configs.F0 / 4)))
+run("periodic z, z-edge tiles asked", pk, {"frequencies": (configs.F0,), "courant": 0.9}, 10, (0, 4, 1))
 print("OK")

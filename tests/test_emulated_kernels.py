@@ -49,3 +49,16 @@ def test_single_pass_step_matches_oracle_under_emulation(emulated_library):
     print(proc.stdout)
     assert proc.returncode == 0, (proc.stdout[-3000:], proc.stderr[-3000:])
     assert proc.stdout.strip().endswith("OK")
+
+
+def test_reference_named_ops_under_emulation(emulated_library):
+    """Every op of the NativeFDTDModule interface through the emulated library:
+    argument structs built from the reference's keywords index only inside their
+    arrays, and each op matches the oracle."""
+    lib, asan = emulated_library
+    env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0")
+    proc = subprocess.run([sys.executable, os.path.join(EMU, "run_shim.py"), lib], capture_output=True, text=True,
+                          env=env, timeout=900)
+    print(proc.stdout)
+    assert proc.returncode == 0, (proc.stdout[-3000:], proc.stderr[-3000:])
+    assert proc.stdout.strip().endswith("OK")

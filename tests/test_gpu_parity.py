@@ -185,9 +185,10 @@ def test_debye_drude_lorentz_mixed_media(fused):
     _check_monitors(ref, dev)
 
 
+@pytest.mark.parametrize("fused", FUSED)
 @pytest.mark.parametrize("kind", ["pec", "pmc", "periodic"])
-def test_boundary_modes(kind):
-    print(f"case: boundary {kind}, 18^3 cells, dipole, 200 steps (fused unless periodic)", flush=True)
+def test_boundary_modes(kind, fused):
+    print(f"case: boundary {kind}, 18^3 cells, dipole, 200 steps, fused={fused}", flush=True)
     dx = configs.DX
     L = 9 * dx
     scene = mw.Scene(domain=mw.Domain(((-L, L),) * 3), grid=mw.GridSpec.uniform(dx),
@@ -196,7 +197,33 @@ def test_boundary_modes(kind):
     scene.add_source(mw.PointDipole((0.3 * dx, 0.2 * dx, 0.1 * dx), (1.0, 0.5, 0.25), profile="ideal",
                                     source_time=mw.GaussianPulse(configs.F0, configs.F0 / 4)))
     scene.add_monitor(mw.PointMonitor("p", (2 * dx, -dx, 0.0), fields=("Ex", "Hz")))
-    plan, ref, dev = _run_both(scene, {"frequencies": (configs.F0,), "courant": 0.99}, 200)
+    plan, ref, dev = _run_both(scene, {"frequencies": (configs.F0,), "courant": 0.99}, 200, fused)
+    _check_fields(ref, dev)
+    _check_monitors(ref, dev)
+
+
+@pytest.mark.parametrize("tiling", [(3, 2, 2), (0, 4, 1)], ids=["aligned", "zedge-asked"])
+@pytest.mark.parametrize("faces", ["x_periodic_y_pmc", "z_periodic", "xy_periodic"])
+def test_periodic_faces_single_pass(faces, tiling):
+    """Periodic faces in the single-pass step (the shell's E pass reads H across
+    the seam), next to PMC/PEC/CPML faces, on grids long enough in z for interior
+    tiles and z-edge tiles; a dielectric and the source straddle the seam."""
+    print(f"case: single-pass periodic faces {faces}, tiling {tiling}", flush=True)
+    dx = configs.DX
+    if faces == "x_periodic_y_pmc":
+        bnd = mw.BoundarySpec.faces(x="periodic", y="pmc", z="pml", num_layers=6)
+    elif faces == "z_periodic":
+        bnd = mw.BoundarySpec.faces(default="pml", z="periodic", num_layers=6)
+    else:
+        bnd = mw.BoundarySpec.faces(x="periodic", y="periodic", z="pec", num_layers=6)
+    scene = mw.Scene(domain=mw.Domain(((-10 * dx, 10 * dx), (-9 * dx, 9 * dx), (-40 * dx, 40 * dx))),
+                     grid=mw.GridSpec.uniform(dx), boundary=bnd)
+    scene.add_structure(mw.Structure(mw.Box((9 * dx, 0, 3 * dx), (6 * dx, 5 * dx, 8 * dx)), mw.Material(eps_r=3.0)))
+    scene.add_source(mw.PointDipole((9.4 * dx, 0.3 * dx, 39.2 * dx), (0.5, 0.3, 1.0), profile="ideal",
+                                    source_time=mw.GaussianPulse(configs.F0, configs.F0 / 4)))
+    scene.add_monitor(mw.PointMonitor("p", (-9 * dx, 2 * dx, -38 * dx), fields=("Ex", "Hz")))
+    plan, ref, dev = _run_both(scene, {"frequencies": (configs.F0,), "courant": 0.95}, 240, True, tiling)
+    assert 2 in plan.bc_lo and plan.shape[2] >= 63 + 4
     _check_fields(ref, dev)
     _check_monitors(ref, dev)
 

@@ -333,12 +333,17 @@ int build_fused_args(const mw_program* p, int n, int chunk_override, mw::FusedAr
   // lean ranges [lo, hi) per axis: strictly interior and outside every psi slab
   auto low = [&](int q) { return cp.enabled ? cp.psi_low[q] : 0; };
   auto high = [&](int q, int size) { return (cp.enabled && cp.psi_high_len[q] > 0) ? cp.psi_high_start[q] : size; };
-  const int ilo = std::max({1, low(8), low(2)}), ihi = std::min({g->nx - 1, high(8, g->nx - 1), high(2, g->nx)});
-  const int jlo = std::max({2, low(6) + 1, low(0)}), jhi = std::min({g->ny - 1, high(6, g->ny - 1), high(0, g->ny)});
+  // x and y stop two samples short of the high face: the march loads E of plane
+  // i+1 (Ex has nx-1 planes) and of row j+1 (Ey has ny-1 rows) for every owned
+  // plane and row, so the last owned ones are nx-3 and ny-3 (without PML layers
+  // on an axis nothing else bounds it there)
+  const int ilo = std::max({1, low(8), low(2)}), ihi = std::min({g->nx - 2, high(8, g->nx - 1), high(2, g->nx)});
+  const int jlo = std::max({2, low(6) + 1, low(0)}), jhi = std::min({g->ny - 2, high(6, g->ny - 1), high(0, g->ny)});
   const int klo = std::max({1, low(7), low(1)}), khi = std::min({g->nz - 1, high(7, g->nz - 1), high(1, g->nz)});
   // z: owned columns start one after the halo lane; rows: row j-1 is recomputed
   // (jlo already accounts for it); planes: one halo plane before each chunk
-  const bool zedge = p->z_tiling == 1;
+  // z-edge tiles handle PML/PEC/PMC/none z faces; periodic z faces stay in the shell
+  const bool zedge = p->z_tiling == 1 && g->bc_lo[2] != MW_BC_PERIODIC;
   const AxisTiling tz = zedge ? tile_axis_z(g->nz, mw::kFOwnCols, klo, khi) : tile_axis(g->nz, mw::kFOwnCols, 1, klo, khi);
   const AxisTiling ty = tile_axis(g->ny, mw::kFRows, 0, jlo, jhi);
   // x chunk: short marches keep the planes that the blocks resident on one XCD
@@ -487,10 +492,13 @@ int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_field
   return MW_OK;
 }
 
-bool has_periodic(const mw_grid* g) {
+// The single-pass step reads H across a periodic seam from the face on the other
+// side, so a periodic axis must have both faces periodic.
+int check_periodic_pairs(const mw_grid* g) {
   for (int a = 0; a < 3; ++a)
-    if (g->bc_lo[a] == MW_BC_PERIODIC || g->bc_hi[a] == MW_BC_PERIODIC) return true;
-  return false;
+    if ((g->bc_lo[a] == MW_BC_PERIODIC) != (g->bc_hi[a] == MW_BC_PERIODIC))
+      return fail(MW_ERR_ARGUMENT, "axis %d is periodic on one face only", a);
+  return MW_OK;
 }
 
 // state of step n (fused mode double-buffers E/H by step parity)
@@ -722,7 +730,8 @@ int collect_batch(const mw_program* const* programs, int count, int xchunk, Sink
         (rc = check_terms(g, p->h_terms, p->n_h_terms, "h_terms", p->total_steps)) ||
         (rc = check_terms(g, p->e_terms, p->n_e_terms, "e_terms", p->total_steps)))
       return rc;
-    if (!p->fused || has_periodic(g)) return fail(MW_ERR_ARGUMENT, "batch: scene %d is not a single-pass program", si);
+    if (!p->fused) return fail(MW_ERR_ARGUMENT, "batch: scene %d is not a single-pass program", si);
+    if ((rc = check_periodic_pairs(g))) return rc;
     if ((p->n_h_terms > 0 && !p->h_terms_dev) || (p->n_e_terms > 0 && !p->e_terms_dev))
       return fail(MW_ERR_ARGUMENT, "batch: scene %d has no device term descriptors", si);
     const bool cpml = p->cpml.enabled != 0;
@@ -924,6 +933,45 @@ int mw_accumulate_points(const float* field, const int32_t* linear_index, int32_
   return check_launch("accumulate_points");
 }
 
+int mw_update_pole_current(int32_t kind, const float* e, const float* drive, float* polarization, float* current,
+                           int64_t count, float decay, float restoring, float dt, void* stream) {
+  if (!e || !drive || !current || (kind != 1 && !polarization))
+    return fail(MW_ERR_ARGUMENT, "update_pole_current: NULL pointer");
+  if (kind < 0 || kind > 2) return fail(MW_ERR_ARGUMENT, "update_pole_current: pole kind %d", kind);
+  if (count <= 0 || count > kMaxElems) return fail(MW_ERR_ARGUMENT, "update_pole_current: bad sample count");
+  if (kind != 1 && !(dt > 0.0f)) return fail(MW_ERR_ARGUMENT, "update_pole_current: dt must be > 0");
+  hipLaunchKernelGGL(mw::pole_current_kernel, dim3(grid_blocks(count)), dim3(256), 0, (hipStream_t)stream, kind, e,
+                     drive, polarization, current, count, decay, restoring, dt);
+  return check_launch("update_pole_current");
+}
+
+int mw_apply_polarization_current(float* e, const float* current, const float* inv_eps, int64_t count, float dt,
+                                  void* stream) {
+  if (!e || !current || !inv_eps) return fail(MW_ERR_ARGUMENT, "apply_polarization_current: NULL pointer");
+  if (count <= 0 || count > kMaxElems) return fail(MW_ERR_ARGUMENT, "apply_polarization_current: bad sample count");
+  hipLaunchKernelGGL(mw::apply_polarization_kernel, dim3(grid_blocks(count)), dim3(256), 0, (hipStream_t)stream, e,
+                     current, inv_eps, count, dt);
+  return check_launch("apply_polarization_current");
+}
+
+int mw_update_aux_magnetic(float* h, const float* e, const float* decay, const float* curl, int32_t n_h,
+                           void* stream) {
+  if (!h || !e || !decay || !curl) return fail(MW_ERR_ARGUMENT, "update_aux_magnetic: NULL pointer");
+  if (n_h <= 0) return fail(MW_ERR_ARGUMENT, "update_aux_magnetic: %d samples", n_h);
+  hipLaunchKernelGGL(mw::aux_magnetic_kernel, dim3(grid_blocks(n_h)), dim3(256), 0, (hipStream_t)stream, h, e, decay,
+                     curl, n_h);
+  return check_launch("update_aux_magnetic");
+}
+
+int mw_update_aux_electric(float* e, const float* h, const float* decay, const float* curl, int32_t n_e,
+                           int32_t source_index, float source_value, void* stream) {
+  if (!e || !h || !decay || !curl) return fail(MW_ERR_ARGUMENT, "update_aux_electric: NULL pointer");
+  if (n_e < 2) return fail(MW_ERR_ARGUMENT, "update_aux_electric: %d samples", n_e);
+  hipLaunchKernelGGL(mw::aux_electric_kernel, dim3(grid_blocks(n_e)), dim3(256), 0, (hipStream_t)stream, e, h, decay,
+                     curl, n_e, source_index, source_value);
+  return check_launch("update_aux_electric");
+}
+
 int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream) {
   if (!p) return fail(MW_ERR_ARGUMENT, "program is NULL");
   int rc;
@@ -964,7 +1012,7 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
       if (!p->dft_re[c] || !p->dft_im[c]) return fail(MW_ERR_ARGUMENT, "DFT accumulator %d missing", c);
   }
   if (p->fused) {
-    if (has_periodic(g)) return fail(MW_ERR_ARGUMENT, "the single-pass step does not support periodic faces");
+    if ((rc = check_periodic_pairs(g))) return rc;
     for (int c = 0; c < 3; ++c)
       if (!p->fields_alt.e[c] || !p->fields_alt.h[c]) return fail(MW_ERR_ARGUMENT, "fields_alt %d is NULL", c);
     if (p->cpml.enabled)

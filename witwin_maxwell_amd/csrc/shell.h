@@ -20,7 +20,8 @@
 //   shell_e: E^{n+1} of the shell cells from the complete H^{n+1/2}
 //
 // shell_h, lean and the z-edge tiles read only the step's input buffers and write
-// disjoint output cells; shell_e runs after them.  The update equations are those of the
+// disjoint output cells; shell_e runs after them, so it can read H across a
+// periodic seam (periodic faces are shell cells).  The update equations are those of the
 // two-kernel path (yee.h: kernels/magnetic.cu:62-117, kernels/electric.cu:463-553)
 // with the single-pass ordering of ADE, source terms, PEC and DFT.
 #pragma once
@@ -253,11 +254,16 @@ __device__ __forceinline__ void shell_h_tile(const ShellArgs& sa, int logical, i
                       G(a.psi_h_out[5]));
 }
 
-// Derivative at a boundary sample of the single-pass step: PMC mirrors (the
-// reference's kernels/electric.cu:23-41), every other non-periodic mode gives 0.
-__device__ __forceinline__ float shell_edge(bool low_end, int mode, float first, float last, float inv) {
-  if (mode != MW_BC_PMC) return 0.0f;
-  return low_end ? 2.0f * first * inv : -2.0f * last * inv;
+// Derivative at a boundary sample of the single-pass step (the reference's
+// kernels/electric.cu:23-41): `cur` / `prev` are the H samples at the sample's own
+// index and the one below, `wrap` the one across the periodic seam (index n-2 at
+// the low end, 0 at the high end).  Periodic wraps through the duplicated end
+// node, PMC mirrors, every other mode gives 0.  H^{n+1/2} is complete when the
+// shell's E pass runs, so the seam sample is there to read.
+__device__ __forceinline__ float shell_edge(bool low_end, int mode, float cur, float prev, float wrap, float inv) {
+  if (mode == MW_BC_PMC) return low_end ? 2.0f * cur * inv : -2.0f * prev * inv;
+  if (mode == MW_BC_PERIODIC) return low_end ? (cur - wrap) * inv : (wrap - prev) * inv;
+  return 0.0f;
 }
 
 template <bool DFT>
@@ -294,6 +300,8 @@ __device__ __forceinline__ void shell_e_cells(
             hzm = a.g.bc_hi[2];
   const bool jin = j > 0 && j < ny - 1;
   const bool kin = k > 0 && k < nz - 1;
+  // a periodic axis has both faces periodic (the host checks it)
+  const bool per_x = lx == MW_BC_PERIODIC, per_y = ly == MW_BC_PERIODIC, per_z = lz == MW_BC_PERIODIC;
   const float iey = G(a.g.inv_e[1])[j];
   const float iez = G(a.g.inv_e[2])[k];
   float bey = 0, cey = 0, key = 1, bez = 0, cez = 0, kez = 1;
@@ -368,13 +376,24 @@ __device__ __forceinline__ void shell_e_cells(
     if (!(iH && k > 0)) hy_km = 0.0f;
     if (!(jH && k > 0)) hx_km = 0.0f;
     if (!(kH && j > 0)) hx_jm = 0.0f;
+    // samples across a periodic seam, read only on the face samples of periodic
+    // axes (offset 0 elsewhere, value discarded)
+    const bool wi = per_x && (i == 0 || i == nx - 1), wj = per_y && (j == 0 || j == ny - 1),
+               wk = per_z && (k == 0 || k == nz - 1);
+    const int iw = i == 0 ? nx - 2 : 0, jw = j == 0 ? ny - 2 : 0, kw = k == 0 ? nz - 2 : 0;
+    const float wz_y = hz[(wj && iH) ? i * sHz + (uint32_t)jw * nz + k : 0];             // Ex: Hz across y
+    const float wy_z = hy[(wk && iH) ? i * sHy + (uint32_t)j * (nz - 1) + kw : 0];       // Ex: Hy across z
+    const float wz_x = hz[(wi && jH) ? (uint32_t)iw * sHz + (uint32_t)j * nz + k : 0];    // Ey: Hz across x
+    const float wx_z = hx[(wk && jH) ? i * sHx + (uint32_t)j * (nz - 1) + kw : 0];       // Ey: Hx across z
+    const float wy_x = hy[(wi && kH) ? (uint32_t)iw * sHy + (uint32_t)j * (nz - 1) + k : 0];  // Ez: Hy across x
+    const float wx_y = hx[(wj && kH) ? i * sHx + (uint32_t)jw * (nz - 1) + k : 0];       // Ez: Hx across y
     if (iH) {  // Ex (nx-1, ny, nz): a = dHz/dy, b = dHy/dz, curl = a - b
       const float jsum = ADE ? ade_advance(a.ad, 0, ox, exv) : 0.0f;
       const bool pec = face_pec(j, ny, ly, hym) || face_pec(k, nz, lz, hzm);
       float e_new = exv;
       if (!pec && !(face_inactive(j, ny, ly, hym) || face_inactive(k, nz, lz, hzm))) {
-        const float da = jin ? (hz_c - hz_jm) * iey : shell_edge(j == 0, j == 0 ? ly : hym, hz_c, hz_jm, iey);
-        const float db = kin ? (hy_c - hy_km) * iez : shell_edge(k == 0, k == 0 ? lz : hzm, hy_c, hy_km, iez);
+        const float da = jin ? (hz_c - hz_jm) * iey : shell_edge(j == 0, j == 0 ? ly : hym, hz_c, hz_jm, wz_y, iey);
+        const float db = kin ? (hy_c - hy_km) * iez : shell_edge(k == 0, k == 0 ? lz : hzm, hy_c, hy_km, wy_z, iez);
         float curl = da - db;
         if (CPML) {
           const float pa = l0 >= 0 ? bey * p0 + cey * da : 0.0f;
@@ -396,8 +415,8 @@ __device__ __forceinline__ void shell_e_cells(
       const bool pec = face_pec(i, nx, lx, hxm) || face_pec(k, nz, lz, hzm);
       float e_new = eyv;
       if (!pec && !(face_inactive(i, nx, lx, hxm) || face_inactive(k, nz, lz, hzm))) {
-        const float da = iin ? (hz_c - hz_m) * iex : shell_edge(i == 0, i == 0 ? lx : hxm, hz_c, hz_m, iex);
-        const float db = kin ? (hx_c - hx_km) * iez : shell_edge(k == 0, k == 0 ? lz : hzm, hx_c, hx_km, iez);
+        const float da = iin ? (hz_c - hz_m) * iex : shell_edge(i == 0, i == 0 ? lx : hxm, hz_c, hz_m, wz_x, iex);
+        const float db = kin ? (hx_c - hx_km) * iez : shell_edge(k == 0, k == 0 ? lz : hzm, hx_c, hx_km, wx_z, iez);
         float curl = db - da;
         if (CPML) {
           const float pa = l2 >= 0 ? bex * p2 + cex * da : 0.0f;
@@ -419,8 +438,8 @@ __device__ __forceinline__ void shell_e_cells(
       const bool pec = face_pec(i, nx, lx, hxm) || face_pec(j, ny, ly, hym);
       float e_new = ezv;
       if (!pec && !(face_inactive(i, nx, lx, hxm) || face_inactive(j, ny, ly, hym))) {
-        const float da = iin ? (hy_c - hy_m) * iex : shell_edge(i == 0, i == 0 ? lx : hxm, hy_c, hy_m, iex);
-        const float db = jin ? (hx_c - hx_jm) * iey : shell_edge(j == 0, j == 0 ? ly : hym, hx_c, hx_jm, iey);
+        const float da = iin ? (hy_c - hy_m) * iex : shell_edge(i == 0, i == 0 ? lx : hxm, hy_c, hy_m, wy_x, iex);
+        const float db = jin ? (hx_c - hx_jm) * iey : shell_edge(j == 0, j == 0 ? ly : hym, hx_c, hx_jm, wx_y, iey);
         float curl = da - db;
         if (CPML) {
           const float pa = l4 >= 0 ? bex * p4 + cex * da : 0.0f;

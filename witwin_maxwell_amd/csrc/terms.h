@@ -120,6 +120,61 @@ __global__ __launch_bounds__(256) void aux_line_e_kernel(mw_aux_line a, int step
   a.e[m] = a.decay_e[m] * a.e[m] - a.curl_e[m] * (a.h[m] - a.h[m - 1]);
 }
 
+// Standalone auxiliary-line updates with the reference's per-op arguments
+// (kernels/sources.cu:803-839): the hard source at `source_index`, the far end
+// held at zero, sample 0 left alone unless it is the source.
+__global__ __launch_bounds__(256) void aux_magnetic_kernel(float* __restrict__ h, const float* __restrict__ e,
+                                                           const float* __restrict__ decay,
+                                                           const float* __restrict__ curl, int n_h) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_h) return;
+  h[m] = decay[m] * h[m] - curl[m] * (e[m + 1] - e[m]);
+}
+
+__global__ __launch_bounds__(256) void aux_electric_kernel(float* __restrict__ e, const float* __restrict__ h,
+                                                           const float* __restrict__ decay,
+                                                           const float* __restrict__ curl, int n_e, int source_index,
+                                                           float source_value) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_e) return;
+  if (m == n_e - 1) { e[m] = 0.0f; return; }
+  if (m == source_index) { e[m] = source_value; return; }
+  if (m > 0) e[m] = decay[m] * e[m] - curl[m] * (h[m] - h[m - 1]);
+}
+
+// Standalone pole update over every sample (kernels/dispersive.cu:18-70):
+// Debye P' = d P + g E, J = (P' - P)/dt; Drude J' = d J + g E; Lorentz
+// J' = d J - r P + g E, P' = P + dt J'.  The fused E passes use ade_advance.
+__global__ __launch_bounds__(256) void pole_current_kernel(int kind, const float* __restrict__ e,
+                                                           const float* __restrict__ drive, float* __restrict__ pol,
+                                                           float* __restrict__ cur, int64_t n, float decay,
+                                                           float restoring, float dt) {
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n) return;
+  const float g = drive[l] * e[l];
+  if (kind == 0) {
+    const float prev = pol[l];
+    const float next = decay * prev + g;
+    pol[l] = next;
+    cur[l] = (next - prev) * (1.0f / dt);
+  } else if (kind == 1) {
+    cur[l] = decay * cur[l] + g;
+  } else {
+    const float nc = decay * cur[l] - restoring * pol[l] + g;
+    cur[l] = nc;
+    pol[l] = pol[l] + dt * nc;
+  }
+}
+
+// E -= dt J / eps_inf (kernels/dispersive.cu:72-83).
+__global__ __launch_bounds__(256) void apply_polarization_kernel(float* __restrict__ e, const float* __restrict__ cur,
+                                                                 const float* __restrict__ inv_eps, int64_t n,
+                                                                 float dt) {
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= n) return;
+  e[l] -= dt * cur[l] * inv_eps[l];
+}
+
 // ------------------------------------------------------------ spectral sums
 // re[f][l] += v[l]*wc[f]; the weights of one step are read once per block.
 __global__ __launch_bounds__(256) void accumulate_kernel(const float* __restrict__ v, int64_t n, int nf,

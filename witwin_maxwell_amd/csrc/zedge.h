@@ -9,7 +9,8 @@
 // neighbouring wavefronts on one XCD), plus what the lean kernel leaves out along
 // z only: lanes outside the grid, Hx/Hy/Ez absent at k = nz-1, the z-derivative
 // CPML terms (psi_hx_z, psi_hy_z ping-ponged; psi_ex_z, psi_ey_z in place) and the
-// Ex/Ey boundary modes of the z faces.  Along x and y these tiles are inside the
+// Ex/Ey boundary modes of the z faces (never periodic: the host keeps periodic z
+// faces in the shell).  Along x and y these tiles are inside the
 // lean box's ranges, so no x/y psi or x/y faces occur.
 #pragma once
 
@@ -183,7 +184,7 @@ __device__ __forceinline__ void zedge_march(
         float e = ec.ex;
         if (!pec && !face_inactive(k, nz, lz, hzm)) {
           const float da = (hz_c - hz_m) * iey;
-          const float db = kint ? (hy_c - hy_km) * iez : shell_edge(k == 0, k == 0 ? lz : hzm, hy_c, hy_km, iez);
+          const float db = kint ? (hy_c - hy_km) * iez : shell_edge(k == 0, k == 0 ? lz : hzm, hy_c, hy_km, 0.0f, iez);
           const float pb = l1 >= 0 ? bez * p1v + cez * db : 0.0f;
           if (l1 >= 0 && p1) p1[q1] = pb;
           e = ec.ex * coef(a.cf.decay[0], a.cf.decay_value[0], ox) +
@@ -207,7 +208,7 @@ __device__ __forceinline__ void zedge_march(
         float e = ec.ey;
         if (!pec && !face_inactive(k, nz, lz, hzm)) {
           const float da = (hz_c - hz_prev) * iex;
-          const float db = kint ? (hx_c - hx_km) * iez : shell_edge(k == 0, k == 0 ? lz : hzm, hx_c, hx_km, iez);
+          const float db = kint ? (hx_c - hx_km) * iez : shell_edge(k == 0, k == 0 ? lz : hzm, hx_c, hx_km, 0.0f, iez);
           const float pb = l3 >= 0 ? bez * p3v + cez * db : 0.0f;
           if (l3 >= 0 && p3) p3[q3] = pb;
           e = ec.ey * coef(a.cf.decay[1], a.cf.decay_value[1], oy) +

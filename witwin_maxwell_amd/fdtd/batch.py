@@ -40,7 +40,7 @@ class BatchRun:
         self.runs = list(runs)
         for r in self.runs:
             if not r.fused:
-                raise ValueError("batched scenes must use the single-pass step (no periodic faces)")
+                raise ValueError("batched scenes must use the single-pass step")
         self.lib = self.runs[0].lib
         self.xchunk = int(xchunk)
         self._handle = None

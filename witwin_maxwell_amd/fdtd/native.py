@@ -111,7 +111,8 @@ EXPORTS = ("mw_update_magnetic", "mw_update_electric", "mw_add_source_patch", "m
            "mw_accumulate_dft", "mw_accumulate_plane", "mw_accumulate_points", "mw_run_steps", "mw_capture_steps",
            "mw_reverse_steps", "mw_batch_create", "mw_batch_run", "mw_batch_destroy",
            "mw_adjoint_batch_create", "mw_adjoint_batch_capture", "mw_adjoint_batch_reverse",
-           "mw_adjoint_batch_destroy", "mw_batch_run_noalloc", "mw_last_error",
+           "mw_adjoint_batch_destroy", "mw_batch_run_noalloc", "mw_update_pole_current",
+           "mw_apply_polarization_current", "mw_update_aux_magnetic", "mw_update_aux_electric", "mw_last_error",
            "mw_abi_version", "mw_abi_struct_sizes")
 
 _LIB = None
@@ -149,6 +150,12 @@ def load_library(path=None):
     lib.mw_batch_run.argtypes = [vp, i32, i32, i32, vp]
     lib.mw_batch_destroy.argtypes = [vp]
     lib.mw_batch_run_noalloc.argtypes = [P(P(MwProgram)), i32, i32, i32, i32, vp]
+    f32 = ctypes.c_float
+    i64 = ctypes.c_int64
+    lib.mw_update_pole_current.argtypes = [i32, vp, vp, vp, vp, i64, f32, f32, f32, vp]
+    lib.mw_apply_polarization_current.argtypes = [vp, vp, vp, i64, f32, vp]
+    lib.mw_update_aux_magnetic.argtypes = [vp, vp, vp, vp, i32, vp]
+    lib.mw_update_aux_electric.argtypes = [vp, vp, vp, vp, i32, i32, f32, vp]
     lib.mw_adjoint_batch_create.argtypes = [vp, P(P(MwProgram)), P(P(MwAdjoint)), P(P(MwWindow)), i32, P(vp)]
     lib.mw_adjoint_batch_capture.argtypes = [vp, i32, i32, vp]
     lib.mw_adjoint_batch_reverse.argtypes = [vp, i32, i32, vp]

@@ -40,9 +40,9 @@ def _uniform_value(arr):
 class DeviceRun:
     """One scene resident in HBM.
 
-    ``fused=None`` picks the single-pass kernel (one launch per step, E/H and the
-    H-side psi double-buffered) unless a face is periodic, where the two-kernel
-    in-place path runs instead."""
+    ``fused=None`` (or True) runs the single-pass step (E, H and the H-side psi
+    double-buffered, one set of launches per step); ``fused=False`` the two-kernel
+    in-place path.  Both handle every face mode, periodic included."""
 
     def __init__(self, plan: FDTDPlan, device="cuda", fused=None):
         self.plan = plan
@@ -53,12 +53,7 @@ class DeviceRun:
         dev = self.device
         self.keep = []
         nx, ny, nz = plan.shape
-        periodic = 2 in plan.bc_lo or 2 in plan.bc_hi
-        if fused is None:
-            fused = not periodic
-        if fused and periodic:
-            raise ValueError("the single-pass step does not support periodic faces")
-        self.fused = bool(fused)
+        self.fused = True if fused is None else bool(fused)
         self._fields = [{c: torch.zeros(plan.field_shape(c), dtype=torch.float32, device=dev) for c in COMPONENTS}]
         if self.fused:
             self._fields.append({c: torch.zeros_like(t) for c, t in self._fields[0].items()})
