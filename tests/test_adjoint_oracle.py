@@ -184,3 +184,58 @@ def test_oracle_gradient_matches_central_differences(oracle_lib, bname):
         fd = (vals[0] - vals[1]) / (2 * h)
         assert abs(fd - g_node[idx]) <= 1e-6 * max(abs(fd), abs(g_node).max()), (idx, fd, g_node[idx])
     assert np.abs(g_node).max() > 0
+
+
+def test_filtered_projected_density_gradient_matches_central_differences(oracle_lib):
+    """dL/d(density) through normalisation to non-unit bounds, the box filter, the
+    tanh projection, trilinear sampling and the region blend (the torch chain of
+    fdtd/materials.py, compiler/materials.py:1310-1470 in the reference) on top of
+    the oracle adjoint, vs central differences of the oracle forward in the raw
+    density: pins the region compile's derivative, filter and projection included."""
+    from oracle.driver import OracleAdjoint, OracleRun
+    from witwin_maxwell_amd.fdtd import materials as mat
+    from witwin_maxwell_amd.scene import prepare_scene
+    rng = np.random.default_rng(9)
+    raw = torch.as_tensor(rng.uniform(0.15, 0.85, size=(5, 4, 3)), dtype=torch.float64)
+
+    def scene_with(dens):
+        scene = _design_scene(dens, BOUNDARIES["cpml"]())
+        r = scene.material_regions[0]
+        scene.material_regions[0] = mw.MaterialRegion(r.name, r.geometry, dens, bounds=(0.1, 0.9),
+                                                      eps_bounds=r.eps_bounds, filter_radius=1.2 * D,
+                                                      projection_beta=3.0)
+        return scene
+
+    steps = 50
+    kw = dict(frequencies=(4e14, 5e14), time_steps=steps, courant=0.9)
+    scene = scene_with(raw)
+    plan = compile_plan(scene, **kw)
+    run = OracleRun(plan)
+    run.trajectory = []
+    run.run(steps)
+    wre = [rng.standard_normal(a.shape) for a in run.plane_re]
+    wim = [rng.standard_normal(a.shape) for a in run.plane_im]
+    adj = OracleAdjoint(run)
+    adj.run_reverse({"plane_re": wre, "plane_im": wim})
+    dens = raw.clone().requires_grad_(True)
+    prepared = prepare_scene(scene)
+    model = mat.NodeMaterialModel(prepared)
+    eps_node = mat.node_permittivity(prepared, model, [dens])
+    total = 0.0
+    for ci, comp in enumerate(("Ex", "Ey", "Ez")):
+        decay, curl = mat.electric_coefficients(mat.average_node_to_component(eps_node, comp),
+                                                mat.average_node_to_component(model.sigma_e, comp), plan.dt)
+        total = total + (decay * torch.as_tensor(adj.g_decay[ci])).sum() + (curl * torch.as_tensor(adj.g_curl[ci])).sum()
+    (g,) = torch.autograd.grad(total, dens)
+    g = g.numpy()
+    assert np.abs(g).max() > 0
+    h = 1e-6
+    for idx in [(0, 0, 0), (2, 1, 1), (4, 3, 2), (1, 2, 0), (3, 0, 2)]:
+        vals = []
+        for sgn in (1, -1):
+            d2 = raw.clone()
+            d2[idx] += sgn * h
+            r2 = OracleRun(compile_plan(scene_with(d2), **kw)).run(steps)
+            vals.append(_loss_from_run(r2, wre, wim))
+        fd = (vals[0] - vals[1]) / (2 * h)
+        assert abs(fd - g[idx]) <= 1e-5 * np.abs(g).max(), (idx, fd, g[idx])

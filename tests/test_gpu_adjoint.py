@@ -163,3 +163,35 @@ def test_density_gradient_matches_oracle(bname, stride):
     g = dens.grad.numpy()
     assert np.abs(ref_grad).max() > 0
     assert norm_err(g, ref_grad) <= 1e-3, (g.ravel()[:6], ref_grad.ravel()[:6])
+
+
+def test_gpu_resident_filtered_projected_region_gradient():
+    """A MaterialRegion whose density lives on the GPU, with non-unit bounds, a
+    box filter and a tanh projection: the region compile (normalise, filter,
+    project, sample, blend, coefficients) runs on the device in float64 and the
+    density gradient of the checkpointed device adjoint equals the float64
+    oracle chain's (the same chain on the CPU over the oracle adjoint, itself
+    pinned by central differences in tests/test_adjoint_oracle.py)."""
+    from witwin_maxwell_amd.fdtd.adjoint import differentiable_run
+    print("case: GPU-resident filtered + projected region gradient", flush=True)
+    scene, density = _design_case("plain_cpml", seed=2)
+    r = scene.material_regions[0]
+
+    def with_density(d):
+        scene.material_regions[0] = mw.MaterialRegion(r.name, r.geometry, d, bounds=(0.05, 0.95),
+                                                      eps_bounds=r.eps_bounds, filter_radius=1.5 * D,
+                                                      projection_beta=4.0)
+        return scene
+
+    plan_kw = dict(frequencies=(4e14, 5e14), time_steps=140, courant=0.9)
+    ref_loss, ref_grad = _oracle_density_gradient(with_density(density), density, plan_kw, _flux_loss)
+    dens = density.detach().clone().to("cuda").requires_grad_(True)
+    plan, mons, session = differentiable_run(with_density(dens), region_densities=[dens], device="cuda", stride=23,
+                                             **plan_kw)
+    loss = _flux_loss(mons)
+    loss.backward()
+    assert dens.grad is not None and dens.grad.is_cuda
+    assert abs(float(loss) - ref_loss) <= 1e-4 * abs(ref_loss)
+    g = dens.grad.double().cpu().numpy()
+    assert np.abs(ref_grad).max() > 0
+    assert norm_err(g, ref_grad) <= 1e-3, (g.ravel()[:6], ref_grad.ravel()[:6])

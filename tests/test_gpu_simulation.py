@@ -58,3 +58,77 @@ def test_simulation_trainable_region_backward():
     ref_loss, ref_grad = ta._oracle_density_gradient(scene2, density, plan_kw, ta._flux_loss)
     assert abs(float(loss) - ref_loss) <= 1e-4 * abs(ref_loss)
     assert norm_err(dens.grad.numpy(), ref_grad) <= 1e-3
+
+
+def test_prepared_run_until_and_resume_equal_uninterrupted_run(tmp_path):
+    """Simulation.prepare() (simulation.py:590, :1830-1916): run() of a prepared
+    simulation equals Simulation.run(); run_until(k) returns a detached checkpoint,
+    saved and loaded with the safe loader, that a fresh prepare() resumes to the
+    same monitors and fields bit for bit; a prepared simulation runs once."""
+    from witwin_maxwell_amd.fdtd.resume import FDTDResumeCheckpoint
+    print("case: prepare / run_until / resume, dielectric boxes with soft source, 160 steps", flush=True)
+    scene, kw = configs.dielectric_boxes(cells=20, pml=6, n_boxes=4, box_range=(3, 7), seed=4)
+    scene.add_monitor(mw.PointMonitor("p", (configs.DX, 0, 2 * configs.DX), fields=("Ex", "Hy")))
+    make = lambda: mw.Simulation.fdtd(scene, frequencies=kw["frequencies"][:2], run_time=mw.TimeConfig(time_steps=160),
+                                      spectral_sampler=mw.SpectralSampler(window="none"))
+    full = make().run()
+    prepared = make().prepare()
+    again = prepared.run()
+    with pytest.raises(RuntimeError):
+        prepared.run()
+    ck = make().prepare().run_until(67)
+    assert ck.step == 67 and ck.total_steps == 160
+    ck.save(tmp_path / "ck.pt")
+    ck2 = FDTDResumeCheckpoint.load(tmp_path / "ck.pt")
+    resumed = make().prepare().run(resume_from=ck2)
+    for res in (again, resumed):
+        for c in ("Ex", "Ey", "Ez"):
+            assert torch.equal(res.tensor(c), full.tensor(c)), c
+        for name in ("flux_lo", "flux_hi", "p"):
+            a, b = full.monitor(name), res.monitor(name)
+            for comp, v in a["components"].items():
+                x = v["data"] if isinstance(v, dict) else v
+                y = b["components"][comp]
+                y = y["data"] if isinstance(y, dict) else y
+                assert torch.equal(x, y), (name, comp)
+    bad = FDTDResumeCheckpoint(step=ck.step, total_steps=ck.total_steps, dt=ck.dt * 2, tensors=ck.tensors,
+                               signature=ck.signature)
+    with pytest.raises(ValueError):
+        make().prepare().run(resume_from=bad)
+
+
+def test_scene_module_gradient():
+    """A SceneModule whose trainable parameter feeds its design region: the
+    Simulation materialises it, runs the checkpointed adjoint and the gradient
+    reaches the module's parameter, equal to the oracle chain's gradient."""
+    import test_gpu_adjoint as ta
+    print("case: SceneModule with a trainable region, gradient to its parameter", flush=True)
+    _, density = ta._design_case("plain_cpml", seed=3)
+    logit = torch.logit(density.clamp(0.05, 0.95))
+
+    class Module(mw.SceneModule):
+        def __init__(self):
+            super().__init__()
+            self.theta = torch.nn.Parameter(logit.clone())
+
+        def to_scene(self):
+            scene, _ = ta._design_case("plain_cpml", seed=3)
+            r = scene.material_regions[0]
+            scene.material_regions[0] = mw.MaterialRegion(r.name, r.geometry, torch.sigmoid(self.theta),
+                                                          eps_bounds=r.eps_bounds)
+            return scene
+
+    module = Module()
+    sim = mw.Simulation.fdtd(module, frequencies=(4e14, 5e14), run_time=mw.TimeConfig(time_steps=140),
+                             spectral_sampler=mw.SpectralSampler(window="none"), adjoint_checkpoint_stride=29)
+    res = sim.run()
+    loss = ta._flux_loss(res.monitors)
+    loss.backward()
+    plan_kw = dict(frequencies=(4e14, 5e14), time_steps=140, dt=res.solver_stats["dt"], window="none",
+                   full_field_dft=True)
+    scene2, _ = ta._design_case("plain_cpml", seed=3)
+    rho = torch.sigmoid(logit)
+    ref_loss, ref_grad_rho = ta._oracle_density_gradient(scene2, rho, plan_kw, ta._flux_loss)
+    ref_grad = ref_grad_rho * (rho * (1 - rho)).numpy()
+    assert abs(float(loss) - ref_loss) <= 1e-4 * abs(ref_loss)
+    assert norm_err(module.theta.grad.numpy(), ref_grad) <= 1e-3

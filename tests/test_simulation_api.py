@@ -95,3 +95,74 @@ def test_pole_activity_mask_matches_drives():
         seen_mixed |= bool((mask == 0).any() and (mask != 0).any())
     assert seen_mixed, "some samples carry no pole and some do"
     assert pole_activity([]) is None
+
+
+class _Boxes(mw.SceneModule):
+    """A scene factory with one trainable parameter feeding a design region."""
+
+    def __init__(self):
+        super().__init__()
+        self.theta = torch.nn.Parameter(torch.zeros(3, 3, 2, dtype=torch.float64))
+        self.calls = 0
+
+    def to_scene(self):
+        self.calls += 1
+        d = 5e-8
+        scene = mw.Scene(domain=mw.Domain(((0, 8 * d), (0, 8 * d), (0, 8 * d))), grid=mw.GridSpec.uniform(d),
+                         boundary=mw.BoundarySpec.pml(2), device="cpu")
+        scene.add_material_region(mw.MaterialRegion("r", mw.Box((4 * d, 4 * d, 4 * d), (3 * d, 3 * d, 2 * d)),
+                                                    torch.sigmoid(self.theta), eps_bounds=(1.0, 4.0)))
+        scene.add_source(mw.PointDipole((2 * d, 2 * d, 2 * d), "Ez", source_time=mw.GaussianPulse(4e14, 1e14)))
+        return scene
+
+
+def test_scene_module_input_is_materialised_per_run():
+    """Simulation.fdtd accepts a SceneModule (simulation.py:234-239) and builds the
+    scene again on every run (simulation.py:927-933), so a parameter update is
+    seen; anything else is refused."""
+    module = _Boxes()
+    sim = mw.Simulation.fdtd(module, frequency=4e14)
+    assert module.calls == 1 and sim.scene_module is module
+    assert sim.has_trainable_parameters
+    with torch.no_grad():
+        module.theta.fill_(2.0)
+    sim._refresh_scene()
+    assert module.calls == 2
+    assert torch.allclose(sim.scene.material_regions[0].density, torch.sigmoid(torch.full((3, 3, 2), 2.0,
+                                                                                          dtype=torch.float64)))
+    with pytest.raises(TypeError):
+        mw.Simulation.fdtd(object(), frequency=4e14)
+
+
+def test_resume_checkpoint_round_trip(tmp_path):
+    """A resume checkpoint saves and loads with the safe loader (weights_only),
+    keeping step, dt, tensors and layout signature (fdtd/resume.py:73-160)."""
+    from witwin_maxwell_amd.fdtd.resume import FDTDResumeCheckpoint
+    ck = FDTDResumeCheckpoint(step=7, total_steps=20, dt=1.5e-17,
+                              tensors={"field/Ex": torch.arange(6, dtype=torch.float32).reshape(1, 2, 3)},
+                              signature={"shape": [2, 3, 4], "cpml": True, "psi": {"psi_ex_y": [1, 2, 3]}})
+    path = tmp_path / "ck.pt"
+    ck.save(path)
+    back = FDTDResumeCheckpoint.load(path)
+    assert (back.step, back.total_steps, back.dt) == (7, 20, 1.5e-17)
+    assert torch.equal(back.tensors["field/Ex"], ck.tensors["field/Ex"]) and back.signature == ck.signature
+    torch.save({"data_type": "something else"}, tmp_path / "bad.pt")
+    with pytest.raises(ValueError):
+        FDTDResumeCheckpoint.load(tmp_path / "bad.pt")
+
+
+def test_result_selection_and_materials():
+    freqs = (1e14, 2e14)
+    ex = torch.arange(2 * 3 * 4 * 5, dtype=torch.float32).reshape(2, 3, 4, 5)
+    eps = torch.full((4, 4, 5), 2.5, dtype=torch.float64)
+    res = Result(method="fdtd", scene=None, prepared_scene=None, frequency=freqs[0], frequencies=freqs,
+                 fields={"Ex": ex}, monitors={}, solver_stats={}, materials={"eps_r": eps, "mu_r": eps * 0 + 1})
+    sel = res.at(freq_index=1)
+    assert torch.equal(sel.tensor("Ex"), ex[1]) and torch.equal(sel.E["EX"], ex[1])
+    assert torch.equal(res.select(frequency=1e14).tensor("Ex"), ex[0])
+    assert torch.equal(res.materials.eps, eps) and torch.equal(res.material("permittivity"), eps)
+    assert torch.equal(sel.materials.mu, eps * 0 + 1)
+    with pytest.raises(KeyError):
+        res.material("chi3")
+    with pytest.raises(ValueError):
+        res.at(frequency=1e14, freq_index=0)

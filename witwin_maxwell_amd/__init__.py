@@ -25,7 +25,7 @@ __all__ = [
 def __getattr__(name):
     # Simulation/Result pull in the native runtime; import them lazily so that the
     # scene API stays usable for compile-only work.
-    if name in ("Simulation", "FDTDConfig", "TimeConfig", "SpectralSampler", "Result", "run",
+    if name in ("Simulation", "PreparedSimulation", "FDTDConfig", "TimeConfig", "SpectralSampler", "Result", "run",
                 "calculate_required_steps"):
         from . import simulation
         return getattr(simulation, name)

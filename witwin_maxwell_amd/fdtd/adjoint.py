@@ -357,7 +357,7 @@ def differentiable_run(scene, region_densities=None, device="cuda", stride=None,
     for ci, c in enumerate(E_COMPS):
         box = tuple(slice(l, h) for l, h in zip(lo, hi))
         dev_curl = np.broadcast_to(plan.coef[c][1], plan.field_shape(c))[box]
-        if not np.allclose(coefs[ci][1].detach().numpy(), dev_curl, rtol=1e-12, atol=0):
+        if not np.allclose(coefs[ci][1].detach().cpu().numpy(), dev_curl, rtol=1e-12, atol=0):
             raise RuntimeError("window coefficients diverge from the compiled plan")
     flat = [d for d, _ in coefs] + [cu for _, cu in coefs]
     outs = _FDTDGradientFunction.apply(session, *flat)

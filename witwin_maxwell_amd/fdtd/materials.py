@@ -113,11 +113,13 @@ class NodeMaterialModel:
         return bool(self.poles)
 
 
-def prepare_region_density(prepared, region) -> torch.Tensor:
-    """Normalise, filter and project a region density (compiler/materials.py:1310-1360)."""
+def prepare_region_density(prepared, region, density=None) -> torch.Tensor:
+    """Normalise, filter and project a region density on its design grid
+    (compiler/materials.py:1310-1360), on the density's device in float64.
+    ``density`` defaults to ``region.density`` (the design variable)."""
     if not isinstance(region.geometry, Box):
         raise ValueError("MaterialRegion currently supports Box geometry only.")
-    density = region.density.to(dtype=torch.float64)
+    density = (region.density if density is None else density).to(dtype=torch.float64)
     lo, hi = region.bounds
     if not np.isclose(lo, 0.0) or not np.isclose(hi, 1.0):
         density = (density - lo) / max(hi - lo, 1e-12)
@@ -161,29 +163,42 @@ def sample_region_density(region, density, x, y, z):
     return (slice(i0, i1), slice(j0, j1), slice(k0, k1)), sampled
 
 
-def node_permittivity(prepared, model: NodeMaterialModel, region_densities=None) -> torch.Tensor:
-    """Node relative permittivity as a float64 torch tensor; design regions are
-    blended in with hard occupancy (compiler/materials.py:1393-1470), keeping the
-    autograd graph back to the region densities."""
-    eps = torch.as_tensor(model.eps_r, dtype=torch.float64)
+def node_permittivity(prepared, model: NodeMaterialModel, region_densities=None, device=None) -> torch.Tensor:
+    """Node relative permittivity as a float64 torch tensor with the design regions
+    blended in (compiler/materials.py:1393-1470): each region's density is
+    normalised, filtered and projected (``prepare_region_density``), sampled
+    trilinearly at the nodes of its box, mapped to eps_lo + rho (eps_hi - eps_lo)
+    and blended with the region's hard occupancy, which also clears the
+    conductivity and the dispersive poles there.  ``region_densities`` (default:
+    each region's ``density``) are the design variables; the autograd graph runs
+    back to them.  Everything runs on ``device`` (default: the first density's
+    device), so a HIP-resident density is compiled on the GPU."""
     regions = list(prepared.scene.material_regions)
+    densities = list(region_densities) if region_densities is not None else [r.density for r in regions]
+    if device is None:
+        device = densities[0].device if densities else torch.device("cpu")
+    eps = torch.as_tensor(model.eps_r, dtype=torch.float64, device=device)
     if not regions:
         return eps
     eps = eps.clone()
     x, y, z = prepared.x_nodes64, prepared.y_nodes64, prepared.z_nodes64
-    densities = region_densities if region_densities is not None else [None] * len(regions)
     for region, dens in zip(regions, densities):
-        if dens is None:
-            dens = prepare_region_density(prepared, region)
-        window, sampled = sample_region_density(region, dens.to("cpu"), x, y, z)
+        dens = prepare_region_density(prepared, region, dens.to(device))
+        window, sampled = sample_region_density(region, dens, x, y, z)
         if window is None:
             continue
-        occ = torch.as_tensor(region.geometry.contains(x[window[0]][:, None, None], y[window[1]][None, :, None],
-                                                       z[window[2]][None, None, :]), dtype=torch.float64)
+        occ_np = region.geometry.contains(x[window[0]][:, None, None], y[window[1]][None, :, None],
+                                          z[window[2]][None, None, :])
+        occ = torch.as_tensor(occ_np, dtype=torch.float64, device=device)
         lo_e, hi_e = region.eps_bounds
         value = lo_e + sampled * (hi_e - lo_e)
         eps[window] = eps[window] * (1.0 - occ) + occ * value
-        model.region_mask[window] |= occ.numpy().astype(bool)
+        hard = np.broadcast_to(np.asarray(occ_np, dtype=bool), sampled.shape)
+        model.region_mask[window] |= hard
+        # the region replaces the medium: no conductivity, no poles (materials.py:1471-1484)
+        model.sigma_e[window][hard] = 0.0
+        for _, _, weight in model.poles:
+            weight[window][hard] = 0.0
     return eps
 
 

@@ -127,7 +127,7 @@ def compile_plan(scene, *, frequencies=(), time_steps=None, dt=None, courant=Non
         eps_node = np.asarray(node_eps_override, dtype=np.float64)
     else:
         eps_node_t = _mat.node_permittivity(prepared, model, region_densities)
-        eps_node = eps_node_t.detach().numpy()
+        eps_node = eps_node_t.detach().cpu().numpy()
     coef, eps_edges = _mat.update_coefficients(prepared, model, eps_node, dt)
     ade = _mat.dispersive_templates(model, eps_edges, dt)
 

@@ -131,7 +131,9 @@ class Result:
     """FDTD result (result.py:1593-1745 subset): monitor payloads, frequency-domain
     (or last-step) E fields, solver statistics."""
 
-    def __init__(self, *, method, scene, prepared_scene, frequency, frequencies, fields, monitors, solver_stats):
+    def __init__(self, *, method, scene, prepared_scene, frequency, frequencies, fields, monitors, solver_stats,
+                 materials=None):
+        self._materials = dict(materials or {})
         self.method = method
         self.scene = scene
         self.prepared_scene = prepared_scene
@@ -198,6 +200,95 @@ class Result:
     def monitor(self, name: str, *, frequency=None, freq_index=None, resolve_modal: bool = True):
         return self.raw_monitor(name, frequency=frequency, freq_index=freq_index)
 
+    def material(self, name: str = "eps_r"):
+        """Node-sampled material of the compiled scene: ``eps_r`` (design regions
+        blended in), ``mu_r`` or ``sigma_e`` (result.py:1088-1092)."""
+        key = {"eps": "eps_r", "permittivity": "eps_r", "mu": "mu_r", "permeability": "mu_r"}.get(name, name)
+        if key not in self._materials:
+            raise KeyError(f"Material {name!r} is not available in this result.")
+        return self._materials[key]
+
+    @property
+    def materials(self) -> "ResultMaterials":
+        return ResultMaterials(self)
+
+    def at(self, *, frequency=None, freq_index=None) -> "ResultSelection":
+        """A view of this result at one frequency (result.py:1110-1130)."""
+        return ResultSelection(self, frequency=frequency, freq_index=freq_index)
+
+    select = at
+
+    @property
+    def E(self):
+        return self.field("E")
+
+    @property
+    def H(self):
+        return self.field("H")
+
+
+class ResultMaterials:
+    """``result.materials.eps`` / ``.mu`` (result.py:1025-1043)."""
+
+    def __init__(self, result):
+        self._result = result
+
+    @property
+    def eps(self):
+        return self._result.material("eps_r")
+
+    @property
+    def mu(self):
+        return self._result.material("mu_r")
+
+    permittivity = eps
+    permeability = mu
+
+
+class ResultSelection:
+    """A result seen at one frequency (result.py:1046-1130): every accessor passes
+    the selected frequency on."""
+
+    def __init__(self, result, frequency=None, freq_index=None):
+        if frequency is not None and freq_index is not None:
+            raise ValueError("Pass either frequency or freq_index, not both.")
+        self.result = result
+        self.frequency = frequency
+        self.freq_index = freq_index
+
+    @property
+    def E(self):
+        return self.field("E")
+
+    @property
+    def H(self):
+        return self.field("H")
+
+    @property
+    def materials(self):
+        return self.result.materials
+
+    def field(self, name: str = "E"):
+        return self.result.field(name, frequency=self.frequency, freq_index=self.freq_index)
+
+    def tensor(self, name: str):
+        return self.result.tensor(name, frequency=self.frequency, freq_index=self.freq_index)
+
+    def material(self, name: str = "eps_r"):
+        return self.result.material(name)
+
+    def raw_monitor(self, name: str):
+        return self.result.raw_monitor(name, frequency=self.frequency, freq_index=self.freq_index)
+
+    def monitor(self, name: str, *, resolve_modal: bool | None = None):
+        return self.result.monitor(name, frequency=self.frequency, freq_index=self.freq_index)
+
+    def at(self, *, frequency=None, freq_index=None):
+        return ResultSelection(self.result, frequency=frequency if frequency is not None else self.frequency,
+                               freq_index=freq_index if freq_index is not None else self.freq_index)
+
+    select = at
+
 
 def _select_frequency(payload, freqs, i):
     out = {}
@@ -216,15 +307,33 @@ def _select_frequency(payload, freqs, i):
     return out
 
 
+def _resolve_scene_input(scene_like):
+    """A Scene, or a SceneModule materialised by its ``to_scene()``
+    (simulation.py:234-239)."""
+    from .scene import Scene, SceneModule
+    if isinstance(scene_like, Scene):
+        return scene_like, None
+    if isinstance(scene_like, SceneModule):
+        return scene_like.to_scene(), scene_like
+    raise TypeError("scene must be a maxwell.Scene or maxwell.SceneModule.")
+
+
 class Simulation:
-    """simulation.py:434-549 (FDTD method)."""
+    """simulation.py:434-549 (FDTD method).  ``scene`` may be a ``SceneModule``:
+    it is materialised again by every ``run()`` / ``prepare()`` (simulation.py:
+    927-933), so trainable parameters feeding its regions are read fresh."""
 
     def __init__(self, scene, method, frequencies, config, excitations=None):
-        self.scene = scene
+        self.scene_input = scene
+        self.scene, self.scene_module = _resolve_scene_input(scene)
         self.method = method
         self.frequencies = tuple(frequencies)
         self.config = config
         self.excitations = tuple(excitations or ())
+
+    def _refresh_scene(self):
+        if self.scene_module is not None:
+            self.scene = self.scene_module.to_scene()
 
     @property
     def frequency(self):
@@ -297,25 +406,49 @@ class Simulation:
                                         rt.steady_cycles, rt.transient_cycles, domain_size, st)
 
     # ------------------------------------------------------------ run
-    def run(self) -> Result:
+    def _compile(self):
+        """The plan for the whole run and its step count (dt is fixed by the scene
+        and frequencies; a 1-step compile resolves it)."""
         from .fdtd.plan import compile_plan
-        self._validate()
-        device = getattr(self.scene, "device", "cuda")
         kw = self._plan_kwargs()
-        # dt is fixed by the scene and frequencies; a 1-step compile resolves it
         probe = compile_plan(self.scene, time_steps=1, **kw)
         steps = self._resolve_time_steps(probe)
+        return probe, steps, kw
+
+    def run(self) -> Result:
+        from .fdtd.plan import compile_plan
+        self._refresh_scene()
+        self._validate()
+        device = getattr(self.scene, "device", "cuda")
+        probe, steps, kw = self._compile()
         if self.has_trainable_parameters:
             from .fdtd.adjoint import differentiable_run
             plan, monitors, session = differentiable_run(self.scene, device=device, dt=probe.dt, time_steps=steps,
                                                          stride=self.config.adjoint_checkpoint_stride, **kw)
-            run = session.run
-        else:
-            from .fdtd.runtime import DeviceRun
-            plan = compile_plan(self.scene, time_steps=steps, dt=probe.dt, **kw)
-            run = DeviceRun(plan, device)
-            run.run()
-            monitors = run.monitor_results()
+            return self._result(plan, session.run, monitors, steps)
+        from .fdtd.runtime import DeviceRun
+        plan = compile_plan(self.scene, time_steps=steps, dt=probe.dt, **kw)
+        run = DeviceRun(plan, device)
+        run.run()
+        return self._result(plan, run, run.monitor_results(), steps)
+
+    def prepare(self) -> "PreparedSimulation":
+        """Compile the scene and upload it (simulation.py:590-640): the returned
+        ``PreparedSimulation`` runs once, from the start or from a resume
+        checkpoint, or advances to a step and returns a checkpoint."""
+        from .fdtd.plan import compile_plan
+        from .fdtd.runtime import DeviceRun
+        self._refresh_scene()
+        self._validate()
+        device = getattr(self.scene, "device", "cuda")
+        probe, steps, kw = self._compile()
+        if self.has_trainable_parameters:
+            return PreparedSimulation(self, None, steps)
+        plan = compile_plan(self.scene, time_steps=steps, dt=probe.dt, **kw)
+        return PreparedSimulation(self, DeviceRun(plan, device), steps)
+
+    def _result(self, plan, run, monitors, steps) -> Result:
+        device = getattr(self.scene, "device", "cuda")
         fields = {}
         if plan.dft_frequencies:
             from .fdtd.spectral import frequency_solution
@@ -333,8 +466,65 @@ class Simulation:
         monitors = _tensorize(monitors, device)
         stats = {"time_steps": steps, "dt": plan.dt, "shape": plan.shape, "fused": run.fused,
                  "full_field_dft": bool(plan.dft_frequencies)}
+        model = plan.material_model
+        materials = {"eps_r": torch.as_tensor(plan.node_eps, device=device),
+                     "mu_r": torch.as_tensor(model.mu_r, device=device),
+                     "sigma_e": torch.as_tensor(model.sigma_e, device=device)}
         return Result(method="fdtd", scene=self.scene, prepared_scene=plan.prepared, frequency=self.frequencies[0],
-                      frequencies=self.frequencies, fields=fields, monitors=monitors, solver_stats=stats)
+                      frequencies=self.frequencies, fields=fields, monitors=monitors, solver_stats=stats,
+                      materials=materials)
+
+
+class PreparedSimulation:
+    """A compiled, uploaded FDTD run (simulation.py:1830-1916) that executes once:
+    ``run()`` from step 0 or from a resume checkpoint, or ``run_until(step)``,
+    which advances exactly ``step`` steps and returns a detached checkpoint that a
+    fresh ``prepare()`` can resume (the finished result equals the uninterrupted
+    run's)."""
+
+    def __init__(self, simulation: Simulation, run, total_steps: int):
+        self.simulation = simulation
+        self.solver = run
+        self.total_steps = int(total_steps)
+        self._consumed = False
+
+    def _claim(self) -> None:
+        if self._consumed:
+            raise RuntimeError("A PreparedSimulation can execute only once; call Simulation.prepare() "
+                               "again for a fresh solver.")
+        self._consumed = True
+
+    def _validate_resume_support(self) -> None:
+        if self.simulation.has_trainable_parameters:
+            raise NotImplementedError("FDTD resume requires a detached forward run; the adjoint replays from "
+                                      "its own checkpoints.")
+
+    def run(self, *, resume_from=None) -> Result:
+        if resume_from is not None:
+            from .fdtd.resume import preflight_resume_checkpoint, restore_resume_checkpoint
+            self._validate_resume_support()
+            preflight_resume_checkpoint(self.solver, resume_from, total_steps=self.total_steps)
+            self._claim()
+            restore_resume_checkpoint(self.solver, resume_from, total_steps=self.total_steps)
+        else:
+            self._claim()
+            if self.simulation.has_trainable_parameters:
+                return self.simulation.run()
+        run = self.solver
+        run.run(self.total_steps - run.step_index)
+        return self.simulation._result(run.plan, run, run.monitor_results(), self.total_steps)
+
+    def run_until(self, step: int):
+        """Advance exactly ``step`` FDTD steps and return a detached resume state."""
+        from .fdtd.resume import capture_resume_checkpoint
+        self._validate_resume_support()
+        if isinstance(step, bool) or not isinstance(step, int):
+            raise TypeError("step must be an integer.")
+        if not 0 <= step < self.total_steps:
+            raise ValueError(f"step must satisfy 0 <= step < {self.total_steps} for this simulation.")
+        self._claim()
+        self.solver.run(step)
+        return capture_resume_checkpoint(self.solver, step=step, total_steps=self.total_steps)
 
 
 def source_spectrum(plan, source, frequencies, weights, window_sum, default_frequency):
