@@ -87,12 +87,13 @@ typedef struct mw_fields {
 } mw_fields;
 
 /* One Debye (0) / Drude (1) / Lorentz (2) pole on one E component; drive and the
- * state arrays have the component's shape (reference kernels/dispersive.cu:18-70). */
+ * state arrays have the component's shape (reference kernels/dispersive.cu:18-70).
+ * drive_scale: the pole's drive on a fully occupied edge, used with mw_ade.weight. */
 typedef struct mw_pole {
   int32_t kind;
   float decay;
   float restoring;
-  int32_t pad;
+  float drive_scale;
   const float* drive;
   float* polarization; /* unused (may be NULL) for Drude */
   float* current;
@@ -109,6 +110,15 @@ typedef struct mw_ade {
    * the drive arrays are read only where some pole acts.  No reference counterpart
    * (the reference streams every pole array at every sample). */
   const uint8_t* active[3];
+  /* Optional (NULL: the drive arrays are read): per-sample byte of the component's
+   * shape, two bits per pole (pole p in bits 2p..2p+1): 0 = the pole does not act
+   * there, 1 = half of an edge inside the medium (drive = drive_scale / 2), 2 = the
+   * whole edge (drive = drive_scale).  Node occupancy is 0 or 1, so every drive is
+   * one of those three values; with the codes the E passes read one byte instead
+   * of one float per pole.  On a sample whose E decay is 1 (no conductivity) the
+   * subtraction uses dt / eps_inf = the E curl coefficient and reads no inv_eps.
+   * No reference counterpart. */
+  const uint8_t* weight[3];
 } mw_ade;
 
 /* H <- H*decay - curl*(curl E) with CPML psi for the components in comp_mask

@@ -166,3 +166,28 @@ def test_result_selection_and_materials():
         res.material("chi3")
     with pytest.raises(ValueError):
         res.at(frequency=1e14, freq_index=0)
+
+
+def test_pole_weight_codes_rebuild_every_drive():
+    """The two-bit weight codes the step kernels read instead of the drive arrays:
+    code/2 * drive_scale is exactly the float32 drive at every sample, and a code
+    is 0 exactly where the drive is 0."""
+    from witwin_maxwell_amd import configs
+    from witwin_maxwell_amd.fdtd.materials import pole_weight_codes
+    from witwin_maxwell_amd.fdtd.plan import compile_plan
+    scene, kw = configs.dispersive_spheres(cells=(24, 24, 20), pml=4, steps=10, radius=(2, 5), device="cpu")
+    plan = compile_plan(scene, **kw)
+    seen = set()
+    for c in ("Ex", "Ey", "Ez"):
+        poles = plan.ade[c]["poles"]
+        codes = pole_weight_codes(poles)
+        assert codes is not None and codes.dtype == np.uint8
+        for p, pole in enumerate(poles):
+            code = (codes >> np.uint8(2 * p)) & np.uint8(3)
+            seen |= set(np.unique(code).tolist())
+            rebuilt = code.astype(np.float32) * np.float32(0.5) * np.float32(pole["drive_scale"])
+            np.testing.assert_array_equal(rebuilt, np.asarray(pole["drive"], dtype=np.float32))
+    assert seen == {0, 1, 2}
+    bad = [dict(plan.ade["Ex"]["poles"][0])]
+    bad[0]["drive"] = np.asarray(bad[0]["drive"]) * 0.3
+    assert pole_weight_codes(bad) is None

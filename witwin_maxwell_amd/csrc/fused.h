@@ -113,6 +113,16 @@ __device__ __forceinline__ float lean_e(const FusedArgs& a, int c, uint32_t o, f
   if (COEF == 2) return e_old + a.cf.curl_value[c] * curl;
   return e_old * coef(a.cf.decay[c], a.cf.decay_value[c], o) + coef(a.cf.curl[c], a.cf.curl_value[c], o) * curl;
 }
+// dt / eps_inf of the polarisation-current subtraction: modes 1 and 2 have E
+// decay 1, so it is the E curl coefficient; mode 0 decides per sample (ade_gain).
+template <int COEF>
+__device__ __forceinline__ float lean_ade_gain(const FusedArgs& a, int c, uint32_t o,
+                                               gptr<const float> __restrict__ curl_coef) {
+  if (COEF == 1) return curl_coef[o];
+  if (COEF == 2) return a.cf.curl_value[c];
+  return ade_gain(a.ad, c, o, coef(a.cf.decay[c], a.cf.decay_value[c], o), coef(a.cf.curl[c], a.cf.curl_value[c], o));
+}
+
 template <int COEF>
 __device__ __forceinline__ float lean_h(const FusedArgs& a, int c, uint32_t o, float h_old, float curl) {
   if (COEF == 1 || COEF == 2) return h_old - a.cf.curl_value[c] * curl;
@@ -185,12 +195,12 @@ __device__ __forceinline__ void lean_plane(const FusedArgs& a, int i, int j, int
     float ex = lean_e<COEF>(a, 0, ox, ec.ex, (hz_c - hz_m) * iey - (hy_c - hy_km) * iez, cex);
     float ey = lean_e<COEF>(a, 1, oy, ec.ey, (hx_c - hx_km) * iez - (hz_c - hz_prev) * iex, cey);
     float ez = lean_e<COEF>(a, 2, oz, ec.ez, (hy_c - hy_prev) * iex - (hx_c - hx_m) * iey, cez);
-    if (ADE) {  // 1/eps_inf is read only where a pole acted
+    if (ADE) {  // dt/eps_inf: the curl coefficient on lossless samples
       const float jx = ade_advance(a.ad, 0, ox, ec.ex), jy = ade_advance(a.ad, 1, oy, ec.ey);
       const float jz = ade_advance(a.ad, 2, oz, ec.ez);
-      if (jx != 0.0f) ex -= a.ad.dt * jx * G(a.ad.inv_eps[0])[ox];
-      if (jy != 0.0f) ey -= a.ad.dt * jy * G(a.ad.inv_eps[1])[oy];
-      if (jz != 0.0f) ez -= a.ad.dt * jz * G(a.ad.inv_eps[2])[oz];
+      if (jx != 0.0f) ex -= lean_ade_gain<COEF>(a, 0, ox, cex) * jx;
+      if (jy != 0.0f) ey -= lean_ade_gain<COEF>(a, 1, oy, cey) * jy;
+      if (jz != 0.0f) ez -= lean_ade_gain<COEF>(a, 2, oz, cez) * jz;
     }
     if (e_terms) {
       ex += term_value(a.e_terms, a.n_e_terms, 0, i, j, k, step, a.dt);

@@ -844,7 +844,7 @@ extern "C" {
 
 const char* mw_last_error(void) { return g_error.c_str(); }
 
-int32_t mw_abi_version(void) { return 2; }
+int32_t mw_abi_version(void) { return 3; }
 
 int32_t mw_abi_struct_sizes(int64_t* out, int32_t capacity) {
   const int64_t sizes[] = {(int64_t)sizeof(mw_grid),     (int64_t)sizeof(mw_coeffs),   (int64_t)sizeof(mw_cpml),

@@ -404,7 +404,7 @@ __device__ __forceinline__ void shell_e_cells(
         }
         e_new = exv * cdx + ccx * curl;
       }
-      if (ADE && jsum != 0.0f) e_new -= a.ad.dt * jsum * G(a.ad.inv_eps[0])[ox];
+      if (ADE && jsum != 0.0f) e_new -= ade_gain(a.ad, 0, ox, cdx, ccx) * jsum;
       if (et) e_new += term_value(a.e_terms, a.n_e_terms, 0, i, j, k, step, a.dt);
       if (pec) e_new = 0.0f;
       exo[ox] = e_new;
@@ -427,7 +427,7 @@ __device__ __forceinline__ void shell_e_cells(
         }
         e_new = eyv * cdy + ccy * curl;
       }
-      if (ADE && jsum != 0.0f) e_new -= a.ad.dt * jsum * G(a.ad.inv_eps[1])[oy];
+      if (ADE && jsum != 0.0f) e_new -= ade_gain(a.ad, 1, oy, cdy, ccy) * jsum;
       if (et) e_new += term_value(a.e_terms, a.n_e_terms, 1, i, j, k, step, a.dt);
       if (pec) e_new = 0.0f;
       eyo[oy] = e_new;
@@ -450,7 +450,7 @@ __device__ __forceinline__ void shell_e_cells(
         }
         e_new = ezv * cdz + ccz * curl;
       }
-      if (ADE && jsum != 0.0f) e_new -= a.ad.dt * jsum * G(a.ad.inv_eps[2])[oz];
+      if (ADE && jsum != 0.0f) e_new -= ade_gain(a.ad, 2, oz, cdz, ccz) * jsum;
       if (et) e_new += term_value(a.e_terms, a.n_e_terms, 2, i, j, k, step, a.dt);
       if (pec) e_new = 0.0f;
       ezo[oz] = e_new;

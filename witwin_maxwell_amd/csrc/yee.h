@@ -197,11 +197,50 @@ __device__ __forceinline__ float edge_diff(bool low_end, int mode, float first, 
   return 0.0f;
 }
 
+// One pole's advance from E^n at sample o with its drive there; returns the
+// pole's new polarisation current (kernels/dispersive.cu:18-70).
+__device__ __forceinline__ float pole_step(const mw_ade& a, const mw_pole& pl, uint32_t o, float drive, float e_old) {
+  if (pl.kind == 0) {
+    const float prev = G(pl.polarization)[o];
+    const float next = pl.decay * prev + drive * e_old;
+    G(pl.polarization)[o] = next;
+    const float cur = (next - prev) * (1.0f / a.dt);
+    G(pl.current)[o] = cur;
+    return cur;
+  }
+  if (pl.kind == 1) {
+    const float cur = pl.decay * G(pl.current)[o] + drive * e_old;
+    G(pl.current)[o] = cur;
+    return cur;
+  }
+  const float pc = G(pl.current)[o];
+  const float pp = G(pl.polarization)[o];
+  const float nc = pl.decay * pc - pl.restoring * pp + drive * e_old;
+  G(pl.current)[o] = nc;
+  G(pl.polarization)[o] = pp + a.dt * nc;
+  return nc;
+}
+
 // Fused ADE on one E sample: advance every pole from E^n, return the summed
-// polarisation current (kernels/dispersive.cu:18-70).
+// polarisation current (kernels/dispersive.cu:18-70).  Where a pole's drive is
+// zero its state starts at zero and stays zero and it adds nothing, so it is
+// skipped there (exact).  With the per-sample weight codes the drive is
+// code/2 * drive_scale and no drive array is read; else the activity mask (when
+// given) and the drive arrays are.
 __device__ __forceinline__ float ade_advance(const mw_ade& a, int c, uint32_t o, float e_old) {
   float jsum = 0.0f;
   const int n = a.count[c];
+  if (a.weight[c]) {
+    const uint32_t code = G(a.weight[c])[o];
+    if (code == 0u) return 0.0f;
+    for (int p = 0; p < n; ++p) {
+      const uint32_t w = (code >> (2 * p)) & 3u;
+      if (w == 0u) continue;
+      const mw_pole& pl = a.poles[c][p];
+      jsum += pole_step(a, pl, o, w == 2u ? pl.drive_scale : 0.5f * pl.drive_scale, e_old);
+    }
+    return jsum;
+  }
   // bit p: pole p acts on this sample (without the mask every pole is read)
   const uint32_t act = a.active[c] ? (uint32_t)G(a.active[c])[o] : 0xffu;
   if (act == 0u) return 0.0f;
@@ -209,30 +248,18 @@ __device__ __forceinline__ float ade_advance(const mw_ade& a, int c, uint32_t o,
     if (((act >> p) & 1u) == 0u) continue;
     const mw_pole& pl = a.poles[c][p];
     const float drive = G(pl.drive)[o];
-    // where a pole does not act its state starts at zero and stays zero, and it
-    // adds nothing: skip its state traffic (exact)
     if (drive == 0.0f) continue;
-    if (pl.kind == 0) {
-      const float prev = G(pl.polarization)[o];
-      const float next = pl.decay * prev + drive * e_old;
-      G(pl.polarization)[o] = next;
-      const float cur = (next - prev) * (1.0f / a.dt);
-      G(pl.current)[o] = cur;
-      jsum += cur;
-    } else if (pl.kind == 1) {
-      const float cur = pl.decay * G(pl.current)[o] + drive * e_old;
-      G(pl.current)[o] = cur;
-      jsum += cur;
-    } else {
-      const float pc = G(pl.current)[o];
-      const float pp = G(pl.polarization)[o];
-      const float nc = pl.decay * pc - pl.restoring * pp + drive * e_old;
-      G(pl.current)[o] = nc;
-      G(pl.polarization)[o] = pp + a.dt * nc;
-      jsum += nc;
-    }
+    jsum += pole_step(a, pl, o, drive, e_old);
   }
   return jsum;
+}
+
+// dt / eps_inf of sample o, the factor of the polarisation-current subtraction
+// E -= dt J / eps_inf (kernels/dispersive.cu:72-83): on a sample with E decay 1
+// (no conductivity) the E curl coefficient is exactly dt / eps, so it is used and
+// 1/eps_inf is not read; otherwise the inv_eps array is.
+__device__ __forceinline__ float ade_gain(const mw_ade& a, int c, uint32_t o, float decay, float curl) {
+  return decay == 1.0f ? curl : a.dt * G(a.inv_eps[c])[o];
 }
 
 // E update for all three components at node (i,j,k) (kernels/electric.cu:463-553):

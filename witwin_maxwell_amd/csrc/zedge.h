@@ -190,7 +190,9 @@ __device__ __forceinline__ void zedge_march(
           e = ec.ex * coef(a.cf.decay[0], a.cf.decay_value[0], ox) +
               coef(a.cf.curl[0], a.cf.curl_value[0], ox) * (da - (db * kez + pb));
         }
-        if (ADE && jsum != 0.0f) e -= a.ad.dt * jsum * G(a.ad.inv_eps[0])[ox];
+        if (ADE && jsum != 0.0f)
+          e -= ade_gain(a.ad, 0, ox, coef(a.cf.decay[0], a.cf.decay_value[0], ox),
+                        coef(a.cf.curl[0], a.cf.curl_value[0], ox)) * jsum;
         if (et) e += term_value(a.e_terms, a.n_e_terms, 0, i, j, k, step, a.dt);
         if (pec) e = 0.0f;
         exo[ox] = e;
@@ -214,7 +216,9 @@ __device__ __forceinline__ void zedge_march(
           e = ec.ey * coef(a.cf.decay[1], a.cf.decay_value[1], oy) +
               coef(a.cf.curl[1], a.cf.curl_value[1], oy) * ((db * kez + pb) - da);
         }
-        if (ADE && jsum != 0.0f) e -= a.ad.dt * jsum * G(a.ad.inv_eps[1])[oy];
+        if (ADE && jsum != 0.0f)
+          e -= ade_gain(a.ad, 1, oy, coef(a.cf.decay[1], a.cf.decay_value[1], oy),
+                        coef(a.cf.curl[1], a.cf.curl_value[1], oy)) * jsum;
         if (et) e += term_value(a.e_terms, a.n_e_terms, 1, i, j, k, step, a.dt);
         if (pec) e = 0.0f;
         eyo[oy] = e;
@@ -230,7 +234,9 @@ __device__ __forceinline__ void zedge_march(
         const float jsum = ADE ? ade_advance(a.ad, 2, oz, ec.ez) : 0.0f;
         float e = ec.ez * coef(a.cf.decay[2], a.cf.decay_value[2], oz) +
                   coef(a.cf.curl[2], a.cf.curl_value[2], oz) * ((hy_c - hy_prev) * iex - (hx_c - hx_m) * iey);
-        if (ADE && jsum != 0.0f) e -= a.ad.dt * jsum * G(a.ad.inv_eps[2])[oz];
+        if (ADE && jsum != 0.0f)
+          e -= ade_gain(a.ad, 2, oz, coef(a.cf.decay[2], a.cf.decay_value[2], oz),
+                        coef(a.cf.curl[2], a.cf.curl_value[2], oz)) * jsum;
         if (et) e += term_value(a.e_terms, a.n_e_terms, 2, i, j, k, step, a.dt);
         ezo[oz] = e;
         if (DFT) {

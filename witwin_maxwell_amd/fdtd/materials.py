@@ -259,20 +259,23 @@ def dispersive_templates(model: NodeMaterialModel, eps_edges, dt: float):
             if kind == "debye":
                 decay = (2.0 * pole.tau - dt) / (2.0 * pole.tau + dt)
                 scale = 2.0 * EPSILON_0 * pole.delta_eps * dt / (2.0 * pole.tau + dt)
-                entries.append({"kind": "debye", "decay": decay, "restoring": 0.0, "drive": weight * scale})
+                entries.append({"kind": "debye", "decay": decay, "restoring": 0.0, "drive": weight * scale,
+                                "drive_scale": scale})
             elif kind == "drude":
                 wp = 2.0 * np.pi * pole.plasma_frequency
                 g = 2.0 * np.pi * pole.gamma
                 denom = 2.0 + g * dt
+                scale = 2.0 * EPSILON_0 * wp * wp * dt / denom
                 entries.append({"kind": "drude", "decay": (2.0 - g * dt) / denom, "restoring": 0.0,
-                                "drive": weight * (2.0 * EPSILON_0 * wp * wp * dt / denom)})
+                                "drive": weight * scale, "drive_scale": scale})
             else:
                 w0 = 2.0 * np.pi * pole.resonance_frequency
                 g = 2.0 * np.pi * pole.gamma
                 denom = 2.0 + g * dt
+                scale = 2.0 * EPSILON_0 * pole.delta_eps * w0 * w0 * dt / denom
                 entries.append({"kind": "lorentz", "decay": (2.0 - g * dt) / denom,
                                 "restoring": 2.0 * w0 * w0 * dt / denom,
-                                "drive": weight * (2.0 * EPSILON_0 * pole.delta_eps * w0 * w0 * dt / denom)})
+                                "drive": weight * scale, "drive_scale": scale})
         out[comp] = {"inv_eps": 1.0 / eps_edges[comp], "poles": entries}
     return out
 
@@ -289,6 +292,27 @@ def material_characteristic_frequency(scene) -> float:
         for p in m.lorentz_poles:
             best = max(best, p.resonance_frequency, p.gamma)
     return best
+
+
+def pole_weight_codes(poles):
+    """Per-sample byte of one E component with two bits per pole: 0 where pole p
+    does not act, 1 where its drive is half its ``drive_scale`` (an edge with one
+    end node in the medium), 2 where it is the whole scale.  Node occupancy is 0 or
+    1, so every drive is one of those values and the step kernels rebuild it from
+    the code (exactly: halving a float32 is exact).  None without poles, with more
+    than four, or when some drive is not one of the three values."""
+    if not poles or len(poles) > 4:
+        return None
+    codes = np.zeros(np.shape(poles[0]["drive"]), dtype=np.uint8)
+    for p, pole in enumerate(poles):
+        drive = np.asarray(pole["drive"], dtype=np.float32)
+        full = np.float32(pole["drive_scale"])
+        half = np.float32(0.5) * full
+        code = np.where(drive == full, 2, np.where(drive == half, 1, 0)).astype(np.uint8)
+        if np.any((code == 0) & (drive != 0)):
+            return None
+        codes |= code << np.uint8(2 * p)
+    return codes
 
 
 def pole_activity(poles):

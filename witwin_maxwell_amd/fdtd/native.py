@@ -51,12 +51,12 @@ class MwFields(ctypes.Structure):
 
 class MwPole(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int32), ("decay", ctypes.c_float), ("restoring", ctypes.c_float),
-                ("pad", ctypes.c_int32), ("drive", vp), ("polarization", vp), ("current", vp)]
+                ("drive_scale", ctypes.c_float), ("drive", vp), ("polarization", vp), ("current", vp)]
 
 
 class MwAde(ctypes.Structure):
     _fields_ = [("count", ctypes.c_int32 * 3), ("dt", ctypes.c_float), ("inv_eps", vp * 3),
-                ("poles", (MwPole * MW_MAX_POLES) * 3), ("active", vp * 3)]
+                ("poles", (MwPole * MW_MAX_POLES) * 3), ("active", vp * 3), ("weight", vp * 3)]
 
 
 class MwTerm(ctypes.Structure):

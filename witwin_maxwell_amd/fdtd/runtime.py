@@ -16,7 +16,7 @@ import torch
 
 from . import native as nat
 from .cpml import PSI_NAMES, PSI_SPECS, compact_shape
-from .materials import pole_activity
+from .materials import pole_activity, pole_weight_codes
 from .plan import COMPONENTS, FDTDPlan
 
 _KIND = {"debye": 0, "drude": 1, "lorentz": 2}
@@ -122,12 +122,16 @@ class DeviceRun:
                 has_ade = True
                 ad.inv_eps[ci] = self._dev(plan.ade[c]["inv_eps"])
                 ad.active[ci] = self._dev(pole_activity(poles), dtype=np.uint8)
+                codes = pole_weight_codes(poles)
+                if codes is not None:
+                    ad.weight[ci] = self._dev(codes, dtype=np.uint8)
             states = []
             for pi, pole in enumerate(poles):
                 pl = ad.poles[ci][pi]
                 pl.kind = _KIND[pole["kind"]]
                 pl.decay = pole["decay"]
                 pl.restoring = pole["restoring"]
+                pl.drive_scale = pole.get("drive_scale", 0.0)
                 pl.drive = self._dev(pole["drive"])
                 cur = torch.zeros(plan.field_shape(c), dtype=torch.float32, device=dev)
                 pol = torch.zeros_like(cur) if pole["kind"] != "drude" else None
