@@ -515,6 +515,8 @@ def main():
     ap.add_argument("--slope-rounds", type=int, default=5)
     ap.add_argument("--two-kernel", action="store_true", help="use the two-launch (H, E) step instead of the single pass")
     ap.add_argument("--no-tune", action="store_true", help="keep the library's default single-pass tiling")
+    ap.add_argument("--plane-stage-depth", type=int, default=32,
+                    help="steps per block of staged z-normal plane monitors (0: sample them after every step)")
     ap.add_argument("--tiling", default="",
                     help="XCHUNK,SHELL_CHUNK[,Z_TILING]: fixed single-pass tiling instead of autotuning")
     ap.add_argument("--copy-probe", action="store_true",
@@ -575,7 +577,7 @@ def main():
     t0 = time.perf_counter()
     plan = compile_plan(scene, **kw)
     compile_s = time.perf_counter() - t0
-    runner = DeviceRun(plan, "cuda:0", fused=not args.two_kernel)
+    runner = DeviceRun(plan, "cuda:0", fused=not args.two_kernel, plane_stage_depth=args.plane_stage_depth)
     nodes = int(np.prod(plan.shape))
     tiling = None
     if runner.fused and args.tiling:

@@ -286,6 +286,22 @@ typedef struct mw_program {
   const mw_term* h_terms_dev;
   const mw_term* e_terms_dev;
   mw_aux_line aux;
+  /* Staged z-normal plane monitors (NULL = every plane added to its sums after
+   * each step).  After step n the samples of every plane with axis 2 are copied
+   * into stage row n % plane_stage_depth; when a block of plane_stage_depth steps
+   * completes, and at the end of each mw_run_steps call, one kernel adds the
+   * block's rows to the planes' sums in step order (the roundings of per-step
+   * accumulation).  A row holds each distinct (component, index) of the axis-2
+   * planes once, in `planes` order, in the sums' layout, so plane_stage needs
+   * plane_stage_depth x (sum of their cells) floats.  A call
+   * that ends inside a block folds that block's rows so far and zeroes them, so a
+   * later call that continues the block adds each step once.  The stage must be
+   * zero before the first call and whenever the next call does not start where
+   * the last one ended (a restart or a restored state). */
+  float* plane_stage;
+  int64_t plane_stage_floats; /* capacity of plane_stage in floats */
+  int32_t plane_stage_depth;  /* 1..64 steps per block */
+  int32_t pad_stage;
 } mw_program;
 
 /* Advance `count` steps starting at step index `first` (t = n*dt).  Replaces the

@@ -45,16 +45,42 @@ from witwin_maxwell_amd.fdtd.plan import COMPONENTS, compile_plan  # noqa: E402
 from conftest import family_err  # noqa: E402
 
 
-def run(name, scene, kw, steps, tiling):
+def _monitor_errs(ref, dev, need_field=False):
+    """Family errors of every plane/point monitor payload against the oracle's
+    (need_field: every monitor must have seen field, so the comparison is not of
+    zeros)."""
+    def data(e, c):
+        return e["components"][c]["data"] if isinstance(e["components"][c], dict) else e["components"][c]
+    errs = {}
+    mr, md = ref.monitor_results(), dev.monitor_results()
+    for name, entry in mr.items():
+        for fam in ("E", "H"):
+            comps = [c for c in entry["fields"] if c[0] == fam]
+            if comps:
+                if need_field:
+                    peak = max(float(np.max(np.abs(data(entry, c)))) for c in comps)
+                    assert peak > 1e-20, (name, fam, "monitor saw no field")
+                got = family_err({c: np.asarray(data(md[name], c)) for c in comps}, {c: data(entry, c) for c in comps})
+                errs.update({f"{name}/{c}": v for c, v in got.items()})
+    return errs
+
+
+def run(name, scene, kw, steps, tiling, splits=(), stage_depth=32, need_field=False, fused=True):
+    """Run `steps` steps in calls ending at `splits` (then the rest) with z planes
+    staged `stage_depth` steps at a time, and compare fields, psi and monitors."""
     scene.device = "cpu"
     kw = dict(kw)
     kw.pop("theta", None)
     kw["time_steps"] = steps
     plan = compile_plan(scene, **kw)
     ref = OracleRun(plan).run(steps)
-    dev = runtime.DeviceRun(plan, "cpu", fused=True).set_tiling(*tiling)
+    dev = runtime.DeviceRun(plan, "cpu", fused=fused, plane_stage_depth=stage_depth).set_tiling(*tiling)
+    for end in splits:
+        dev.run(end - dev.step_index)
     dev.run()
-    errs = {}
+    errs = _monitor_errs(ref, dev, need_field)
+    if plan.observers is not None and any(a == "z" for _, a, _ in plan.observers.planes):
+        assert dev.plane_stage is not None, "z planes not staged"
     for fam in (COMPONENTS[:3], COMPONENTS[3:]):
         scale = max(ref.peak[c] for c in fam)
         errs.update(family_err({c: dev.fields[c].double().numpy() for c in fam}, {c: ref.fields[c] for c in fam},
@@ -70,11 +96,41 @@ def run(name, scene, kw, steps, tiling):
 
 dx = configs.DX
 # a grid with an interior box, PML shell boxes, soft-source terms and per-sample
-# curl coefficients, under both z tilings
+# curl coefficients, under both z tilings; z-normal planes in the interior (the
+# flux monitors), in the z PML slab and at the last interior column are staged
+# (blocks of 3 steps, calls ending inside and at the end of a block), an x-normal
+# plane is sampled after each step
 s, kw = configs.dielectric_boxes(cells=12, pml=5, n_boxes=4, box_range=(3, 6), seed=5)
 s.domain = mw.Domain(((-6 * dx, 6 * dx), (-7 * dx, 7 * dx), (-37 * dx, 37 * dx)))
-run("boxes, aligned z tiling", s, kw, 8, (3, 2, 2))
-run("boxes, z-edge tiles", s, kw, 8, (0, 4, 1))
+s.add_monitor(mw.PlaneMonitor("z_pml", axis="z", position=-34.5 * dx, fields=("Ez", "Hx"),
+                              frequencies=kw["frequencies"]))
+s.add_monitor(mw.PlaneMonitor("z_edge", axis="z", position=31 * dx, fields=("Hz",),
+                              frequencies=kw["frequencies"]))
+s.add_monitor(mw.PlaneMonitor("x_mid", axis="x", position=0.0, fields=("Ey",), frequencies=kw["frequencies"]))
+run("boxes, aligned z tiling", s, kw, 8, (3, 2, 2), splits=(2, 6), stage_depth=3)
+run("boxes, z-edge tiles", s, kw, 8, (0, 4, 1), splits=(3,), stage_depth=4)
+
+# staged z planes under field from the first step: CW dipoles (full amplitude at
+# t = 0) beside planes in the z PML slab, the interior and the high z end (shell
+# or z-edge tile); blocks of 4 steps, calls ending inside blocks and at their ends,
+# so a fold that adds a step twice or drops one shows at once
+st = mw.Scene(domain=mw.Domain(((-6 * dx, 6 * dx), (-7 * dx, 7 * dx), (-37 * dx, 37 * dx))),
+              grid=mw.GridSpec.uniform(dx), boundary=mw.BoundarySpec.pml(5), device="cpu")
+for zc in (-34.3 * dx, 0.2 * dx, 30.8 * dx):
+    st.add_source(mw.PointDipole((0.3 * dx, -0.2 * dx, zc), (0.4, 0.6, 1.0), profile="ideal",
+                                 source_time=mw.CW(configs.F0)))
+freqs = (configs.F0, 1.3 * configs.F0)
+st.add_monitor(mw.PlaneMonitor("z_pml", axis="z", position=-34.5 * dx, fields=("Ez", "Hx"), frequencies=freqs))
+st.add_monitor(mw.PlaneMonitor("z_mid", axis="z", position=0.0, fields=("Ey", "Hy"), frequencies=freqs))
+st.add_monitor(mw.PlaneMonitor("z_end", axis="z", position=31 * dx, fields=("Ex", "Hz"), frequencies=freqs))
+st.add_monitor(mw.PlaneMonitor("x_mid", axis="x", position=0.0, fields=("Ez",), frequencies=freqs))
+# a second monitor on one of those planes: both read the same staged samples
+st.add_monitor(mw.PlaneMonitor("z_mid_again", axis="z", position=0.0, fields=("Ey",), frequencies=freqs[:1]))
+skw = {"frequencies": freqs, "courant": 0.9, "window": "none"}
+run("staged planes, aligned z tiling", st, skw, 11, (3, 2, 2), splits=(2, 6, 9), stage_depth=4, need_field=True)
+run("staged planes, z-edge tiles", st, skw, 11, (0, 4, 1), splits=(3, 5), stage_depth=4, need_field=True)
+run("staged planes, two-kernel step", st, skw, 11, (0, 0, 0), splits=(4,), stage_depth=3, need_field=True,
+    fused=False)
 
 # every pole kind (Debye, Drude, two Lorentz) with the per-sample activity mask,
 # in the interior box, the z-edge tiles and the shell, under both z tilings

@@ -21,8 +21,8 @@
 // walks a contiguous run of neighbouring tiles, so rows re-read by adjacent tiles
 // hit in that XCD's L2 instead of going back to HBM.
 //
-// Periodic faces need H from the far side of the grid, which a single pass does
-// not have; scenes with periodic axes use the two-kernel path (yee.h).
+// Periodic faces need H from the far side of the grid: they are shell cells, and
+// the shell's E pass runs after every H of the step is stored (shell.h).
 #pragma once
 
 #include <hip/hip_runtime.h>

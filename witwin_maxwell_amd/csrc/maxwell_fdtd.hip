@@ -844,7 +844,7 @@ extern "C" {
 
 const char* mw_last_error(void) { return g_error.c_str(); }
 
-int32_t mw_abi_version(void) { return 3; }
+int32_t mw_abi_version(void) { return 4; }
 
 int32_t mw_abi_struct_sizes(int64_t* out, int32_t capacity) {
   const int64_t sizes[] = {(int64_t)sizeof(mw_grid),     (int64_t)sizeof(mw_coeffs),   (int64_t)sizeof(mw_cpml),
@@ -988,9 +988,15 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
       (rc = check_terms(g, p->e_terms, p->n_e_terms, "e_terms", p->total_steps)))
     return rc;
   if (p->n_planes < 0 || p->n_planes > MW_MAX_PLANES) return fail(MW_ERR_ARGUMENT, "%d planes (max %d)", p->n_planes, MW_MAX_PLANES);
+  // planes with axis 2 are staged when the program gives a stage; the others are
+  // added to their sums after each step
+  const bool staging = p->plane_stage && p->n_planes > 0;
+  if (staging && (p->plane_stage_depth < 1 || p->plane_stage_depth > mw::kMaxStageDepth))
+    return fail(MW_ERR_ARGUMENT, "plane_stage_depth %d outside [1, %d]", p->plane_stage_depth, mw::kMaxStageDepth);
   mw::PlaneSet planes{};
-  planes.count = p->n_planes;
-  int64_t plane_cells = 0;
+  mw::ZGatherSet zstage{};
+  mw::ZFlushSet zflush{};
+  int64_t plane_cells = 0, zstage_cells = 0, row_floats = 0;
   for (int q = 0; q < p->n_planes; ++q) {
     const mw_plane_obs& o = p->planes[q];
     if (o.component < 0 || o.component > 5 || o.axis < 0 || o.axis > 2 || !o.re || !o.im)
@@ -999,8 +1005,31 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
     component_shape(g, o.component, shp);
     if (o.index < 0 || o.index >= shp[o.axis]) return fail(MW_ERR_ARGUMENT, "plane %d index %d outside axis", q, o.index);
     const int64_t n = o.axis == 0 ? (int64_t)shp[1] * shp[2] : (o.axis == 1 ? (int64_t)shp[0] * shp[2] : (int64_t)shp[0] * shp[1]);
-    plane_cells = std::max(plane_cells, n);
-    planes.p[q] = o;
+    if (staging && o.axis == 2) {
+      // one stage plane per (component, column); planes that repeat one share it
+      int z = 0;
+      while (z < zstage.count && !(zstage.comp[z] == o.component && zstage.k[z] == o.index)) ++z;
+      if (z == zstage.count) {
+        zstage.comp[z] = o.component;
+        zstage.k[z] = o.index;
+        zstage.off[z] = row_floats;
+        ++zstage.count;
+        row_floats += n;
+      }
+      zflush.p[zflush.count++] = mw::ZFlushPlane{o.component, 0, zstage.off[z], n, o.re, o.im};
+      zstage_cells = std::max(zstage_cells, n);
+    } else {
+      planes.p[planes.count++] = o;
+      plane_cells = std::max(plane_cells, n);
+    }
+  }
+  if (zstage.count > 0) {
+    if ((int64_t)p->plane_stage_depth * row_floats > p->plane_stage_floats)
+      return fail(MW_ERR_ARGUMENT, "plane_stage holds %lld floats; %d rows of the %d z planes need %lld",
+                  (long long)p->plane_stage_floats, p->plane_stage_depth, zstage.count,
+                  (long long)((int64_t)p->plane_stage_depth * row_floats));
+    zflush.row_floats = row_floats;
+    zflush.stage = p->plane_stage;
   }
   if ((p->n_planes > 0 || p->n_points > 0) && (p->nf_obs <= 0 || !p->obs_e_cos || !p->obs_e_sin || !p->obs_h_cos || !p->obs_h_sin))
     return fail(MW_ERR_ARGUMENT, "observer weight tables missing");
@@ -1063,12 +1092,35 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
       }
       if ((rc = check_launch("accumulate_dft"))) return rc;
     }
-    if (p->n_planes > 0) {
+    if (planes.count > 0) {
       const int64_t off = (int64_t)n * p->nf_obs;
-      hipLaunchKernelGGL(mw::accumulate_planes_kernel, dim3(grid_blocks(plane_cells), p->n_planes), dim3(256), 0, s,
+      hipLaunchKernelGGL(mw::accumulate_planes_kernel, dim3(grid_blocks(plane_cells), planes.count), dim3(256), 0, s,
                          planes, fs, p->nf_obs, p->obs_e_cos + off, p->obs_e_sin + off, p->obs_h_cos + off,
                          p->obs_h_sin + off);
       if ((rc = check_launch("accumulate_planes"))) return rc;
+    }
+    if (zstage.count > 0) {
+      const int row = n % p->plane_stage_depth;
+      zstage.row = p->plane_stage + (int64_t)row * row_floats;
+      hipLaunchKernelGGL(mw::gather_zplanes_kernel, dim3(grid_blocks(zstage_cells * zstage.count)), dim3(256), 0, s,
+                         zstage, fs, zstage_cells);
+      if ((rc = check_launch("gather_zplanes"))) return rc;
+      // fold the block's rows when it completes or the call ends inside it (then
+      // clear them: rows of steps already folded must add nothing at the block's end)
+      const bool block_end = row == p->plane_stage_depth - 1, call_end = n == first + count - 1;
+      if (block_end || call_end) {
+        const int64_t off = (int64_t)(n - row) * p->nf_obs;
+        hipLaunchKernelGGL(mw::flush_zstage_kernel, dim3(grid_blocks(zstage_cells), zflush.count), dim3(256), 0, s,
+                           zflush, row + 1, p->nf_obs, p->obs_e_cos + off, p->obs_e_sin + off, p->obs_h_cos + off,
+                           p->obs_h_sin + off);
+        if ((rc = check_launch("flush_zstage"))) return rc;
+        if (!block_end) {
+          const int64_t folded = (int64_t)(row + 1) * row_floats;
+          hipLaunchKernelGGL(mw::clear_floats_kernel, dim3(grid_blocks(folded)), dim3(256), 0, s, p->plane_stage,
+                             folded);
+          if ((rc = check_launch("clear_zstage"))) return rc;
+        }
+      }
     }
     if (p->n_points > 0) {
       const int64_t off = (int64_t)n * p->nf_obs;

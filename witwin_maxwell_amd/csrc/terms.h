@@ -233,6 +233,98 @@ __global__ __launch_bounds__(256) void accumulate_planes_kernel(PlaneSet set, Fi
   dft_column(x, nf, n, l, c < 3 ? e_cos : h_cos, c < 3 ? e_sin : h_sin, q.re, q.im);
 }
 
+// z-normal plane monitors, staged.  Sampling a z-normal plane reads one float per
+// (x, y) column of a z-fastest field, a cache line each, and adding it to nf
+// complex running sums per step reads and writes 16 nf bytes per sample.  Staged,
+// each step only copies the plane's samples into a stage row (gather below), and
+// once per block of steps one pass adds the block's rows to the sums (flush), so
+// the sums move once per block instead of once per step.  Plane q's samples of
+// stage row t sit at stage + t * row_floats + off, in the layout of its sums
+// (nf, cells): sample (i, j) at i * sy + j.
+constexpr int kMaxStageDepth = 64;
+
+// The staged (component, column) pairs, each listed once; row = this step's row.
+struct ZGatherSet {
+  int32_t count;
+  int32_t pad;
+  float* row;
+  int32_t comp[MW_MAX_PLANES];
+  int32_t k[MW_MAX_PLANES];
+  int64_t off[MW_MAX_PLANES];
+};
+
+// row[off + i * sy + j] = component(i, j, k) for each staged pair.  Thread t
+// takes pair t % count of sample t / count: the pairs of one sample sit in one
+// wavefront, so pairs that share a cache line (one component at neighbouring
+// columns, as a flux plane's H samples) fetch it once, when it is read, instead of
+// once per pair in passes far apart.
+__global__ __launch_bounds__(256) void gather_zplanes_kernel(ZGatherSet set, FieldSet fs, int64_t samples) {
+  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= samples * set.count) return;
+  const int q = (int)(t % set.count);
+  const int64_t l = t / set.count;
+  const int c = set.comp[q];
+  if (l >= (int64_t)fs.shape[c][0] * fs.shape[c][1]) return;
+  set.row[set.off[q] + l] = fs.ptr[c][l * fs.shape[c][2] + set.k[q]];
+}
+
+struct ZFlushPlane {
+  int32_t component;
+  int32_t pad;
+  int64_t off;
+  int64_t cells;
+  float* re;
+  float* im;
+};
+struct ZFlushSet {
+  int32_t count;
+  int32_t pad;
+  int64_t row_floats;
+  const float* stage;
+  ZFlushPlane p[MW_MAX_PLANES];
+};
+
+// Fold stage rows [0, rows) -- the steps first .. first+rows-1, weights wc/ws
+// already offset to step `first` -- into the running DFT sums: for each sample the
+// rows are added in step order with one fused multiply-add per step, the same
+// chain of roundings as adding each step as it is taken.  Planes that observe the
+// same (component, column) read the same stage samples.  blockIdx.y = plane.
+__global__ __launch_bounds__(256) void flush_zstage_kernel(ZFlushSet set, int rows, int nf,
+                                                           const float* __restrict__ e_cos,
+                                                           const float* __restrict__ e_sin,
+                                                           const float* __restrict__ h_cos,
+                                                           const float* __restrict__ h_sin) {
+  const int qi = blockIdx.y;
+  if (qi >= set.count) return;
+  const ZFlushPlane& q = set.p[qi];
+  const int64_t l = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (l >= q.cells) return;
+  const float* __restrict__ wc = q.component < 3 ? e_cos : h_cos;
+  const float* __restrict__ ws = q.component < 3 ? e_sin : h_sin;
+  const float* __restrict__ col = set.stage + q.off + l;
+  float v[kMaxStageDepth];
+#pragma unroll
+  for (int t = 0; t < kMaxStageDepth; ++t) v[t] = t < rows ? col[t * set.row_floats] : 0.0f;
+  for (int f = 0; f < nf; ++f) {
+    float re = q.re[f * q.cells + l], im = q.im[f * q.cells + l];
+#pragma unroll
+    for (int t = 0; t < kMaxStageDepth; ++t) {
+      if (t < rows) {
+        re = __builtin_fmaf(v[t], wc[t * nf + f], re);
+        im = __builtin_fmaf(v[t], ws[t * nf + f], im);
+      }
+    }
+    q.re[f * q.cells + l] = re;
+    q.im[f * q.cells + l] = im;
+  }
+}
+
+// p[0, n) = 0 (folded stage rows of a block that a later call continues).
+__global__ __launch_bounds__(256) void clear_floats_kernel(float* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0.0f;
+}
+
 __global__ __launch_bounds__(64) void accumulate_points_kernel(FieldSet fs, int count, const int32_t* comp,
                                                                const int32_t* lin, int nf, const float* e_cos,
                                                                const float* e_sin, const float* h_cos,

@@ -90,7 +90,9 @@ class MwProgram(ctypes.Structure):
                 ("point_linear", vp), ("point_re", vp), ("point_im", vp), ("nf_dft", ctypes.c_int32),
                 ("shell_chunk", ctypes.c_int32), ("dft_cos", vp), ("dft_sin", vp), ("dft_re", vp * 3), ("dft_im", vp * 3),
                 ("fused", ctypes.c_int32), ("xchunk", ctypes.c_int32), ("fields_alt", MwFields), ("psi_h_alt", vp * 6),
-                ("h_terms_dev", vp), ("e_terms_dev", vp), ("aux", MwAuxLine)]
+                ("h_terms_dev", vp), ("e_terms_dev", vp), ("aux", MwAuxLine),
+                ("plane_stage", vp), ("plane_stage_floats", ctypes.c_int64), ("plane_stage_depth", ctypes.c_int32),
+                ("pad_stage", ctypes.c_int32)]
 
 
 class MwWindow(ctypes.Structure):

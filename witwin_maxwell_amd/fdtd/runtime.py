@@ -42,9 +42,13 @@ class DeviceRun:
 
     ``fused=None`` (or True) runs the single-pass step (E, H and the H-side psi
     double-buffered, one set of launches per step); ``fused=False`` the two-kernel
-    in-place path.  Both handle every face mode, periodic included."""
+    in-place path.  Both handle every face mode, periodic included.
 
-    def __init__(self, plan: FDTDPlan, device="cuda", fused=None):
+    ``plane_stage_depth`` (0 = off): the samples of z-normal plane monitors are
+    copied into a stage of that many steps and added to their sums once per block
+    of steps, instead of the sums being read and written after every step."""
+
+    def __init__(self, plan: FDTDPlan, device="cuda", fused=None, plane_stage_depth=32):
         self.plan = plan
         self.device = torch.device(device)
         if self.device.type != "cuda":
@@ -238,6 +242,21 @@ class DeviceRun:
         self._planes = (nat.MwPlaneObs * max(1, len(planes)))(*planes)
         prog.planes = ctypes.cast(self._planes, ctypes.POINTER(nat.MwPlaneObs))
         prog.n_planes = len(planes)
+        # z-normal planes: a row per step of each distinct (component, index)
+        self.plane_stage = None
+        z_pairs = {}
+        for q, po in enumerate(planes):
+            if po.axis == 2:
+                z_pairs.setdefault((po.component, po.index), int(self.plane_re[q][0].numel()))
+        z_cells = sum(z_pairs.values())
+        depth = int(plane_stage_depth)
+        if z_cells > 0 and depth > 0:
+            if depth > 64:
+                raise ValueError("plane_stage_depth must be at most 64")
+            self.plane_stage = torch.zeros((depth, z_cells), dtype=torch.float32, device=dev)
+            prog.plane_stage = nat.ptr(self.plane_stage)
+            prog.plane_stage_floats = self.plane_stage.numel()
+            prog.plane_stage_depth = depth
 
         # ---- full-field DFT
         self.dft_re, self.dft_im = {}, {}
@@ -404,6 +423,8 @@ class DeviceRun:
                     s["polarization"].zero_()
         for t in self.plane_re + self.plane_im + list(self.dft_re.values()) + list(self.dft_im.values()):
             t.zero_()
+        if self.plane_stage is not None:
+            self.plane_stage.zero_()
         if self.aux_e is not None:
             self.aux_e.zero_()
             self.aux_h.zero_()
