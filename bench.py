@@ -15,8 +15,9 @@ OpenMP transcription of the same update equations (oracle/fdtd_oracle.c) on a
 bounded sample of the same scene on the host cores.
 
 Other workloads: ``--scene vacuum|dispersive`` (forward), ``--workload gradient``
-(one checkpointed design-region gradient), ``--workload batch`` (32 scenes
-batched vs one scene per launch).  ``--summarize stats DIR`` and ``--summarize
+(one checkpointed design-region gradient), ``--workload batch`` (a design
+gradient for each of 32 scenes: batched forward and stream-overlapped reverse
+sweeps against one scene at a time).  ``--summarize stats DIR`` and ``--summarize
 traffic FETCH_DIR WRITE_DIR`` turn rocprofv3 CSV output into the per-kernel
 tables and HBM bytes per dispatch kept under profiles/.
 """
@@ -324,85 +325,115 @@ def bench_gradient(args):
 
 
 def bench_batch(args):
-    """Batched forward over the 32-scene batch configuration (seeded uneven sizes
-    64^3 .. 192x192x128 cells, CPML 10, 32x32x16 design region, dipole,
-    single-frequency flux plane, ``--batch-steps`` steps): all scenes advanced by
-    the batched launches (autotuned x-chunk and graph length) against the same
-    scenes run one after another.  The batched adjoint is switched off, so this
-    workload times the forward half of the gradient pipeline."""
+    """Batched forward+adjoint over the 32-scene batch configuration (seeded
+    uneven sizes 64^3 .. 192x192x128 cells, CPML 10, 32x32x16 design region,
+    dipole, single-frequency flux plane, ``--batch-steps`` steps, checkpoints
+    every sqrt(T) steps): one design gradient per scene.
+
+    Plain: one scene at a time (``GradientSession``: checkpointed forward, then
+    the reverse sweep).  Batched (``BatchGradientSession``): the forward advances
+    all scenes in the batched launches (autotuned x-chunk, graph length and z
+    tiling), then each scene's reverse sweep runs on one of ``--batch-streams``
+    streams.  Both take the gradient of sum |o|^2 over each scene's flux-plane
+    sums; the per-scene window gradients of the two are compared.
+
+    Roofline: algorithmic bytes of every scene's gradient (forward, replay and
+    reverse steps at the forward step's bytes, and each checkpoint's snapshot
+    and restore) over the batched wall time."""
     import numpy as np
     import torch
     from witwin_maxwell_amd import configs
-    from witwin_maxwell_amd.fdtd.batch import BatchRun
+    from witwin_maxwell_amd.fdtd.adjoint import GradientSession, default_stride
+    from witwin_maxwell_amd.fdtd.batch import BatchGradientSession
     from witwin_maxwell_amd.fdtd.plan import compile_plan
-    from witwin_maxwell_amd.fdtd.runtime import DeviceRun
 
     scs, kw = configs.batched_scenes(count=args.batch_count, steps=args.batch_steps)
     plans = [compile_plan(sc, region_densities=[r.density.detach() for r in sc.material_regions], **kw)
              for sc, _ in scs]
     T = kw["time_steps"]
+    K = default_stride(T)
+    n = len(plans)
     nodes = sum(int(np.prod(p.shape)) for p in plans)
-    # one scene per launch sequence
-    lat = []
-    singles = []
-    step_bytes = 0
+
+    def seeds(outs):  # d/do of sum o^2
+        return [2.0 * o.detach() for o in outs]
+
+    def window_grads(res):
+        g_decay, g_curl = res
+        return [g.detach().cpu() for g in list(g_decay) + list(g_curl)]
+
+    # plain: one scene at a time (the first scene once untimed: library load, first launches)
+    warm = GradientSession(plans[0], "cuda", stride=K)
+    warm.backward(seeds(warm.forward()))
+    del warm
+    lat, plain, alg = [], [], 0
     for p in plans:
-        r = DeviceRun(p, "cuda")
-        step_bytes += algorithmic_bytes(p, r)[0]
-        r.run(min(8, T))  # warm-up
-        r.reset()
+        sess = GradientSession(p, "cuda", stride=K)
         torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        r.run(T)
+        t0 = time.perf_counter()
+        res = sess.backward(seeds(sess.forward()))
         torch.cuda.synchronize()
-        lat.append(time.perf_counter() - t1)
-        singles.append(r.monitor_results())
-        del r
-    t_plain = sum(lat)
+        lat.append(time.perf_counter() - t0)
+        plain.append(window_grads(res))
+        step_bytes = algorithmic_bytes(p, sess.run)[0]
+        alg += T * 3 * step_bytes + len(range(K, T, K)) * 2 * 2 * state_bytes(sess.run)
+        del sess, res
     torch.cuda.empty_cache()
-    batch = BatchRun([DeviceRun(p, "cuda") for p in plans])
-    choice = batch.autotune(trial_steps=16)
+    t_plain = sum(lat)
+    print(f"[bench] plain: {n} gradients in {t_plain:.2f}s", file=sys.stderr, flush=True)
+    # batched
+    session = BatchGradientSession(plans, "cuda", stride=K, streams=args.batch_streams)
+    choice = session.batch.autotune(trial_steps=16)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    batch.run(T)
+    outs = session.forward()
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    res = session.backward([seeds(o) for o in outs])
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
+    batched = [window_grads(r) for r in res]
+    blat = [t_fwd + d for d in session.scene_done_s]
     err = 0.0
-    for single, r in zip(singles, batch.runs):
-        mb = r.monitor_results()
-        for name, mon in single.items():
-            for comp, v in mon["components"].items():
-                a = np.asarray(v["data"] if isinstance(v, dict) else v)
-                b = mb[name]["components"][comp]
-                b = np.asarray(b["data"] if isinstance(b, dict) else b)
-                err = max(err, float(np.max(np.abs(a - b)) / max(np.max(np.abs(a)), 1e-30)))
-    batch.close()
-    n = len(plans)
-    roofline = {"bound": "hbm", "achieved": step_bytes * T / t_batch / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": step_bytes * T / t_batch / HBM_PEAK, "traffic": None,
-                "kernel": "batched step (all scenes' step kernels)", "alg_bytes_per_step": step_bytes}
+    for a, b in zip(plain, batched):
+        num = sum(float((x.double() - y.double()).norm() ** 2) for x, y in zip(a, b)) ** 0.5
+        den = sum(float(x.double().norm() ** 2) for x in a) ** 0.5
+        err = max(err, num / max(den, 1e-300))
+    del session, res, outs
+    torch.cuda.empty_cache()
+    roofline = {"bound": "hbm", "achieved": alg / t_batch / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": alg / t_batch / HBM_PEAK, "traffic": None,
+                "kernel": "every scene's gradient (batched forward, per-scene replay and reverse step kernels)",
+                "alg_bytes_per_batch": alg}
     cpu = None
     if not args.no_cpu:
         mid = plans[int(np.argsort([int(np.prod(p.shape)) for p in plans])[len(plans) // 2])]
-        base = cpu_baseline(mid, steps=args.cpu_steps)
-        rate = base["value"] * 1e9  # node-updates/s of the port on a median-size scene
-        cpu = {"value": rate / (nodes * T), "unit": "scene-runs/s", "cores": base["cores"], "kind": "port",
-               "cpu_model": base["cpu_model"],
-               "sample": f"median-size scene {mid.shape}: {base['sample']}; scene-runs/s = node-updates/s over "
-                         f"the batch's {nodes} nodes x {T} steps"}
-    line = {"metric": "batched forward scene-runs/s vs one scene per launch", "value": n / t_batch,
-            "unit": "scene-runs/s", "n_gpus": 1, "steps": T, "warmup": 0, "ms_per_step": t_batch * 1e3 / T,
-            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        print("[bench] cpu baseline (forward and reverse steps of the float32 port)", file=sys.stderr, flush=True)
+        t_f, t_r = cpu_gradient_rates(mid, steps=args.cpu_grad_steps)
+        mid_nodes = int(np.prod(mid.shape))
+        per_node_step = (2 * t_f + t_r) / mid_nodes
+        per_batch = per_node_step * nodes * T
+        cpu = {"value": n / per_batch, "unit": "gradients/s", "cores": _threads(), "kind": "port",
+               "cpu_model": host_info()["cpu_model"],
+               "sample": f"{args.cpu_grad_steps} forward steps ({t_f:.3f} s each) and {args.cpu_grad_steps} reverse "
+                         f"steps ({t_r:.3f} s each) of the median-size scene {mid.shape}, float32 OpenMP; scaled "
+                         f"per node to the batch's {nodes} nodes x {T} x (2 forward + 1 reverse) steps"}
+    line = {"metric": "batched fwd+adjoint grads/s vs plain", "value": n / t_batch, "unit": "grads/s", "n_gpus": 1,
+            "steps": T, "warmup": 1, "ms_per_step": t_batch * 1e3 / T, "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
             "config": {"workload": f"{n} scenes 64^3..192x192x128 cells (+2x10 CPML), 32x32x16 design region, "
-                                   f"dipole, single-frequency flux plane, {T} steps, forward only",
-                       "total_nodes": nodes, "xchunk_graph_steps_z_tiling": list(choice)},
+                                   f"dipole, single-frequency flux plane, {T} steps, checkpoints every {K} steps, "
+                                   f"one gradient per scene",
+                       "total_nodes": nodes, "xchunk_graph_steps_z_tiling": list(choice),
+                       "reverse_streams": args.batch_streams},
             "roofline": roofline, "cpu_baseline": cpu, "host": host_info(),
-            "detail": {"batched_s": t_batch, "plain_s": t_plain, "speedup": t_plain / t_batch,
-                       "batched_gnode_updates_per_s": nodes * T / t_batch / 1e9,
-                       "plain_gnode_updates_per_s": nodes * T / t_plain / 1e9,
+            "detail": {"batched_s": t_batch, "batched_forward_s": t_fwd, "plain_s": t_plain,
+                       "speedup_vs_plain": t_plain / t_batch, "plain_grads_per_s": n / t_plain,
+                       "batched_latency_p50_s": float(np.percentile(blat, 50)),
+                       "batched_latency_p99_s": float(np.percentile(blat, 99)),
                        "plain_latency_p50_s": float(np.percentile(lat, 50)),
                        "plain_latency_p99_s": float(np.percentile(lat, 99)),
-                       "max_rel_monitor_diff_batched_vs_plain": err}}
+                       "max_rel_gradient_diff_batched_vs_plain": err}}
     print(json.dumps(line), flush=True)
 
 
@@ -530,6 +561,7 @@ def main():
     ap.add_argument("--grad-steps", type=int, default=3000)
     ap.add_argument("--batch-count", type=int, default=32)
     ap.add_argument("--batch-steps", type=int, default=1500)
+    ap.add_argument("--batch-streams", type=int, default=4, help="streams of the batched reverse sweeps")
     ap.add_argument("--summarize", nargs="+", metavar="ARG",
                     help="stats DIR | traffic FETCH_DIR WRITE_DIR: summarise rocprofv3 CSV output and exit")
     args = ap.parse_args()

@@ -2,7 +2,7 @@
 with the float64 oracle.  Run by tests/test_emulated_kernels.py in a child process
 (AddressSanitizer must be preloaded into the interpreter that loads the library).
 
-usage: python run_case.py LIB"""
+usage: python run_case.py LIB [CASE ...]   (cases: boxes staged dispersive periodic; default all)"""
 import contextlib
 import ctypes
 import inspect
@@ -95,74 +95,90 @@ def run(name, scene, kw, steps, tiling, splits=(), stage_depth=32, need_field=Fa
 
 
 dx = configs.DX
-# a grid with an interior box, PML shell boxes, soft-source terms and per-sample
-# curl coefficients, under both z tilings; z-normal planes in the interior (the
-# flux monitors), in the z PML slab and at the last interior column are staged
-# (blocks of 3 steps, calls ending inside and at the end of a block), an x-normal
-# plane is sampled after each step
-s, kw = configs.dielectric_boxes(cells=12, pml=5, n_boxes=4, box_range=(3, 6), seed=5)
-s.domain = mw.Domain(((-6 * dx, 6 * dx), (-7 * dx, 7 * dx), (-37 * dx, 37 * dx)))
-s.add_monitor(mw.PlaneMonitor("z_pml", axis="z", position=-34.5 * dx, fields=("Ez", "Hx"),
-                              frequencies=kw["frequencies"]))
-s.add_monitor(mw.PlaneMonitor("z_edge", axis="z", position=31 * dx, fields=("Hz",),
-                              frequencies=kw["frequencies"]))
-s.add_monitor(mw.PlaneMonitor("x_mid", axis="x", position=0.0, fields=("Ey",), frequencies=kw["frequencies"]))
-run("boxes, aligned z tiling", s, kw, 8, (3, 2, 2), splits=(2, 6), stage_depth=3)
-run("boxes, z-edge tiles", s, kw, 8, (0, 4, 1), splits=(3,), stage_depth=4)
 
-# staged z planes under field from the first step: CW dipoles (full amplitude at
-# t = 0) beside planes in the z PML slab, the interior and the high z end (shell
-# or z-edge tile); blocks of 4 steps, calls ending inside blocks and at their ends,
-# so a fold that adds a step twice or drops one shows at once
-st = mw.Scene(domain=mw.Domain(((-6 * dx, 6 * dx), (-7 * dx, 7 * dx), (-37 * dx, 37 * dx))),
-              grid=mw.GridSpec.uniform(dx), boundary=mw.BoundarySpec.pml(5), device="cpu")
-for zc in (-34.3 * dx, 0.2 * dx, 30.8 * dx):
-    st.add_source(mw.PointDipole((0.3 * dx, -0.2 * dx, zc), (0.4, 0.6, 1.0), profile="ideal",
-                                 source_time=mw.CW(configs.F0)))
-freqs = (configs.F0, 1.3 * configs.F0)
-st.add_monitor(mw.PlaneMonitor("z_pml", axis="z", position=-34.5 * dx, fields=("Ez", "Hx"), frequencies=freqs))
-st.add_monitor(mw.PlaneMonitor("z_mid", axis="z", position=0.0, fields=("Ey", "Hy"), frequencies=freqs))
-st.add_monitor(mw.PlaneMonitor("z_end", axis="z", position=31 * dx, fields=("Ex", "Hz"), frequencies=freqs))
-st.add_monitor(mw.PlaneMonitor("x_mid", axis="x", position=0.0, fields=("Ez",), frequencies=freqs))
-# a second monitor on one of those planes: both read the same staged samples
-st.add_monitor(mw.PlaneMonitor("z_mid_again", axis="z", position=0.0, fields=("Ey",), frequencies=freqs[:1]))
-skw = {"frequencies": freqs, "courant": 0.9, "window": "none"}
-run("staged planes, aligned z tiling", st, skw, 11, (3, 2, 2), splits=(2, 6, 9), stage_depth=4, need_field=True)
-run("staged planes, z-edge tiles", st, skw, 11, (0, 4, 1), splits=(3, 5), stage_depth=4, need_field=True)
-run("staged planes, two-kernel step", st, skw, 11, (0, 0, 0), splits=(4,), stage_depth=3, need_field=True,
-    fused=False)
 
-# every pole kind (Debye, Drude, two Lorentz) with the per-sample activity mask,
-# in the interior box, the z-edge tiles and the shell, under both z tilings
-two_pi = 2 * np.pi
-d = mw.Scene(domain=mw.Domain(((-6 * dx, 6 * dx), (-7 * dx, 7 * dx), (-37 * dx, 37 * dx))),
-             grid=mw.GridSpec.uniform(dx), boundary=mw.BoundarySpec.pml(5), device="cpu")
-d.add_structure(mw.Structure(mw.Box((0, 0, -20 * dx), (8 * dx, 9 * dx, 20 * dx)),
-                             mw.Material(eps_r=2.0, debye_poles=(mw.DebyePole(2.0, 3e-15),))))
-d.add_structure(mw.Structure(mw.Sphere((1 * dx, -1 * dx, 12 * dx), 5 * dx),
-                             mw.Material(eps_r=2.0, drude_poles=(mw.DrudePole(1.2e15 / two_pi, 1e14 / two_pi),),
-                                         lorentz_poles=(mw.LorentzPole(1.5, 4e15 / two_pi, 2e14 / two_pi),
-                                                        mw.LorentzPole(0.8, 6e15 / two_pi, 2e14 / two_pi)))))
-d.add_source(mw.PointDipole((0.2 * dx, 0.3 * dx, -0.4 * dx), (0.5, 0.3, 1.0), profile="ideal",
-                            source_time=mw.GaussianPulse(configs.F0, configs.F0 / 4)))
-dkw = {"frequencies": (configs.F0,), "courant": 0.9}
-run("dispersive, aligned z tiling", d, dkw, 12, (3, 2, 2))
-run("dispersive, z-edge tiles", d, dkw, 12, (0, 4, 1))
-# periodic x faces (H read across the seam by the shell's E pass), PMC y faces,
-# CPML z, with an interior box and a dielectric crossing the seam
-pz = mw.Scene(domain=mw.Domain(((-8 * dx, 8 * dx), (-7 * dx, 7 * dx), (-37 * dx, 37 * dx))),
-              grid=mw.GridSpec.uniform(dx),
-              boundary=mw.BoundarySpec.faces(x="periodic", y="pmc", z="pml", num_layers=5), device="cpu")
-pz.add_structure(mw.Structure(mw.Box((7 * dx, 0, 3 * dx), (6 * dx, 5 * dx, 8 * dx)), mw.Material(eps_r=3.0)))
-pz.add_source(mw.PointDipole((6.6 * dx, 0.3 * dx, -0.4 * dx), (0.5, 0.3, 1.0), profile="ideal",
-                             source_time=mw.GaussianPulse(configs.F0, configs.F0 / 4)))
-run("periodic x, PMC y, aligned z tiling", pz, {"frequencies": (configs.F0,), "courant": 0.9}, 10, (3, 2, 2))
-run("periodic x, PMC y, z-edge tiles", pz, {"frequencies": (configs.F0,), "courant": 0.9}, 10, (0, 4, 1))
-# periodic z faces: the z ends stay shell cells even when z-edge tiles are asked for
-pk = mw.Scene(domain=mw.Domain(((-6 * dx, 6 * dx), (-7 * dx, 7 * dx), (-37 * dx, 37 * dx))),
-              grid=mw.GridSpec.uniform(dx), boundary=mw.BoundarySpec.faces(default="pml", z="periodic",
-                                                                            num_layers=5), device="cpu")
-pk.add_source(mw.PointDipole((0.2 * dx, 0.3 * dx, 36.6 * dx), (0.5, 0.3, 1.0), profile="ideal",
-                             source_time=mw.GaussianPulse(configs.F0, configs.F0 / 4)))
-run("periodic z, z-edge tiles asked", pk, {"frequencies": (configs.F0,), "courant": 0.9}, 10, (0, 4, 1))
+def case_boxes():
+    # a grid with an interior box, PML shell boxes, soft-source terms and per-sample
+    # curl coefficients, under both z tilings; z-normal planes in the interior (the
+    # flux monitors), in the z PML slab and at the last interior column are staged
+    # (blocks of 3 steps, calls ending inside and at the end of a block), an x-normal
+    # plane is sampled after each step
+    s, kw = configs.dielectric_boxes(cells=12, pml=5, n_boxes=4, box_range=(3, 6), seed=5)
+    s.domain = mw.Domain(((-6 * dx, 6 * dx), (-7 * dx, 7 * dx), (-37 * dx, 37 * dx)))
+    s.add_monitor(mw.PlaneMonitor("z_pml", axis="z", position=-34.5 * dx, fields=("Ez", "Hx"),
+                                  frequencies=kw["frequencies"]))
+    s.add_monitor(mw.PlaneMonitor("z_edge", axis="z", position=31 * dx, fields=("Hz",),
+                                  frequencies=kw["frequencies"]))
+    s.add_monitor(mw.PlaneMonitor("x_mid", axis="x", position=0.0, fields=("Ey",), frequencies=kw["frequencies"]))
+    run("boxes, aligned z tiling", s, kw, 8, (3, 2, 2), splits=(2, 6), stage_depth=3)
+    run("boxes, z-edge tiles", s, kw, 8, (0, 4, 1), splits=(3,), stage_depth=4)
+
+
+def case_staged():
+    # staged z planes under field from the first step: CW dipoles (full amplitude at
+    # t = 0) beside planes in the z PML slab, the interior and the high z end (shell
+    # or z-edge tile); blocks of 4 steps, calls ending inside blocks and at their ends,
+    # so a fold that adds a step twice or drops one shows at once
+    st = mw.Scene(domain=mw.Domain(((-6 * dx, 6 * dx), (-7 * dx, 7 * dx), (-37 * dx, 37 * dx))),
+                  grid=mw.GridSpec.uniform(dx), boundary=mw.BoundarySpec.pml(5), device="cpu")
+    for zc in (-34.3 * dx, 0.2 * dx, 30.8 * dx):
+        st.add_source(mw.PointDipole((0.3 * dx, -0.2 * dx, zc), (0.4, 0.6, 1.0), profile="ideal",
+                                     source_time=mw.CW(configs.F0)))
+    freqs = (configs.F0, 1.3 * configs.F0)
+    st.add_monitor(mw.PlaneMonitor("z_pml", axis="z", position=-34.5 * dx, fields=("Ez", "Hx"), frequencies=freqs))
+    st.add_monitor(mw.PlaneMonitor("z_mid", axis="z", position=0.0, fields=("Ey", "Hy"), frequencies=freqs))
+    st.add_monitor(mw.PlaneMonitor("z_end", axis="z", position=31 * dx, fields=("Ex", "Hz"), frequencies=freqs))
+    st.add_monitor(mw.PlaneMonitor("x_mid", axis="x", position=0.0, fields=("Ez",), frequencies=freqs))
+    # a second monitor on one of those planes: both read the same staged samples
+    st.add_monitor(mw.PlaneMonitor("z_mid_again", axis="z", position=0.0, fields=("Ey",), frequencies=freqs[:1]))
+    skw = {"frequencies": freqs, "courant": 0.9, "window": "none"}
+    run("staged planes, aligned z tiling", st, skw, 11, (3, 2, 2), splits=(2, 6, 9), stage_depth=4, need_field=True)
+    run("staged planes, z-edge tiles", st, skw, 11, (0, 4, 1), splits=(3, 5), stage_depth=4, need_field=True)
+    run("staged planes, two-kernel step", st, skw, 11, (0, 0, 0), splits=(4,), stage_depth=3, need_field=True,
+        fused=False)
+
+
+def case_dispersive():
+    # every pole kind (Debye, Drude, two Lorentz) with the per-sample activity mask,
+    # in the interior box, the z-edge tiles and the shell, under both z tilings
+    two_pi = 2 * np.pi
+    d = mw.Scene(domain=mw.Domain(((-6 * dx, 6 * dx), (-7 * dx, 7 * dx), (-37 * dx, 37 * dx))),
+                 grid=mw.GridSpec.uniform(dx), boundary=mw.BoundarySpec.pml(5), device="cpu")
+    d.add_structure(mw.Structure(mw.Box((0, 0, -20 * dx), (8 * dx, 9 * dx, 20 * dx)),
+                                 mw.Material(eps_r=2.0, debye_poles=(mw.DebyePole(2.0, 3e-15),))))
+    d.add_structure(mw.Structure(mw.Sphere((1 * dx, -1 * dx, 12 * dx), 5 * dx),
+                                 mw.Material(eps_r=2.0, drude_poles=(mw.DrudePole(1.2e15 / two_pi, 1e14 / two_pi),),
+                                             lorentz_poles=(mw.LorentzPole(1.5, 4e15 / two_pi, 2e14 / two_pi),
+                                                            mw.LorentzPole(0.8, 6e15 / two_pi, 2e14 / two_pi)))))
+    d.add_source(mw.PointDipole((0.2 * dx, 0.3 * dx, -0.4 * dx), (0.5, 0.3, 1.0), profile="ideal",
+                                source_time=mw.GaussianPulse(configs.F0, configs.F0 / 4)))
+    dkw = {"frequencies": (configs.F0,), "courant": 0.9}
+    run("dispersive, aligned z tiling", d, dkw, 12, (3, 2, 2))
+    run("dispersive, z-edge tiles", d, dkw, 12, (0, 4, 1))
+
+
+def case_periodic():
+    # periodic x faces (H read across the seam by the shell's E pass), PMC y faces,
+    # CPML z, with an interior box and a dielectric crossing the seam
+    pz = mw.Scene(domain=mw.Domain(((-8 * dx, 8 * dx), (-7 * dx, 7 * dx), (-37 * dx, 37 * dx))),
+                  grid=mw.GridSpec.uniform(dx),
+                  boundary=mw.BoundarySpec.faces(x="periodic", y="pmc", z="pml", num_layers=5), device="cpu")
+    pz.add_structure(mw.Structure(mw.Box((7 * dx, 0, 3 * dx), (6 * dx, 5 * dx, 8 * dx)), mw.Material(eps_r=3.0)))
+    pz.add_source(mw.PointDipole((6.6 * dx, 0.3 * dx, -0.4 * dx), (0.5, 0.3, 1.0), profile="ideal",
+                                 source_time=mw.GaussianPulse(configs.F0, configs.F0 / 4)))
+    run("periodic x, PMC y, aligned z tiling", pz, {"frequencies": (configs.F0,), "courant": 0.9}, 10, (3, 2, 2))
+    run("periodic x, PMC y, z-edge tiles", pz, {"frequencies": (configs.F0,), "courant": 0.9}, 10, (0, 4, 1))
+    # periodic z faces: the z ends stay shell cells even when z-edge tiles are asked for
+    pk = mw.Scene(domain=mw.Domain(((-6 * dx, 6 * dx), (-7 * dx, 7 * dx), (-37 * dx, 37 * dx))),
+                  grid=mw.GridSpec.uniform(dx), boundary=mw.BoundarySpec.faces(default="pml", z="periodic",
+                                                                                num_layers=5), device="cpu")
+    pk.add_source(mw.PointDipole((0.2 * dx, 0.3 * dx, 36.6 * dx), (0.5, 0.3, 1.0), profile="ideal",
+                                 source_time=mw.GaussianPulse(configs.F0, configs.F0 / 4)))
+    run("periodic z, z-edge tiles asked", pk, {"frequencies": (configs.F0,), "courant": 0.9}, 10, (0, 4, 1))
+
+
+CASES = {"boxes": case_boxes, "staged": case_staged, "dispersive": case_dispersive, "periodic": case_periodic}
+
+for name in (sys.argv[2:] or list(CASES)):
+    CASES[name]()
 print("OK")

@@ -42,13 +42,19 @@ def emulated_library(tmp_path_factory):
 
 
 def test_single_pass_step_matches_oracle_under_emulation(emulated_library):
+    """The case groups of run_case.py, each in its own process, side by side."""
     lib, asan = emulated_library
-    env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0")
-    proc = subprocess.run([sys.executable, os.path.join(EMU, "run_case.py"), lib], capture_output=True, text=True,
-                          env=env, timeout=900)
-    print(proc.stdout)
-    assert proc.returncode == 0, (proc.stdout[-3000:], proc.stderr[-3000:])
-    assert proc.stdout.strip().endswith("OK")
+    env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0", OMP_NUM_THREADS="1")
+    groups = ("boxes", "staged", "dispersive", "periodic")
+    procs = {g: subprocess.Popen([sys.executable, os.path.join(EMU, "run_case.py"), lib, g], stdout=subprocess.PIPE,
+                                 stderr=subprocess.PIPE, text=True, env=env) for g in groups}
+    failed = []
+    for g, proc in procs.items():
+        out, err = proc.communicate(timeout=900)
+        print(out)
+        if proc.returncode != 0 or not out.strip().endswith("OK"):
+            failed.append((g, out[-3000:], err[-3000:]))
+    assert not failed, failed
 
 
 def test_reference_named_ops_under_emulation(emulated_library):

@@ -169,13 +169,24 @@ def run_batch_noalloc(runs, steps=None, xchunk: int = 0):
 
 # ---------------------------------------------------------------- gradients
 class BatchGradientSession:
-    """Checkpointed forward + reverse sweep of many scenes at once: the forward
-    advances through ``BatchRun``; backward restores each scene's segment
-    snapshot, replays the segment for all scenes with the design-window capture
-    and walks it backwards with the batched reverse kernels."""
+    """Checkpointed forward + reverse sweep of many scenes at once.
 
-    def __init__(self, plans, device="cuda", windows=None, stride=None, xchunk=0, graph_steps=1):
+    Forward: every scene advances through ``BatchRun`` (one launch per step kernel
+    for all scenes), with a snapshot of each scene at every segment start.
+    Backward (``reverse="streams"``, the default): each scene's sweep -- restore
+    the segment's snapshot, replay it with the design-window capture
+    (``mw_capture_steps``), walk it backwards (``mw_reverse_steps``), last segment
+    first -- runs on one of ``streams`` HIP streams, so the scenes' sweeps overlap
+    on the GPU; the caller's stream waits for all of them.  ``reverse="kernels"``
+    names the batched reverse kernels (``mw_adjoint_batch_*``, all scenes per
+    launch), which are switched off: their first device run ended in a GPU
+    memory fault whose cause is not established."""
+
+    def __init__(self, plans, device="cuda", windows=None, stride=None, xchunk=0, graph_steps=1, streams=4,
+                 reverse="streams"):
         from .adjoint import default_stride, design_window
+        if reverse not in ("streams", "kernels"):
+            raise ValueError("reverse must be 'streams' or 'kernels'")
         self.plans = list(plans)
         T = {p.time_steps for p in self.plans}
         if len(T) != 1:
@@ -186,7 +197,10 @@ class BatchGradientSession:
         self.batch.graph_steps = int(graph_steps)
         self.windows = list(windows) if windows is not None else [design_window(p) for p in self.plans]
         self.stride = int(stride or default_stride(self.T))
+        self.streams = max(1, int(streams))
+        self.reverse = reverse
         self.checkpoints = []
+        self.scene_done_s = None
 
     def outputs(self, si):
         from .adjoint import run_outputs
@@ -202,59 +216,73 @@ class BatchGradientSession:
             self.batch.run(min(K, self.T - s0))
         return [self.outputs(i) for i in range(len(self.runs))]
 
+    def _adjoint(self, run, win, grads):
+        """Adjoint state of one scene seeded with its output gradients scaled to
+        unit peak (the sweep is linear in its seeds); returns (adjoint, scale)."""
+        from .adjoint import DeviceAdjoint
+        grads = list(grads)
+        point = (grads.pop(0), grads.pop(0)) if run.point_re is not None else None
+        n = len(run.plane_re)
+        planes = list(zip(grads[:n], grads[n:2 * n]))
+        peak = 0.0
+        for t in ([*point] if point else []) + [x for pair in planes for x in pair]:
+            peak = max(peak, float(t.detach().abs().max()) if t.numel() else 0.0)
+        scale = peak if peak > 0 else 1.0
+        sc = lambda t: t.detach().double() / scale
+        adj = DeviceAdjoint(run, win, capacity=self.stride)
+        adj.zero()
+        adj.set_seeds(point=(sc(point[0]), sc(point[1])) if point else None,
+                      planes=[(sc(r), sc(i)) for r, i in planes])
+        return adj, scale
+
     def backward(self, grads_per_scene):
         """grads_per_scene[i] follows outputs(i); returns per scene the window
-        (g_decay[3], g_curl[3]) as float64 tensors.
-
-        Switched off: its first device run ended in a GPU memory fault whose cause
-        is not established (the same path runs clean under a CPU AddressSanitizer
-        emulation of the kernels), so it raises until that cause is found."""
-        raise NotImplementedError(
-            "the batched adjoint is switched off pending an unexplained GPU memory fault; "
-            "differentiate scenes one at a time with fdtd.adjoint.differentiable_run")
-        from .adjoint import DeviceAdjoint, restore  # noqa: E402  (kept for when the path is re-enabled)
+        (g_decay[3], g_curl[3]) as float64 tensors.  ``scene_done_s`` then holds
+        each scene's sweep time from the start of the backward pass."""
+        if self.reverse == "kernels":
+            raise NotImplementedError(
+                "the batched reverse kernels are switched off pending an unexplained GPU memory fault; "
+                "use reverse='streams' (each scene's sweep on its own stream)")
+        from .adjoint import restore
         K, T = self.stride, self.T
-        adjs, scales = [], []
-        for run, win, grads in zip(self.runs, self.windows, grads_per_scene):
-            grads = list(grads)
-            point = (grads.pop(0), grads.pop(0)) if run.point_re is not None else None
-            n = len(run.plane_re)
-            planes = list(zip(grads[:n], grads[n:2 * n]))
-            peak = 0.0
-            for t in ([*point] if point else []) + [x for pair in planes for x in pair]:
-                peak = max(peak, float(t.detach().abs().max()) if t.numel() else 0.0)
-            scale = peak if peak > 0 else 1.0
-            sc = lambda t: t.detach().double() / scale
-            adj = DeviceAdjoint(run, win, capacity=K)
-            adj.zero()
-            adj.set_seeds(point=(sc(point[0]), sc(point[1])) if point else None,
-                          planes=[(sc(r), sc(i)) for r, i in planes])
-            adjs.append(adj)
-            scales.append(scale)
-        lib = self.batch.lib
-        count = len(self.runs)
-        progs = (ctypes.POINTER(nat.MwProgram) * count)(*[ctypes.pointer(r.program) for r in self.runs])
-        adjp = (ctypes.POINTER(nat.MwAdjoint) * count)(*[ctypes.pointer(a.adjoint) for a in adjs])
-        winp = (ctypes.POINTER(nat.MwWindow) * count)(*[ctypes.pointer(a.window) for a in adjs])
-        handle = ctypes.c_void_p()
-        with torch.cuda.device(self.runs[0].device):
-            nat.check(lib.mw_adjoint_batch_create(self.batch._handle, progs, adjp, winp, count, ctypes.byref(handle)),
-                      lib)
-            try:
-                starts = list(range(0, T, K))
+        dev = self.runs[0].device
+        made = [self._adjoint(run, win, g) for run, win, g in zip(self.runs, self.windows, grads_per_scene)]
+        adjs, scales = [a for a, _ in made], [s for _, s in made]
+        # the replays run the single-scene step: give them its default tiling (the
+        # batch's tuned tiling suits the batched launches), restored afterwards
+        tilings = [(r.program.xchunk, r.program.shell_chunk, r.program.z_tiling) for r in self.runs]
+        for r in self.runs:
+            r.set_tiling(0, 0, 0)
+        main = torch.cuda.current_stream(dev)
+        pool = [torch.cuda.Stream(device=dev) for _ in range(min(self.streams, len(self.runs)))]
+        begin = torch.cuda.Event(enable_timing=True)
+        begin.record(main)
+        done = []
+        for st in pool:
+            st.wait_stream(main)  # snapshots, seeds and zeroed adjoint state are on the caller's stream
+        starts = list(range(0, T, K))
+        for si, (run, adj) in enumerate(zip(self.runs, adjs)):
+            st = pool[si % len(pool)]
+            with torch.cuda.stream(st):
                 for ci, s0 in reversed(list(enumerate(starts))):
                     cnt = min(K, T - s0)
                     snaps = self.checkpoints[ci]
-                    for ri, run in enumerate(self.runs):
-                        restore(run, None if snaps is None else snaps[ri], s0)
-                    stream = nat.current_stream()
-                    nat.check(lib.mw_adjoint_batch_capture(handle, s0, cnt, stream), lib)
-                    nat.check(lib.mw_adjoint_batch_reverse(handle, s0, cnt, stream), lib)
-            finally:
-                lib.mw_adjoint_batch_destroy(handle)
-        for run in self.runs:
+                    restore(run, None if snaps is None else snaps[si], s0)
+                    adj.capture(s0, cnt)
+                    adj.reverse(s0, cnt)
+                ev = torch.cuda.Event(enable_timing=True)
+                ev.record(st)
+                done.append(ev)
+        for st in pool:
+            main.wait_stream(st)
+        for run, t in zip(self.runs, tilings):
             run.step_index = T
+            run.set_tiling(*t)
         self.adjoints = adjs
+        done[-1].synchronize()
+        for ev in done:
+            ev.synchronize()
+        self.scene_done_s = [begin.elapsed_time(ev) * 1e-3 for ev in done]
         return [([g.double() * sc_ for g in a.g_decay], [g.double() * sc_ for g in a.g_curl])
                 for a, sc_ in zip(adjs, scales)]
 
@@ -285,7 +313,8 @@ class _BatchGradientFunction(torch.autograd.Function):
         return (None, None, *[g.to(device=d, dtype=t) for g, (d, t) in zip(flat, ctx.coef_meta)])
 
 
-def batched_differentiable_run(scenes, plan_kwargs, device="cuda", stride=None, xchunk=0, graph_steps=1):
+def batched_differentiable_run(scenes, plan_kwargs, device="cuda", stride=None, xchunk=0, graph_steps=1, streams=4,
+                               reverse="streams"):
     """Differentiable run of many scenes at once.  ``scenes`` is a list of
     ``Scene`` whose material regions carry the densities to differentiate;
     returns (plans, monitors per scene, session)."""
@@ -304,7 +333,8 @@ def batched_differentiable_run(scenes, plan_kwargs, device="cuda", stride=None, 
         c = window_coefficients(plan, eps_node, lo, hi)
         coefs += [d for d, _ in c] + [cu for _, cu in c]
         plans.append(plan)
-    session = BatchGradientSession(plans, device, stride=stride, xchunk=xchunk, graph_steps=graph_steps)
+    session = BatchGradientSession(plans, device, stride=stride, xchunk=xchunk, graph_steps=graph_steps,
+                                   streams=streams, reverse=reverse)
     outs = list(_BatchGradientFunction.apply(session, len(scenes), *coefs))
     monitors, pos = [], 0
     for i, (plan, run) in enumerate(zip(plans, session.runs)):
