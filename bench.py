@@ -512,24 +512,40 @@ def summarize(argv):
         for k, v in sorted(kernel_stats(argv[1]).items(), key=lambda kv: -kv[1]["total_ms"]):
             print(f"{v['calls']:8d} calls  avg {v['avg_us']:10.2f} us  total {v['total_ms']:10.2f} ms  {k}")
     elif argv[0] == "traffic":
-        print(json.dumps(traffic_summary(argv[1], argv[2]), indent=1))
+        summary = traffic_summary(argv[1], argv[2])
+        if len(argv) > 3:
+            # the committed form that the forward workload reads back as roofline.traffic:
+            # the step kernels' bytes per step, keyed by the tiling they were measured at
+            calib = summary.pop("_calibration")
+            step = sum(v["total_bytes"] for k, v in summary.items() if any(
+                f in k for f in ("lean_step_kernel", "shell_h_kernel", "shell_e_kernel", "zedge_step_kernel",
+                                 "shifted_values_kernel")))
+            summary = {"by_tiling": {argv[3]: {STEP_KERNEL: step, "_per_kernel_bytes": summary,
+                                               "_calibration": calib}}}
+        print(json.dumps(summary, indent=1))
     else:
-        raise SystemExit("--summarize stats DIR | traffic FETCH_DIR WRITE_DIR")
+        raise SystemExit("--summarize stats DIR | traffic FETCH_DIR WRITE_DIR [XCHUNK,SHELL_CHUNK,Z_TILING]")
+
+
+STEP_KERNEL = "single-pass step (lean_step_kernel + shell_h_kernel + shell_e_kernel)"
+
+
+def _traffic_table(path):
+    """{tiling tuple: entry} of a committed counter summary (profiles/traffic_*.json)."""
+    if not path or not os.path.exists(path):
+        return {}
+    try:
+        data = json.load(open(path))
+    except (OSError, ValueError):
+        return {}
+    return {tuple(int(v) for v in k.split(",")): e for k, e in data.get("by_tiling", {}).items()}
 
 
 def _traffic_for(tiling, path, kernel):
     """Measured HBM bytes of the step from a committed counter summary, only when
     it was measured at the tiling this run uses."""
-    if not path or not os.path.exists(path):
-        return None
-    try:
-        data = json.load(open(path))
-    except (OSError, ValueError):
-        return None
-    measured = data.get("_tiling")
-    if tiling is None or measured is None or list(measured) != list(tiling):
-        return None
-    return data.get(kernel)
+    entry = _traffic_table(path).get(tuple(tiling)) if tiling is not None else None
+    return None if entry is None else entry.get(kernel)
 
 
 # --------------------------------------------------------------------- main
@@ -619,7 +635,9 @@ def main():
         runner.set_tiling(*tiling)
     elif runner.fused and not args.no_tune:
         # launch geometry of the single-pass step, timed on this run and reset
-        tiling = runner.autotune(trial_steps=12, cache=False)
+        # ties (1%) go to a tiling whose HBM traffic is on record
+        prefer = sorted(_traffic_table(os.path.join(ROOT, "profiles", traffic_file))) if traffic_file else []
+        tiling = runner.autotune(trial_steps=12, cache=False, prefer=prefer)
         print(f"[bench] tiling (xchunk, shell_chunk, z_tiling) = {tiling}", file=sys.stderr, flush=True)
     print(f"[bench] plan {plan.shape} compiled in {compile_s:.1f}s; warmup {args.warmup} steps", file=sys.stderr,
           flush=True)
@@ -635,8 +653,7 @@ def main():
     if runner.fused:
         # the single-pass step: PML-shell H, lean interior, PML-shell E (+ the tiny
         # per-step term prep); rocprof splits the time between them
-        dominant, t_dom, b_dom = ("single-pass step (lean_step_kernel + shell_h_kernel + shell_e_kernel)",
-                                  time_fused(runner, reps), step_bytes)
+        dominant, t_dom, b_dom = (STEP_KERNEL, time_fused(runner, reps), step_bytes)
     else:
         t_e = time_kernel(runner, "E", reps)
         t_h = time_kernel(runner, "H", reps)

@@ -278,14 +278,14 @@ __host__ __device__ inline int zedge_tiles(const FusedArgs& a) {
 }
 
 template <bool ADE, bool DFT>
-__global__ __launch_bounds__(256, 6) void zedge_step_kernel(FusedArgs a) {
+__global__ __launch_bounds__(256) void zedge_step_kernel(FusedArgs a) {
   const int logical = xcd_logical_block();
   if (logical >= zedge_tiles(a)) return;
   zedge_tile<ADE, DFT>(a, a.step, a.dft_cos, a.dft_sin, logical);
 }
 
 template <bool ADE, bool DFT>
-__global__ __launch_bounds__(256, 6) void zedge_batch_kernel(const FusedArgs* __restrict__ args,
+__global__ __launch_bounds__(256) void zedge_batch_kernel(const FusedArgs* __restrict__ args,
                                                           const int* __restrict__ prefix, int nscenes, int total,
                                                           const int* __restrict__ counter) {
   const int logical = xcd_logical_block();

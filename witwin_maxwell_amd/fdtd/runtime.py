@@ -355,10 +355,12 @@ class DeviceRun:
         return self
 
     def autotune(self, trial_steps: int = 12, xchunks=(0, 3, 4, 6, 8), shell_chunks=(2, 4), z_tilings=(1, 2),
-                 cache=True):
+                 cache=True, prefer=(), margin=0.01):
         """Pick the single-pass tiling by timing ``trial_steps`` steps of each
-        candidate on this run, then reset the run to step 0.  The choice is cached
-        per (shape, CPML, dispersive, DFT) signature in the JSON file named by
+        candidate on this run, then reset the run to step 0.  Among candidates
+        within ``margin`` of the fastest, the first of ``prefer`` (tilings the
+        caller has other data for) is taken.  The choice is cached per (shape,
+        CPML, dispersive, DFT) signature in the JSON file named by
         WITWIN_MAXWELL_AMD_TUNE_CACHE, when set.  Returns (xchunk, shell_chunk,
         z_tiling)."""
         import json
@@ -376,22 +378,26 @@ class DeviceRun:
             self.set_tiling(*table[sig])
             return tuple(table[sig])
         steps = min(int(trial_steps), max(1, p.time_steps - 1))
-        best, best_t = (0, 0, 0), float("inf")
-        for zt in z_tilings:
-            for xc in xchunks:
-                for sc in shell_chunks:
-                    self.set_tiling(xc, sc, zt)
-                    self.reset()
-                    self.run(1)  # warm-up
-                    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-                    torch.cuda.synchronize()
-                    start.record()
-                    self.run(min(steps, p.time_steps - self.step_index))
-                    end.record()
-                    end.synchronize()
-                    t = start.elapsed_time(end)
-                    if t < best_t:
-                        best, best_t = (xc, sc, zt), t
+        timed = {}
+        cands = [(xc, sc, zt) for zt in z_tilings for xc in xchunks for sc in shell_chunks]
+        cands += [tuple(int(v) for v in c) for c in prefer if tuple(int(v) for v in c) not in cands]
+        for cand in cands:
+            self.set_tiling(*cand)
+            self.reset()
+            self.run(1)  # warm-up
+            start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            torch.cuda.synchronize()
+            start.record()
+            self.run(min(steps, p.time_steps - self.step_index))
+            end.record()
+            end.synchronize()
+            timed[cand] = start.elapsed_time(end)
+        best = min(timed, key=timed.get)
+        for c in prefer:
+            c = tuple(int(v) for v in c)
+            if timed.get(c, float("inf")) <= timed[best] * (1.0 + margin):
+                best = c
+                break
         self.set_tiling(*best)
         self.reset()
         if cache and path:
