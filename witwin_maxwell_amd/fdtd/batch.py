@@ -125,6 +125,9 @@ class BatchRun:
                 if per_step < best_t:
                     best, best_t = (chunk, g, zt), per_step
         self.reset()
+        if best is None:
+            raise ValueError(f"autotune needs at least {max(trial_steps, 2 * min(graphs))} steps per trial; the "
+                             f"batch has {self.total_steps}")
         self._apply(*best)
         if cache and path:
             table[sig] = list(best)
