@@ -571,9 +571,10 @@ def main():
     ap.add_argument("--workload", choices=("forward", "gradient", "batch"), default="forward",
                     help="forward: Yee step of the dielectric-box scene (default); gradient: one design-region "
                          "gradient; batch: batched vs one-scene-per-launch forward runs")
-    ap.add_argument("--scene", choices=("boxes", "vacuum", "dispersive"), default="boxes",
+    ap.add_argument("--scene", choices=("boxes", "vacuum", "dispersive", "design"), default="boxes",
                     help="forward workload scene: dielectric boxes 281^3 nodes (default), vacuum dipole 85^3 with "
-                         "full-field DFT, dispersive spheres 409x409x281 with TFSF")
+                         "full-field DFT, dispersive spheres 409x409x281 with TFSF, design-region scene "
+                         "225x225x145 (the gradient workload's forward step)")
     ap.add_argument("--grad-steps", type=int, default=3000)
     ap.add_argument("--batch-count", type=int, default=32)
     ap.add_argument("--batch-steps", type=int, default=1500)
@@ -605,6 +606,13 @@ def main():
                     f"soft +z plane wave, 2 flux monitors x 8 freq")
         cells = args.cells ** 3
         traffic_file = "traffic_dielectric_boxes_256.json" if args.cells == 256 else None
+    elif args.scene == "design":
+        scene, kw = configs.design_region(steps=max(3000, total))
+        kw.pop("theta")
+        workload = ("design region 200x200x120 cells (+2x12 CPML), 64x64x32 design voxels, Ez dipole, objective "
+                    "plane at f0, forward step")
+        cells = 200 * 200 * 120
+        traffic_file = None
     elif args.scene == "vacuum":
         scene, kw = configs.vacuum_dipole(cells=64, pml=10, steps=max(500, total))
         workload = ("vacuum 64^3 cells (+2x10 CPML), centred Ez ideal dipole, full-field DFT at 3 frequencies "
