@@ -1820,7 +1820,9 @@ int mw_adjoint_batch_destroy(mw_adjoint_batch* b) {
 
 int mw_batch_destroy(mw_batch* b) {
   if (!b) return MW_OK;
-  (void)hipStreamSynchronize(b->stream);
+  // the batch's tables are read by launches on its own stream and, through
+  // mw_adjoint_batch_capture's replay, on a caller's stream: wait for the device
+  (void)hipDeviceSynchronize();
   batch_free(b);
   return MW_OK;
 }
