@@ -302,3 +302,42 @@ def test_tfsf_plane_wave(mode, fused):
     assert plan.aux is not None and (plan.line_e_terms and plan.line_h_terms)
     _check_fields(ref, dev)
     _check_monitors(ref, dev)
+
+
+@pytest.mark.parametrize("tiling", [None, (3, 1, 1)], ids=["aligned", "zedge"])
+def test_staged_planes_equal_per_step_sums(tiling):
+    """z-normal plane monitors staged and folded a block at a time (in calls that
+    end inside blocks and at their ends) give the per-step sums to the bit: the
+    fold adds a block's rows in step order with one fused multiply-add each, the
+    rounding chain of adding each step as it is taken.  Planes in the interior,
+    the z CPML and at a z end of the lean box; both against the oracle too."""
+    from oracle.driver import OracleRun
+    from witwin_maxwell_amd.fdtd.runtime import DeviceRun
+    print(f"case: staged vs per-step z planes, tiling {tiling}", flush=True)
+    dx = configs.DX
+    scene, kw = configs.dielectric_boxes(cells=20, pml=6, n_boxes=5, box_range=(3, 8), seed=3)
+    scene.domain = mw.Domain(((-10 * dx, 10 * dx), (-11 * dx, 11 * dx), (-40 * dx, 40 * dx)))
+    scene.monitors.clear()
+    for zc in (-38.3 * dx, 0.2 * dx, 33.7 * dx):
+        scene.add_source(mw.PointDipole((0.3 * dx, -0.2 * dx, zc), (0.4, 0.6, 1.0), profile="ideal",
+                                        source_time=mw.CW(configs.F0)))
+    for name, z in (("z_pml", -38.5 * dx), ("z_mid", 0.0), ("z_end", 34 * dx)):
+        scene.add_monitor(mw.PlaneMonitor(name, axis="z", position=z, fields=("Ex", "Hy"), frequencies=kw["frequencies"]))
+    kw = dict(kw, time_steps=150, window="none")
+    kw.pop("theta", None)
+    plan = compile_plan(scene, **kw)
+    runs = {}
+    for depth, splits in ((0, ()), (16, (7, 40, 64, 99))):
+        dev = DeviceRun(plan, "cuda", plane_stage_depth=depth)
+        if tiling is not None:
+            dev.set_tiling(*tiling)
+        assert (dev.plane_stage is not None) == (depth > 0)
+        for end in splits:
+            dev.run(end - dev.step_index)
+        dev.run()
+        torch.cuda.synchronize()
+        runs[depth] = dev
+    for a, b in zip(runs[0].plane_re + runs[0].plane_im, runs[16].plane_re + runs[16].plane_im):
+        assert torch.equal(a, b)
+    ref = OracleRun(plan).run()
+    _check_monitors(ref, runs[16])
