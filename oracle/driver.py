@@ -13,6 +13,14 @@ OpenMP build used as the CPU baseline), in the reference's per-step order
     ADE subtraction                  stepping.py:2316 (materials.py:783-798)
     full-field DFT                   spectral.py:627-652
     observers                        observers.py:577-659
+    TFSF auxiliary line (numpy)      sources.cu:803-839
+
+Pins (beyond those of fdtd_oracle.c): the auxiliary line by
+tests/test_tfsf_oracle.py (the 3-D run of a transversely uniform field equals
+the line in the total-field region and vanishes in the scattered-field region);
+monitor payloads and fluxes by the pins of oracle/monitors.py; the sparse pole
+update (state touched only where a pole acts) by
+tests/test_oracle_pinning.py::test_sparse_pole_update_equals_dense.
 
 Only tests/, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg use this.
 """

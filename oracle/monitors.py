@@ -19,6 +19,10 @@ Reference behaviour restated:
                                                    fdtd/observers.py:25-37, :126-200
   * flux = sum_cells 1/2 Re((E x H*) . n) * exact primal cell widths
                                                    fdtd/quadrature.py:78-110, :112-174
+Pinned by tests/test_monitor_pinning.py (the flux of a closed-form tangential
+field equals its analytic integral; point and interpolated plane payloads equal
+direct evaluations) and tests/test_gpu_power.py (a soft plane wave's flux is its
+analytic incident power within 2%).
 Only tests/ import this module.
 """
 
