@@ -347,7 +347,10 @@ def bench_batch(args):
     from witwin_maxwell_amd.fdtd.batch import BatchGradientSession
     from witwin_maxwell_amd.fdtd.plan import compile_plan
 
-    scs, kw = configs.batched_scenes(count=args.batch_count, steps=args.batch_steps)
+    lo_cells, hi_cells = (tuple(int(v) for v in c.split("x")) for c in args.batch_cells.split(","))
+    scs, kw = configs.batched_scenes(count=args.batch_count, steps=args.batch_steps, smallest=lo_cells,
+                                     largest=hi_cells, design=tuple(min(d, s // 2) for d, s in zip((32, 32, 16),
+                                                                                                      lo_cells)))
     plans = [compile_plan(sc, region_densities=[r.density.detach() for r in sc.material_regions], **kw)
              for sc, _ in scs]
     T = kw["time_steps"]
@@ -421,9 +424,9 @@ def bench_batch(args):
     line = {"metric": "batched fwd+adjoint grads/s vs plain", "value": n / t_batch, "unit": "grads/s", "n_gpus": 1,
             "steps": T, "warmup": 1, "ms_per_step": t_batch * 1e3 / T, "higher_is_better": True, "scaling": "weak",
             "vs_baseline": None, "dtype": "f32", "data": "synthetic",
-            "config": {"workload": f"{n} scenes 64^3..192x192x128 cells (+2x10 CPML), 32x32x16 design region, "
-                                   f"dipole, single-frequency flux plane, {T} steps, checkpoints every {K} steps, "
-                                   f"one gradient per scene",
+            "config": {"workload": f"{n} scenes {args.batch_cells.replace(',', '..')} cells (+2x10 CPML), design "
+                                   f"region, dipole, single-frequency flux plane, {T} steps, checkpoints every {K} "
+                                   f"steps, one gradient per scene",
                        "total_nodes": nodes, "xchunk_graph_steps_z_tiling": list(choice),
                        "reverse_streams": args.batch_streams},
             "roofline": roofline, "cpu_baseline": cpu, "host": host_info(),
@@ -577,8 +580,10 @@ def main():
                          "225x225x145 (the gradient workload's forward step)")
     ap.add_argument("--grad-steps", type=int, default=3000)
     ap.add_argument("--batch-count", type=int, default=32)
+    ap.add_argument("--batch-cells", default="64x64x64,192x192x128",
+                    help="SMALLEST,LARGEST scene cells of the batch workload (default: the 32-scene configuration)")
     ap.add_argument("--batch-steps", type=int, default=1500)
-    ap.add_argument("--batch-streams", type=int, default=4, help="streams of the batched reverse sweeps")
+    ap.add_argument("--batch-streams", type=int, default=8, help="streams of the batched reverse sweeps")
     ap.add_argument("--summarize", nargs="+", metavar="ARG",
                     help="stats DIR | traffic FETCH_DIR WRITE_DIR: summarise rocprofv3 CSV output and exit")
     args = ap.parse_args()

@@ -179,13 +179,14 @@ class BatchGradientSession:
     Backward (``reverse="streams"``, the default): each scene's sweep -- restore
     the segment's snapshot, replay it with the design-window capture
     (``mw_capture_steps``), walk it backwards (``mw_reverse_steps``), last segment
-    first -- runs on one of ``streams`` HIP streams, so the scenes' sweeps overlap
-    on the GPU; the caller's stream waits for all of them.  ``reverse="kernels"``
+    first -- runs on one of ``streams`` HIP streams, each stream's sweeps issued by
+    a host thread of its own, so the scenes' sweeps overlap on the GPU and their
+    launches are issued side by side; the caller's stream waits for all of them.  ``reverse="kernels"``
     names the batched reverse kernels (``mw_adjoint_batch_*``, all scenes per
     launch), which are switched off: their first device run ended in a GPU
     memory fault whose cause is not established."""
 
-    def __init__(self, plans, device="cuda", windows=None, stride=None, xchunk=0, graph_steps=1, streams=4,
+    def __init__(self, plans, device="cuda", windows=None, stride=None, xchunk=0, graph_steps=1, streams=8,
                  reverse="streams"):
         from .adjoint import default_stride, design_window
         if reverse not in ("streams", "kernels"):
@@ -260,29 +261,42 @@ class BatchGradientSession:
         pool = [torch.cuda.Stream(device=dev) for _ in range(min(self.streams, len(self.runs)))]
         begin = torch.cuda.Event(enable_timing=True)
         begin.record(main)
-        done = []
         for st in pool:
             st.wait_stream(main)  # snapshots, seeds and zeroed adjoint state are on the caller's stream
         starts = list(range(0, T, K))
-        for si, (run, adj) in enumerate(zip(self.runs, adjs)):
-            st = pool[si % len(pool)]
-            with torch.cuda.stream(st):
-                for ci, s0 in reversed(list(enumerate(starts))):
-                    cnt = min(K, T - s0)
-                    snaps = self.checkpoints[ci]
-                    restore(run, None if snaps is None else snaps[si], s0)
-                    adj.capture(s0, cnt)
-                    adj.reverse(s0, cnt)
-                ev = torch.cuda.Event(enable_timing=True)
-                ev.record(st)
-                done.append(ev)
+        done = [None] * len(self.runs)
+
+        def sweep_stream(w):
+            # scenes w, w + len(pool), ... on stream w, issued by this host thread: the
+            # sweeps of small scenes are launch-bound, so one thread per stream issues
+            # them side by side (the library calls release the interpreter lock)
+            st = pool[w]
+            with torch.cuda.device(dev), torch.cuda.stream(st):
+                for si in range(w, len(self.runs), len(pool)):
+                    run, adj = self.runs[si], adjs[si]
+                    for ci, s0 in reversed(list(enumerate(starts))):
+                        cnt = min(K, T - s0)
+                        snaps = self.checkpoints[ci]
+                        restore(run, None if snaps is None else snaps[si], s0)
+                        adj.capture(s0, cnt)
+                        adj.reverse(s0, cnt)
+                    ev = torch.cuda.Event(enable_timing=True)
+                    ev.record(st)
+                    done[si] = ev
+
+        if len(pool) == 1:
+            sweep_stream(0)
+        else:
+            from concurrent.futures import ThreadPoolExecutor
+            with ThreadPoolExecutor(max_workers=len(pool)) as ex:
+                for f in [ex.submit(sweep_stream, w) for w in range(len(pool))]:
+                    f.result()
         for st in pool:
             main.wait_stream(st)
         for run, t in zip(self.runs, tilings):
             run.step_index = T
             run.set_tiling(*t)
         self.adjoints = adjs
-        done[-1].synchronize()
         for ev in done:
             ev.synchronize()
         self.scene_done_s = [begin.elapsed_time(ev) * 1e-3 for ev in done]
@@ -316,7 +330,7 @@ class _BatchGradientFunction(torch.autograd.Function):
         return (None, None, *[g.to(device=d, dtype=t) for g, (d, t) in zip(flat, ctx.coef_meta)])
 
 
-def batched_differentiable_run(scenes, plan_kwargs, device="cuda", stride=None, xchunk=0, graph_steps=1, streams=4,
+def batched_differentiable_run(scenes, plan_kwargs, device="cuda", stride=None, xchunk=0, graph_steps=1, streams=8,
                                reverse="streams"):
     """Differentiable run of many scenes at once.  ``scenes`` is a list of
     ``Scene`` whose material regions carry the densities to differentiate;
