@@ -191,3 +191,17 @@ def test_pole_weight_codes_rebuild_every_drive():
     bad = [dict(plan.ade["Ex"]["poles"][0])]
     bad[0]["drive"] = np.asarray(bad[0]["drive"]) * 0.3
     assert pole_weight_codes(bad) is None
+
+
+def test_plan_dt_is_the_compiled_plans_step():
+    """Simulation resolves dt (and from it the step count) before compiling the
+    plan once: plan_dt gives compile_plan's step, with a Courant number and with
+    the automatic step of a dispersive scene."""
+    from witwin_maxwell_amd import configs
+    from witwin_maxwell_amd.fdtd.plan import compile_plan, plan_dt
+    cases = [configs.vacuum_dipole(cells=12, pml=4, device="cpu"),
+             configs.dispersive_spheres(cells=(24, 24, 20), pml=4, steps=10, radius=(2, 5), device="cpu")]
+    for scene, kw in cases:
+        kw = dict(kw, time_steps=5)
+        kw.pop("theta", None)
+        assert plan_dt(scene, **kw) == compile_plan(scene, **kw).dt

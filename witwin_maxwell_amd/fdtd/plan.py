@@ -85,6 +85,26 @@ class FDTDPlan:
         return any(self.ade[c]["poles"] for c in ("Ex", "Ey", "Ez"))
 
 
+def _step_dt(scene, prepared, sts, default_f, courant):
+    """The time step: Courant number times the 3-D stability limit of the smallest
+    cells, or the reference's automatic step for the highest frequency in play."""
+    min_d = [float(prepared.primal(a).min()) for a in "xyz"]
+    if courant is not None:
+        return float(courant) / (C_0 * math.sqrt(sum(1.0 / d ** 2 for d in min_d)))
+    f_char = max([st["characteristic_frequency"] for st in sts] or [default_f])
+    f_char = max(f_char, _mat.material_characteristic_frequency(scene))
+    return float(auto_dt(*min_d, f_char))
+
+
+def plan_dt(scene, *, frequencies=(), courant=None, default_frequency=None, **_unused) -> float:
+    """The time step ``compile_plan`` would choose for these arguments, without
+    compiling the plan (takes and ignores compile_plan's other keywords)."""
+    prepared = prepare_scene(scene)
+    default_f = float(default_frequency or (frequencies[0] if frequencies else 1e9))
+    sts = [_exc.primary_source_time(s, default_f) for s in scene.sources]
+    return _step_dt(scene, prepared, sts, default_f, courant)
+
+
 def compile_plan(scene, *, frequencies=(), time_steps=None, dt=None, courant=None, full_field_dft=False,
                  window="hanning", cpml_config=None, default_frequency=None, region_densities=None,
                  node_eps_override=None) -> FDTDPlan:
@@ -102,14 +122,7 @@ def compile_plan(scene, *, frequencies=(), time_steps=None, dt=None, courant=Non
     pulsed = any(st["kind"] != "cw" for st in sts[:1])
 
     min_d = [float(prepared.primal(a).min()) for a in "xyz"]
-    if dt is None:
-        if courant is not None:
-            dt = float(courant) / (C_0 * math.sqrt(sum(1.0 / d ** 2 for d in min_d)))
-        else:
-            f_char = max([st["characteristic_frequency"] for st in sts] or [default_f])
-            f_char = max(f_char, _mat.material_characteristic_frequency(scene))
-            dt = auto_dt(*min_d, f_char)
-    dt = float(dt)
+    dt = float(dt) if dt is not None else _step_dt(scene, prepared, sts, default_f, courant)
     if time_steps is None:
         raise ValueError("time_steps must be given")
     steps = int(time_steps)

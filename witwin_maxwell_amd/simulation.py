@@ -394,7 +394,7 @@ class Simulation:
                     window=self.config.spectral_sampler.window, cpml_config=self.config.cpml_config or None,
                     default_frequency=self.frequencies[0])
 
-    def _resolve_time_steps(self, plan):
+    def _resolve_time_steps(self, dt):
         rt = self.config.run_time
         if rt.time_steps is not None:
             return int(rt.time_steps)
@@ -402,32 +402,31 @@ class Simulation:
         domain_size = max(float(b[i][1] - b[i][0]) for i in range(3))
         from .fdtd.excitation import primary_source_time
         st = primary_source_time(self.scene.sources[0], self.frequencies[0]) if self.scene.sources else None
-        return calculate_required_steps(min(self._requested_frequencies()), plan.dt, C_0,
+        return calculate_required_steps(min(self._requested_frequencies()), dt, C_0,
                                         rt.steady_cycles, rt.transient_cycles, domain_size, st)
 
     # ------------------------------------------------------------ run
     def _compile(self):
-        """The plan for the whole run and its step count (dt is fixed by the scene
-        and frequencies; a 1-step compile resolves it)."""
-        from .fdtd.plan import compile_plan
+        """The run's time step, step count and plan keywords (dt is fixed by the
+        scene and frequencies, so the plan is compiled once, with its steps)."""
+        from .fdtd.plan import plan_dt
         kw = self._plan_kwargs()
-        probe = compile_plan(self.scene, time_steps=1, **kw)
-        steps = self._resolve_time_steps(probe)
-        return probe, steps, kw
+        dt = plan_dt(self.scene, **kw)
+        return dt, self._resolve_time_steps(dt), kw
 
     def run(self) -> Result:
         from .fdtd.plan import compile_plan
         self._refresh_scene()
         self._validate()
         device = getattr(self.scene, "device", "cuda")
-        probe, steps, kw = self._compile()
+        dt, steps, kw = self._compile()
         if self.has_trainable_parameters:
             from .fdtd.adjoint import differentiable_run
-            plan, monitors, session = differentiable_run(self.scene, device=device, dt=probe.dt, time_steps=steps,
+            plan, monitors, session = differentiable_run(self.scene, device=device, dt=dt, time_steps=steps,
                                                          stride=self.config.adjoint_checkpoint_stride, **kw)
             return self._result(plan, session.run, monitors, steps)
         from .fdtd.runtime import DeviceRun
-        plan = compile_plan(self.scene, time_steps=steps, dt=probe.dt, **kw)
+        plan = compile_plan(self.scene, time_steps=steps, dt=dt, **kw)
         run = DeviceRun(plan, device)
         run.run()
         return self._result(plan, run, run.monitor_results(), steps)
@@ -441,10 +440,10 @@ class Simulation:
         self._refresh_scene()
         self._validate()
         device = getattr(self.scene, "device", "cuda")
-        probe, steps, kw = self._compile()
+        dt, steps, kw = self._compile()
         if self.has_trainable_parameters:
             return PreparedSimulation(self, None, steps)
-        plan = compile_plan(self.scene, time_steps=steps, dt=probe.dt, **kw)
+        plan = compile_plan(self.scene, time_steps=steps, dt=dt, **kw)
         return PreparedSimulation(self, DeviceRun(plan, device), steps)
 
     def _result(self, plan, run, monitors, steps) -> Result:
