@@ -387,6 +387,7 @@ def bench_batch(args):
     # batched
     session = BatchGradientSession(plans, "cuda", stride=K, streams=args.batch_streams)
     choice = session.batch.autotune(trial_steps=16)
+    tune_table = _fastest(getattr(session.batch, "autotune_timings", None))
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     outs = session.forward()
@@ -428,6 +429,7 @@ def bench_batch(args):
                                    f"region, dipole, single-frequency flux plane, {T} steps, checkpoints every {K} "
                                    f"steps, one gradient per scene",
                        "total_nodes": nodes, "xchunk_graph_steps_z_tiling": list(choice),
+                       "autotune_ms_per_step": tune_table,
                        "reverse_streams": args.batch_streams},
             "roofline": roofline, "cpu_baseline": cpu, "host": host_info(),
             "detail": {"batched_s": t_batch, "batched_forward_s": t_fwd, "plain_s": t_plain,
@@ -531,6 +533,13 @@ def summarize(argv):
 
 
 STEP_KERNEL = "single-pass step (lean_step_kernel + shell_h_kernel + shell_e_kernel)"
+
+
+def _fastest(timings, n=6):
+    """The n fastest autotuner candidates as {"a,b,c": ms per step} (None if none)."""
+    if not timings:
+        return None
+    return {",".join(str(v) for v in c): round(t, 4) for c, t in sorted(timings.items(), key=lambda kv: kv[1])[:n]}
 
 
 def _traffic_table(path):
@@ -702,7 +711,8 @@ def main():
             "config": {"workload": workload,
                        "nodes": list(plan.shape), "courant": 0.99,
                        "step": "single-pass" if runner.fused else "two-kernel",
-                       "tiling": None if tiling is None else dict(zip(("xchunk", "shell_chunk", "z_tiling"), tiling))},
+                       "tiling": None if tiling is None else dict(zip(("xchunk", "shell_chunk", "z_tiling"), tiling)),
+                       "autotune_ms_per_step": _fastest(getattr(runner, "autotune_timings", None))},
             "physical_gcell_updates_per_s": cells * args.steps / elapsed / 1e9,
             "roofline": roofline, "protocol": protocol, "cpu_baseline": cpu, "host": host_info()}
     print(json.dumps(line), flush=True)

@@ -106,6 +106,7 @@ class BatchRun:
             self._apply(*table[sig])
             return tuple(table[sig])
         best, best_t = None, float("inf")
+        self.autotune_timings = {}  # (xchunk, graph_steps, z_tiling) -> ms per step
         for chunk, zt in [(c, z) for z in z_tilings for c in chunks]:
             self._apply(chunk, 1, zt)
             for g in graphs:
@@ -122,6 +123,7 @@ class BatchRun:
                 t1.record()
                 torch.cuda.synchronize()
                 per_step = t0.elapsed_time(t1) / steps
+                self.autotune_timings[(chunk, g, zt)] = per_step
                 if per_step < best_t:
                     best, best_t = (chunk, g, zt), per_step
         self.reset()

@@ -392,6 +392,8 @@ class DeviceRun:
             end.record()
             end.synchronize()
             timed[cand] = start.elapsed_time(end)
+        # ms per step of every candidate, for the caller's records
+        self.autotune_timings = {c: t / steps for c, t in timed.items()}
         best = min(timed, key=timed.get)
         for c in prefer:
             c = tuple(int(v) for v in c)
