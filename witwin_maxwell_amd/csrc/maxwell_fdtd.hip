@@ -234,6 +234,13 @@ void dispatch_lean(dim3 grid, dim3 block, hipStream_t s, const mw::FusedArgs& a,
   else hipLaunchKernelGGL((mw::lean_step_kernel<ADE, DFT, 0>), grid, block, 0, s, a);
 }
 
+template <bool ADE, bool DFT>
+void dispatch_zedge(dim3 grid, dim3 block, hipStream_t s, const mw::FusedArgs& a, int mode) {
+  if (mode == 2) hipLaunchKernelGGL((mw::zedge_step_kernel<ADE, DFT, 2>), grid, block, 0, s, a);
+  else if (mode == 1) hipLaunchKernelGGL((mw::zedge_step_kernel<ADE, DFT, 1>), grid, block, 0, s, a);
+  else hipLaunchKernelGGL((mw::zedge_step_kernel<ADE, DFT, 0>), grid, block, 0, s, a);
+}
+
 // Tile origin and lean-box extent along one axis.  Tile t covers `width`
 // samples starting at off + width*t; the lean box holds the tiles whose samples,
 // plus `lead` samples before them (the halo), all lie in [lo, hi).
@@ -438,9 +445,18 @@ void build_shell_args(const mw::FusedArgs& a, int nlean, int chunk, mw::ShellArg
   }
 }
 
+// Any periodic face: the shell E pass then reads H across the seams (the
+// kernel variant without them leaves those loads out).
+bool grid_periodic(const mw_grid& g) {
+  for (int q = 0; q < 3; ++q)
+    if (g.bc_lo[q] == MW_BC_PERIODIC || g.bc_hi[q] == MW_BC_PERIODIC) return true;
+  return false;
+}
+
 template <bool CPML, bool ADE, bool DFT>
 int launch_shell_e(dim3 grid, hipStream_t s, const mw::ShellArgs& sa) {
-  hipLaunchKernelGGL((mw::shell_e_kernel<CPML, ADE, DFT>), grid, dim3(64, 4, 1), 0, s, sa);
+  if (grid_periodic(sa.f.g)) hipLaunchKernelGGL((mw::shell_e_kernel<CPML, ADE, DFT, true>), grid, dim3(64, 4, 1), 0, s, sa);
+  else hipLaunchKernelGGL((mw::shell_e_kernel<CPML, ADE, DFT, false>), grid, dim3(64, 4, 1), 0, s, sa);
   return check_launch("fused_step (shell E)");
 }
 
@@ -473,8 +489,8 @@ int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_field
   const int nze = mw::zedge_tiles(a);
   if (nze > 0) {
     const dim3 zgrid(((nze + 7) / 8) * 8);
-    if (ade) { if (dft) hipLaunchKernelGGL((mw::zedge_step_kernel<true, true>), zgrid, block, 0, s, a); else hipLaunchKernelGGL((mw::zedge_step_kernel<true, false>), zgrid, block, 0, s, a); }
-    else { if (dft) hipLaunchKernelGGL((mw::zedge_step_kernel<false, true>), zgrid, block, 0, s, a); else hipLaunchKernelGGL((mw::zedge_step_kernel<false, false>), zgrid, block, 0, s, a); }
+    if (ade) { if (dft) dispatch_zedge<true, true>(zgrid, block, s, a, mode); else dispatch_zedge<true, false>(zgrid, block, s, a, mode); }
+    else { if (dft) dispatch_zedge<false, true>(zgrid, block, s, a, mode); else dispatch_zedge<false, false>(zgrid, block, s, a, mode); }
     if ((rc = check_launch("fused_step (z edges)"))) return rc;
   }
   // shell E reads the complete H^{n+1/2}
@@ -536,6 +552,15 @@ int check_window(const mw_program* p, const mw_window* w) {
 }
 
 template <bool ADE, bool DFT>
+void launch_batch_zedge(dim3 grid, hipStream_t s, const mw::FusedArgs* args, const int* prefix, int count, int total,
+                        const int* counter, int mode) {
+  const dim3 block(mw::kFLanes, mw::kFRows, 1);
+  if (mode == 2) hipLaunchKernelGGL((mw::zedge_batch_kernel<ADE, DFT, 2>), grid, block, 0, s, args, prefix, count, total, counter);
+  else if (mode == 1) hipLaunchKernelGGL((mw::zedge_batch_kernel<ADE, DFT, 1>), grid, block, 0, s, args, prefix, count, total, counter);
+  else hipLaunchKernelGGL((mw::zedge_batch_kernel<ADE, DFT, 0>), grid, block, 0, s, args, prefix, count, total, counter);
+}
+
+template <bool ADE, bool DFT>
 void launch_batch_lean(dim3 grid, hipStream_t s, const mw::FusedArgs* args, const int* prefix, int count, int total,
                        const int* counter, int mode) {
   const dim3 block(mw::kFLanes, mw::kFRows, 1);
@@ -546,9 +571,13 @@ void launch_batch_lean(dim3 grid, hipStream_t s, const mw::FusedArgs* args, cons
 
 template <bool CPML, bool ADE, bool DFT>
 void launch_batch_shell_e(dim3 grid, hipStream_t s, const mw::ShellArgs* args, const int* prefix, int count, int total,
-                          const int* counter) {
-  hipLaunchKernelGGL((mw::shell_e_batch_kernel<CPML, ADE, DFT>), grid, dim3(64, 4, 1), 0, s, args, prefix, count, total,
-                     counter);
+                          const int* counter, bool periodic) {
+  if (periodic)
+    hipLaunchKernelGGL((mw::shell_e_batch_kernel<CPML, ADE, DFT, true>), grid, dim3(64, 4, 1), 0, s, args, prefix, count,
+                       total, counter);
+  else
+    hipLaunchKernelGGL((mw::shell_e_batch_kernel<CPML, ADE, DFT, false>), grid, dim3(64, 4, 1), 0, s, args, prefix,
+                       count, total, counter);
 }
 
 }  // namespace
@@ -557,6 +586,7 @@ struct mw_batch {
   int count = 0;
   int total_steps = 0;
   bool cpml = false, ade = false, dft = false;
+  bool periodic = false;  // some scene has a periodic face
   int mode = 0;
   mw::FusedArgs* d_args = nullptr;   // 2 per scene (step parity): lean interior
   mw::ShellArgs* d_shell = nullptr;  // 2 per scene: PML shell
@@ -644,14 +674,14 @@ int batch_step(const mw_batch* b, hipStream_t s, bool observe = true) {
     if ((rc = check_launch("batch fused_step (lean)"))) return rc;
   }
   if (b->total_zedge > 0) {
-    const dim3 zgrid(((b->total_zedge + 7) / 8) * 8), zb(mw::kFLanes, mw::kFRows, 1);
+    const dim3 zgrid(((b->total_zedge + 7) / 8) * 8);
     const int* zp = b->d_zedge_prefix;
     if (b->ade) {
-      if (b->dft) hipLaunchKernelGGL((mw::zedge_batch_kernel<true, true>), zgrid, zb, 0, s, b->d_args, zp, b->count, b->total_zedge, b->d_counter);
-      else hipLaunchKernelGGL((mw::zedge_batch_kernel<true, false>), zgrid, zb, 0, s, b->d_args, zp, b->count, b->total_zedge, b->d_counter);
+      if (b->dft) launch_batch_zedge<true, true>(zgrid, s, b->d_args, zp, b->count, b->total_zedge, b->d_counter, b->mode);
+      else launch_batch_zedge<true, false>(zgrid, s, b->d_args, zp, b->count, b->total_zedge, b->d_counter, b->mode);
     } else {
-      if (b->dft) hipLaunchKernelGGL((mw::zedge_batch_kernel<false, true>), zgrid, zb, 0, s, b->d_args, zp, b->count, b->total_zedge, b->d_counter);
-      else hipLaunchKernelGGL((mw::zedge_batch_kernel<false, false>), zgrid, zb, 0, s, b->d_args, zp, b->count, b->total_zedge, b->d_counter);
+      if (b->dft) launch_batch_zedge<false, true>(zgrid, s, b->d_args, zp, b->count, b->total_zedge, b->d_counter, b->mode);
+      else launch_batch_zedge<false, false>(zgrid, s, b->d_args, zp, b->count, b->total_zedge, b->d_counter, b->mode);
     }
     if ((rc = check_launch("batch fused_step (z edges)"))) return rc;
   }
@@ -661,11 +691,11 @@ int batch_step(const mw_batch* b, hipStream_t s, bool observe = true) {
     const int n = b->count, t = b->total_shell;
     const int* ctr = b->d_counter;
     if (b->cpml) {
-      if (b->ade) { if (b->dft) launch_batch_shell_e<true, true, true>(sgrid, s, sa, sp, n, t, ctr); else launch_batch_shell_e<true, true, false>(sgrid, s, sa, sp, n, t, ctr); }
-      else { if (b->dft) launch_batch_shell_e<true, false, true>(sgrid, s, sa, sp, n, t, ctr); else launch_batch_shell_e<true, false, false>(sgrid, s, sa, sp, n, t, ctr); }
+      if (b->ade) { if (b->dft) launch_batch_shell_e<true, true, true>(sgrid, s, sa, sp, n, t, ctr, b->periodic); else launch_batch_shell_e<true, true, false>(sgrid, s, sa, sp, n, t, ctr, b->periodic); }
+      else { if (b->dft) launch_batch_shell_e<true, false, true>(sgrid, s, sa, sp, n, t, ctr, b->periodic); else launch_batch_shell_e<true, false, false>(sgrid, s, sa, sp, n, t, ctr, b->periodic); }
     } else {
-      if (b->ade) { if (b->dft) launch_batch_shell_e<false, true, true>(sgrid, s, sa, sp, n, t, ctr); else launch_batch_shell_e<false, true, false>(sgrid, s, sa, sp, n, t, ctr); }
-      else { if (b->dft) launch_batch_shell_e<false, false, true>(sgrid, s, sa, sp, n, t, ctr); else launch_batch_shell_e<false, false, false>(sgrid, s, sa, sp, n, t, ctr); }
+      if (b->ade) { if (b->dft) launch_batch_shell_e<false, true, true>(sgrid, s, sa, sp, n, t, ctr, b->periodic); else launch_batch_shell_e<false, true, false>(sgrid, s, sa, sp, n, t, ctr, b->periodic); }
+      else { if (b->dft) launch_batch_shell_e<false, false, true>(sgrid, s, sa, sp, n, t, ctr, b->periodic); else launch_batch_shell_e<false, false, false>(sgrid, s, sa, sp, n, t, ctr, b->periodic); }
     }
     if ((rc = check_launch("batch fused_step (shell E)"))) return rc;
   }
@@ -740,10 +770,12 @@ int collect_batch(const mw_program* const* programs, int count, int xchunk, Sink
     const int mode = coef_mode(p->coeffs);
     if (si == 0) {
       b->cpml = cpml; b->ade = ade; b->dft = dft; b->mode = mode; b->total_steps = p->total_steps;
+      b->periodic = false;
     } else if (cpml != b->cpml || ade != b->ade || dft != b->dft) {
       return fail(MW_ERR_ARGUMENT, "batch: scene %d differs in CPML/ADE/full-field-DFT use", si);
     }
     if (mode != b->mode) b->mode = 0;
+    if (grid_periodic(*g)) b->periodic = true;
     b->total_steps = std::min(b->total_steps, p->total_steps);
     for (int par = 0; par < 2; ++par) {
       mw::FusedArgs a;

@@ -277,7 +277,7 @@ __device__ __forceinline__ void shell_dft(const FusedArgs& a, gptr<float> __rest
   }
 }
 
-template <bool CPML, bool ADE, bool DFT>
+template <bool CPML, bool ADE, bool DFT, bool PER>
 __device__ __forceinline__ void shell_e_cells(
     const ShellArgs& sa, const ShellCell& c, int step, gptr<const float> __restrict__ wc,
     gptr<const float> __restrict__ ws, gptr<const float> __restrict__ hx, gptr<const float> __restrict__ hy,
@@ -300,8 +300,11 @@ __device__ __forceinline__ void shell_e_cells(
             hzm = a.g.bc_hi[2];
   const bool jin = j > 0 && j < ny - 1;
   const bool kin = k > 0 && k < nz - 1;
-  // a periodic axis has both faces periodic (the host checks it)
-  const bool per_x = lx == MW_BC_PERIODIC, per_y = ly == MW_BC_PERIODIC, per_z = lz == MW_BC_PERIODIC;
+  // a periodic axis has both faces periodic (the host checks it); PER is false
+  // when no face of the grid is periodic, which keeps the six seam loads below
+  // (and their registers) out of the kernel
+  const bool per_x = PER && lx == MW_BC_PERIODIC, per_y = PER && ly == MW_BC_PERIODIC,
+             per_z = PER && lz == MW_BC_PERIODIC;
   const float iey = G(a.g.inv_e[1])[j];
   const float iez = G(a.g.inv_e[2])[k];
   float bey = 0, cey = 0, key = 1, bez = 0, cez = 0, kez = 1;
@@ -381,12 +384,15 @@ __device__ __forceinline__ void shell_e_cells(
     const bool wi = per_x && (i == 0 || i == nx - 1), wj = per_y && (j == 0 || j == ny - 1),
                wk = per_z && (k == 0 || k == nz - 1);
     const int iw = i == 0 ? nx - 2 : 0, jw = j == 0 ? ny - 2 : 0, kw = k == 0 ? nz - 2 : 0;
-    const float wz_y = hz[(wj && iH) ? i * sHz + (uint32_t)jw * nz + k : 0];             // Ex: Hz across y
-    const float wy_z = hy[(wk && iH) ? i * sHy + (uint32_t)j * (nz - 1) + kw : 0];       // Ex: Hy across z
-    const float wz_x = hz[(wi && jH) ? (uint32_t)iw * sHz + (uint32_t)j * nz + k : 0];    // Ey: Hz across x
-    const float wx_z = hx[(wk && jH) ? i * sHx + (uint32_t)j * (nz - 1) + kw : 0];       // Ey: Hx across z
-    const float wy_x = hy[(wi && kH) ? (uint32_t)iw * sHy + (uint32_t)j * (nz - 1) + k : 0];  // Ez: Hy across x
-    const float wx_y = hx[(wj && kH) ? i * sHx + (uint32_t)jw * (nz - 1) + k : 0];       // Ez: Hx across y
+    float wz_y = 0.0f, wy_z = 0.0f, wz_x = 0.0f, wx_z = 0.0f, wy_x = 0.0f, wx_y = 0.0f;
+    if (PER) {
+      wz_y = hz[(wj && iH) ? i * sHz + (uint32_t)jw * nz + k : 0];                 // Ex: Hz across y
+      wy_z = hy[(wk && iH) ? i * sHy + (uint32_t)j * (nz - 1) + kw : 0];           // Ex: Hy across z
+      wz_x = hz[(wi && jH) ? (uint32_t)iw * sHz + (uint32_t)j * nz + k : 0];       // Ey: Hz across x
+      wx_z = hx[(wk && jH) ? i * sHx + (uint32_t)j * (nz - 1) + kw : 0];           // Ey: Hx across z
+      wy_x = hy[(wi && kH) ? (uint32_t)iw * sHy + (uint32_t)j * (nz - 1) + k : 0];  // Ez: Hy across x
+      wx_y = hx[(wj && kH) ? i * sHx + (uint32_t)jw * (nz - 1) + k : 0];           // Ez: Hx across y
+    }
     if (iH) {  // Ex (nx-1, ny, nz): a = dHz/dy, b = dHy/dz, curl = a - b
       const float jsum = ADE ? ade_advance(a.ad, 0, ox, exv) : 0.0f;
       const bool pec = face_pec(j, ny, ly, hym) || face_pec(k, nz, lz, hzm);
@@ -463,13 +469,13 @@ __device__ __forceinline__ void shell_e_cells(
 
 // E of the shell cells of block `logical` for step `step` with that step's DFT
 // weights wc / ws.
-template <bool CPML, bool ADE, bool DFT>
+template <bool CPML, bool ADE, bool DFT, bool PER>
 __device__ __forceinline__ void shell_e_tile(const ShellArgs& sa, int logical, int step, const float* wc,
                                              const float* ws) {
   ShellCell c;
   if (!shell_cell(sa, logical, c)) return;
   const FusedArgs& a = sa.f;
-  shell_e_cells<CPML, ADE, DFT>(sa, c, step, G(wc), G(ws), G(a.out.h[0]), G(a.out.h[1]), G(a.out.h[2]), G(a.in.e[0]),
+  shell_e_cells<CPML, ADE, DFT, PER>(sa, c, step, G(wc), G(ws), G(a.out.h[0]), G(a.out.h[1]), G(a.out.h[2]), G(a.in.e[0]),
                                 G(a.in.e[1]), G(a.in.e[2]), G(a.out.e[0]), G(a.out.e[1]), G(a.out.e[2]),
                                 G(a.cp.psi[0]), G(a.cp.psi[1]), G(a.cp.psi[2]), G(a.cp.psi[3]), G(a.cp.psi[4]),
                                 G(a.cp.psi[5]), G(a.cf.decay[0]), G(a.cf.decay[1]), G(a.cf.decay[2]),
@@ -482,9 +488,9 @@ __global__ __launch_bounds__(256) void shell_h_kernel(ShellArgs sa) {
   shell_h_tile<CPML>(sa, xcd_logical_block(), sa.f.step);
 }
 
-template <bool CPML, bool ADE, bool DFT>
+template <bool CPML, bool ADE, bool DFT, bool PER>
 __global__ __launch_bounds__(256) void shell_e_kernel(ShellArgs sa) {
-  shell_e_tile<CPML, ADE, DFT>(sa, xcd_logical_block(), sa.f.step, sa.f.dft_cos, sa.f.dft_sin);
+  shell_e_tile<CPML, ADE, DFT, PER>(sa, xcd_logical_block(), sa.f.step, sa.f.dft_cos, sa.f.dft_sin);
 }
 
 // Batches: args[2 s + p] is scene s's shell for steps of parity p (dft_cos /
@@ -501,7 +507,7 @@ __global__ __launch_bounds__(256) void shell_h_batch_kernel(const ShellArgs* __r
   shell_h_tile<CPML>(args[2 * s + (n & 1)], logical - prefix[s], n);
 }
 
-template <bool CPML, bool ADE, bool DFT>
+template <bool CPML, bool ADE, bool DFT, bool PER>
 __global__ __launch_bounds__(256) void shell_e_batch_kernel(const ShellArgs* __restrict__ args,
                                                             const int* __restrict__ prefix, int nscenes, int total,
                                                             const int* __restrict__ counter) {
@@ -511,7 +517,7 @@ __global__ __launch_bounds__(256) void shell_e_batch_kernel(const ShellArgs* __r
   const int n = *counter;
   const ShellArgs& sa = args[2 * s + (n & 1)];
   const int64_t w = (int64_t)n * sa.f.nf_dft;
-  shell_e_tile<CPML, ADE, DFT>(sa, logical - prefix[s], n, sa.f.dft_cos + w, sa.f.dft_sin + w);
+  shell_e_tile<CPML, ADE, DFT, PER>(sa, logical - prefix[s], n, sa.f.dft_cos + w, sa.f.dft_sin + w);
 }
 
 }  // namespace mw

@@ -24,7 +24,20 @@
 
 namespace mw {
 
-template <bool ADE, bool DFT>
+// H coefficient of the z-edge tiles in the lean kernel's coefficient modes
+// (fused.h): modes 1 and 2 need no per-sample load at all.
+template <int COEF>
+__device__ __forceinline__ float zedge_h(const FusedArgs& a, int c, uint32_t o, float h_old, float curl) {
+  if (COEF == 1 || COEF == 2) return h_old - a.cf.curl_value[c] * curl;
+  return h_old * coef(a.cf.decay[c], a.cf.decay_value[c], o) - coef(a.cf.curl[c], a.cf.curl_value[c], o) * curl;
+}
+
+// Every load of a plane is issued together at the top of the plane, at offset 0
+// for a sample that does not exist (the value is then unused) and through a
+// stand-in buffer for a psi array the scene does not have, so the plane costs one
+// memory round trip: a load behind a run-time branch (a null test, a lane
+// condition) cannot be hoisted above the previous component's stores.
+template <bool ADE, bool DFT, int COEF>
 __device__ __forceinline__ void zedge_march(
     const FusedArgs& a, int step, int tz, int ty, int tx, gptr<const float> __restrict__ wc_t,
     gptr<const float> __restrict__ ws_t, gptr<const float> __restrict__ exi, gptr<const float> __restrict__ eyi,
@@ -33,7 +46,8 @@ __device__ __forceinline__ void zedge_march(
     gptr<float> __restrict__ ezo, gptr<float> __restrict__ hxo, gptr<float> __restrict__ hyo,
     gptr<float> __restrict__ hzo, gptr<const float> __restrict__ p7i, gptr<const float> __restrict__ p9i,
     gptr<float> __restrict__ p7o, gptr<float> __restrict__ p9o, gptr<float> __restrict__ p1,
-    gptr<float> __restrict__ p3, gptr<float> __restrict__ dre0, gptr<float> __restrict__ dim0,
+    gptr<float> __restrict__ p3, gptr<const float> __restrict__ kcx, gptr<const float> __restrict__ kcy,
+    gptr<const float> __restrict__ kcz, gptr<float> __restrict__ dre0, gptr<float> __restrict__ dim0,
     gptr<float> __restrict__ dre1, gptr<float> __restrict__ dim1, gptr<float> __restrict__ dre2,
     gptr<float> __restrict__ dim2) {
   const int nx = a.g.nx, ny = a.g.ny, nz = a.g.nz;
@@ -71,6 +85,12 @@ __device__ __forceinline__ void zedge_march(
       l3 = slab_local(k, cp.psi_low[3], cp.psi_high_start[3], cp.psi_high_len[3]);
     }
   }
+  // psi arrays the scene does not have read the E^n field instead (offset 0, never
+  // used: their slab positions are all -1); the stores test the real pointer
+  gptr<const float> p7r = p7i ? p7i : exi;
+  gptr<const float> p9r = p9i ? p9i : exi;
+  gptr<const float> p1r = p1 ? (gptr<const float>)p1 : exi;
+  gptr<const float> p3r = p3 ? (gptr<const float>)p3 : exi;
   const int len7 = cp.psi_low[7] + cp.psi_high_len[7], len9 = cp.psi_low[9] + cp.psi_high_len[9];
   const int len1 = cp.psi_low[1] + cp.psi_high_len[1], len3 = cp.psi_low[3] + cp.psi_high_len[3];
 
@@ -99,11 +119,11 @@ __device__ __forceinline__ void zedge_march(
     const bool store = lane > 0 && i >= i0;
     const float ihx = G(a.g.inv_h[0])[i];
     const float ihy_m = G(a.g.inv_h[1])[j - 1], ihy_c = G(a.g.inv_h[1])[j];
-    float eym_k1 = lane_down(em.ey), exm_k1 = lane_down(em.ex);
+    const float iex = G(a.g.inv_e[0])[i], iey = G(a.g.inv_e[1])[j];
+    float eym_k1 = lane_down(em.ey);
     float eyc_k1 = lane_down(ec.ey), exc_k1 = lane_down(ec.ex);
-    if (lane63) { eym_k1 = em.ey_k1; exm_k1 = em.ex_k1; eyc_k1 = ec.ey_k1; exc_k1 = ec.ex_k1; }
-    (void)exm_k1;
-    // loads of this plane
+    if (lane63) { eym_k1 = em.ey_k1; eyc_k1 = ec.ey_k1; exc_k1 = ec.ex_k1; }
+    // every load of this plane
     const uint32_t oxm = kv ? i * sHx + (uint32_t)(j - 1) * (nz - 1) + k : 0;
     const uint32_t ozm = kin ? i * sHz + (uint32_t)(j - 1) * nz + k : 0;
     const uint32_t oxc = kv ? i * sHx + (uint32_t)j * (nz - 1) + k : 0;
@@ -113,39 +133,36 @@ __device__ __forceinline__ void zedge_march(
     const uint32_t q7m = l7 >= 0 ? slab_offset(2, i, j - 1, k, l7, ny - 1, nz - 1, len7) : 0;
     const uint32_t q7c = l7 >= 0 ? slab_offset(2, i, j, k, l7, ny - 1, nz - 1, len7) : 0;
     const uint32_t q9c = l9 >= 0 ? slab_offset(2, i, j, k, l9, ny, nz - 1, len9) : 0;
-    float p7m = 0.0f, p7c = 0.0f, p9c = 0.0f;
-    if (p7i) { p7m = p7i[q7m]; p7c = p7i[q7c]; }
-    if (p9i) p9c = p9i[q9c];
+    const float p7m = p7r[q7m], p7c = p7r[q7c], p9c = p9r[q9c];
+    const uint32_t ox = kin ? i * sEx + (uint32_t)j * nz + k : 0;
+    const uint32_t oy = kin ? i * sEy + (uint32_t)j * nz + k : 0;
+    const uint32_t oz = kv ? i * sEz + (uint32_t)j * (nz - 1) + k : 0;
+    const uint32_t q1 = l1 >= 0 ? slab_offset(2, i, j, k, l1, ny, nz, len1) : 0;
+    const uint32_t q3 = l3 >= 0 ? slab_offset(2, i, j, k, l3, ny - 1, nz, len3) : 0;
+    const float p1v = p1r[q1], p3v = p3r[q3];
+    float ccx = 0.0f, ccy = 0.0f, ccz = 0.0f;  // E curl coefficients of mode 1
+    if (COEF == 1) { ccx = kcx[ox]; ccy = kcy[oy]; ccz = kcz[oz]; }
     // H, row j-1 (Hx, Hz: what row j's E needs)
-    float hx_m = 0.0f, hz_m = 0.0f, hx_c = 0.0f, hy_c = 0.0f, hz_c = 0.0f;
+    float hx_m, hz_m, hx_c, hy_c, hz_c;
     {
       const float da = (ec.ez - em.ez) * ihy_m, db = (eym_k1 - em.ey) * ihz;
       const float pz = l7 >= 0 ? bhz * p7m + chz * db : 0.0f;
-      const float curl = da - (db * khz + pz);
-      if (kv) hx_m = hxm_old * coef(a.cf.decay[3], a.cf.decay_value[3], oxm) - coef(a.cf.curl[3], a.cf.curl_value[3], oxm) * curl;
+      hx_m = kv ? zedge_h<COEF>(a, 3, oxm, hxm_old, da - (db * khz + pz)) : 0.0f;
     }
-    {
-      const float curl = (nm.ey - em.ey) * ihx - (ec.ex - em.ex) * ihy_m;
-      if (kin) hz_m = hzm_old * coef(a.cf.decay[5], a.cf.decay_value[5], ozm) - coef(a.cf.curl[5], a.cf.curl_value[5], ozm) * curl;
-    }
+    hz_m = kin ? zedge_h<COEF>(a, 5, ozm, hzm_old, (nm.ey - em.ey) * ihx - (ec.ex - em.ex) * ihy_m) : 0.0f;
     // H, row j
-    float p7n = 0.0f, p9n = 0.0f;
+    float p7n, p9n;
     {
       const float da = (ep.ez - ec.ez) * ihy_c, db = (eyc_k1 - ec.ey) * ihz;
       p7n = l7 >= 0 ? bhz * p7c + chz * db : 0.0f;
-      const float curl = da - (db * khz + p7n);
-      if (kv) hx_c = hxc_old * coef(a.cf.decay[3], a.cf.decay_value[3], oxc) - coef(a.cf.curl[3], a.cf.curl_value[3], oxc) * curl;
+      hx_c = kv ? zedge_h<COEF>(a, 3, oxc, hxc_old, da - (db * khz + p7n)) : 0.0f;
     }
     {
       const float da = (nc.ez - ec.ez) * ihx, db = (exc_k1 - ec.ex) * ihz;
       p9n = l9 >= 0 ? bhz * p9c + chz * db : 0.0f;
-      const float curl = (db * khz + p9n) - da;
-      if (kv) hy_c = hyc_old * coef(a.cf.decay[4], a.cf.decay_value[4], oyc) - coef(a.cf.curl[4], a.cf.curl_value[4], oyc) * curl;
+      hy_c = kv ? zedge_h<COEF>(a, 4, oyc, hyc_old, (db * khz + p9n) - da) : 0.0f;
     }
-    {
-      const float curl = (nc.ey - ec.ey) * ihx - (ep.ex - ec.ex) * ihy_c;
-      if (kin) hz_c = hzc_old * coef(a.cf.decay[5], a.cf.decay_value[5], ozc) - coef(a.cf.curl[5], a.cf.curl_value[5], ozc) * curl;
-    }
+    hz_c = kin ? zedge_h<COEF>(a, 5, ozc, hzc_old, (nc.ey - ec.ey) * ihx - (ep.ex - ec.ex) * ihy_c) : 0.0f;
     if (ht) {
       if (kv) {
         hx_m += term_value(a.h_terms, a.n_h_terms, 3, i, j - 1, k, step, a.dt);
@@ -168,82 +185,76 @@ __device__ __forceinline__ void zedge_march(
     }
     const float hy_km = lane_up(hy_c);
     const float hx_km = lane_up(hx_c);
+    // E of row j: the face modes of the z faces, z CPML on Ex / Ey
+    const bool zpec = face_pec(k, nz, lz, hzm);
+    const bool zupd = !zpec && !face_inactive(k, nz, lz, hzm);
+    float ex, ey, ez;
+    {  // Ex: a = dHz/dy, b = dHy/dz, curl = a - b
+      const float da = (hz_c - hz_m) * iey;
+      const float db = kint ? (hy_c - hy_km) * iez : shell_edge(k == 0, k == 0 ? lz : hzm, hy_c, hy_km, 0.0f, iez);
+      const float pb = l1 >= 0 ? bez * p1v + cez * db : 0.0f;
+      if (store && zupd && l1 >= 0 && p1) p1[q1] = pb;
+      const float curl = da - (db * kez + pb);
+      if (COEF == 1) ex = ec.ex + ccx * curl;
+      else if (COEF == 2) ex = ec.ex + a.cf.curl_value[0] * curl;
+      else ex = ec.ex * coef(a.cf.decay[0], a.cf.decay_value[0], ox) + coef(a.cf.curl[0], a.cf.curl_value[0], ox) * curl;
+      if (!zupd) ex = ec.ex;
+    }
+    {  // Ey: a = dHz/dx, b = dHx/dz, curl = b - a
+      const float da = (hz_c - hz_prev) * iex;
+      const float db = kint ? (hx_c - hx_km) * iez : shell_edge(k == 0, k == 0 ? lz : hzm, hx_c, hx_km, 0.0f, iez);
+      const float pb = l3 >= 0 ? bez * p3v + cez * db : 0.0f;
+      if (store && zupd && l3 >= 0 && p3) p3[q3] = pb;
+      const float curl = (db * kez + pb) - da;
+      if (COEF == 1) ey = ec.ey + ccy * curl;
+      else if (COEF == 2) ey = ec.ey + a.cf.curl_value[1] * curl;
+      else ey = ec.ey * coef(a.cf.decay[1], a.cf.decay_value[1], oy) + coef(a.cf.curl[1], a.cf.curl_value[1], oy) * curl;
+      if (!zupd) ey = ec.ey;
+    }
+    {  // Ez: a = dHy/dx, b = dHx/dy, curl = a - b (no z derivative)
+      const float curl = (hy_c - hy_prev) * iex - (hx_c - hx_m) * iey;
+      if (COEF == 1) ez = ec.ez + ccz * curl;
+      else if (COEF == 2) ez = ec.ez + a.cf.curl_value[2] * curl;
+      else ez = ec.ez * coef(a.cf.decay[2], a.cf.decay_value[2], oz) + coef(a.cf.curl[2], a.cf.curl_value[2], oz) * curl;
+    }
     if (store) {
-      const float iex = G(a.g.inv_e[0])[i], iey = G(a.g.inv_e[1])[j];
-      const uint32_t ox = kin ? i * sEx + (uint32_t)j * nz + k : 0;
-      const uint32_t oy = kin ? i * sEy + (uint32_t)j * nz + k : 0;
-      const uint32_t oz = kv ? i * sEz + (uint32_t)j * (nz - 1) + k : 0;
-      const uint32_t q1 = l1 >= 0 ? slab_offset(2, i, j, k, l1, ny, nz, len1) : 0;
-      const uint32_t q3 = l3 >= 0 ? slab_offset(2, i, j, k, l3, ny - 1, nz, len3) : 0;
-      float p1v = 0.0f, p3v = 0.0f;
-      if (p1) p1v = p1[q1];
-      if (p3) p3v = p3[q3];
-      if (kin) {  // Ex: a = dHz/dy, b = dHy/dz, curl = a - b
-        const float jsum = ADE ? ade_advance(a.ad, 0, ox, ec.ex) : 0.0f;
-        const bool pec = face_pec(k, nz, lz, hzm);
-        float e = ec.ex;
-        if (!pec && !face_inactive(k, nz, lz, hzm)) {
-          const float da = (hz_c - hz_m) * iey;
-          const float db = kint ? (hy_c - hy_km) * iez : shell_edge(k == 0, k == 0 ? lz : hzm, hy_c, hy_km, 0.0f, iez);
-          const float pb = l1 >= 0 ? bez * p1v + cez * db : 0.0f;
-          if (l1 >= 0 && p1) p1[q1] = pb;
-          e = ec.ex * coef(a.cf.decay[0], a.cf.decay_value[0], ox) +
-              coef(a.cf.curl[0], a.cf.curl_value[0], ox) * (da - (db * kez + pb));
+      if (ADE) {
+        if (kin) {
+          const float jx = ade_advance(a.ad, 0, ox, ec.ex), jy = ade_advance(a.ad, 1, oy, ec.ey);
+          if (jx != 0.0f) ex -= lean_ade_gain<COEF>(a, 0, ox, kcx) * jx;
+          if (jy != 0.0f) ey -= lean_ade_gain<COEF>(a, 1, oy, kcy) * jy;
         }
-        if (ADE && jsum != 0.0f)
-          e -= ade_gain(a.ad, 0, ox, coef(a.cf.decay[0], a.cf.decay_value[0], ox),
-                        coef(a.cf.curl[0], a.cf.curl_value[0], ox)) * jsum;
-        if (et) e += term_value(a.e_terms, a.n_e_terms, 0, i, j, k, step, a.dt);
-        if (pec) e = 0.0f;
-        exo[ox] = e;
-        if (DFT) {
-          const uint64_t n = (uint64_t)sEx * (nx - 1);
-          for (int f = 0; f < a.nf_dft; ++f) {
-            dre0[f * n + ox] += e * wc_t[f];
-            dim0[f * n + ox] += e * ws_t[f];
-          }
+        if (kv) {
+          const float jz = ade_advance(a.ad, 2, oz, ec.ez);
+          if (jz != 0.0f) ez -= lean_ade_gain<COEF>(a, 2, oz, kcz) * jz;
         }
       }
-      if (kin) {  // Ey: a = dHz/dx, b = dHx/dz, curl = b - a
-        const float jsum = ADE ? ade_advance(a.ad, 1, oy, ec.ey) : 0.0f;
-        const bool pec = face_pec(k, nz, lz, hzm);
-        float e = ec.ey;
-        if (!pec && !face_inactive(k, nz, lz, hzm)) {
-          const float da = (hz_c - hz_prev) * iex;
-          const float db = kint ? (hx_c - hx_km) * iez : shell_edge(k == 0, k == 0 ? lz : hzm, hx_c, hx_km, 0.0f, iez);
-          const float pb = l3 >= 0 ? bez * p3v + cez * db : 0.0f;
-          if (l3 >= 0 && p3) p3[q3] = pb;
-          e = ec.ey * coef(a.cf.decay[1], a.cf.decay_value[1], oy) +
-              coef(a.cf.curl[1], a.cf.curl_value[1], oy) * ((db * kez + pb) - da);
+      if (et) {
+        if (kin) {
+          ex += term_value(a.e_terms, a.n_e_terms, 0, i, j, k, step, a.dt);
+          ey += term_value(a.e_terms, a.n_e_terms, 1, i, j, k, step, a.dt);
         }
-        if (ADE && jsum != 0.0f)
-          e -= ade_gain(a.ad, 1, oy, coef(a.cf.decay[1], a.cf.decay_value[1], oy),
-                        coef(a.cf.curl[1], a.cf.curl_value[1], oy)) * jsum;
-        if (et) e += term_value(a.e_terms, a.n_e_terms, 1, i, j, k, step, a.dt);
-        if (pec) e = 0.0f;
-        eyo[oy] = e;
-        if (DFT) {
-          const uint64_t n = (uint64_t)sEy * nx;
-          for (int f = 0; f < a.nf_dft; ++f) {
-            dre1[f * n + oy] += e * wc_t[f];
-            dim1[f * n + oy] += e * ws_t[f];
-          }
-        }
+        if (kv) ez += term_value(a.e_terms, a.n_e_terms, 2, i, j, k, step, a.dt);
       }
-      if (kv) {  // Ez: a = dHy/dx, b = dHx/dy, curl = a - b (no z derivative)
-        const float jsum = ADE ? ade_advance(a.ad, 2, oz, ec.ez) : 0.0f;
-        float e = ec.ez * coef(a.cf.decay[2], a.cf.decay_value[2], oz) +
-                  coef(a.cf.curl[2], a.cf.curl_value[2], oz) * ((hy_c - hy_prev) * iex - (hx_c - hx_m) * iey);
-        if (ADE && jsum != 0.0f)
-          e -= ade_gain(a.ad, 2, oz, coef(a.cf.decay[2], a.cf.decay_value[2], oz),
-                        coef(a.cf.curl[2], a.cf.curl_value[2], oz)) * jsum;
-        if (et) e += term_value(a.e_terms, a.n_e_terms, 2, i, j, k, step, a.dt);
-        ezo[oz] = e;
-        if (DFT) {
-          const uint64_t n = (uint64_t)sEz * nx;
-          for (int f = 0; f < a.nf_dft; ++f) {
-            dre2[f * n + oz] += e * wc_t[f];
-            dim2[f * n + oz] += e * ws_t[f];
+      if (zpec) { ex = 0.0f; ey = 0.0f; }
+      if (kin) {
+        exo[ox] = ex;
+        eyo[oy] = ey;
+      }
+      if (kv) ezo[oz] = ez;
+      if (DFT) {
+        const uint64_t nxs = (uint64_t)sEx * (nx - 1), nys = (uint64_t)sEy * nx, nzs = (uint64_t)sEz * nx;
+        for (int f = 0; f < a.nf_dft; ++f) {
+          const float wc = wc_t[f], ws = ws_t[f];
+          if (kin) {
+            dre0[f * nxs + ox] += ex * wc;
+            dim0[f * nxs + ox] += ex * ws;
+            dre1[f * nys + oy] += ey * wc;
+            dim1[f * nys + oy] += ey * ws;
+          }
+          if (kv) {
+            dre2[f * nzs + oz] += ez * wc;
+            dim2[f * nzs + oz] += ez * ws;
           }
         }
       }
@@ -258,7 +269,7 @@ __device__ __forceinline__ void zedge_march(
 
 // Tile t of the z-edge launch: the z index runs over the low edge tiles
 // [0, lz0) then the high ones [lz1, tiles_z), over the lean box's y and x tiles.
-template <bool ADE, bool DFT>
+template <bool ADE, bool DFT, int COEF>
 __device__ __forceinline__ void zedge_tile(const FusedArgs& a, int step, const float* wc_t, const float* ws_t, int t) {
   const int nze = a.lz0 + (a.tiles_z - a.lz1);
   const int by = a.ly1 - a.ly0;
@@ -266,25 +277,26 @@ __device__ __forceinline__ void zedge_tile(const FusedArgs& a, int step, const f
   const int tz = zi < a.lz0 ? zi : a.lz1 + (zi - a.lz0);
   const int ty = a.ly0 + (t / nze) % by;
   const int tx = a.lx0 + t / (nze * by);
-  zedge_march<ADE, DFT>(a, step, tz, ty, tx, G(wc_t), G(ws_t), G(a.in.e[0]), G(a.in.e[1]), G(a.in.e[2]),
-                        G(a.in.h[0]), G(a.in.h[1]), G(a.in.h[2]), G(a.out.e[0]), G(a.out.e[1]), G(a.out.e[2]),
-                        G(a.out.h[0]), G(a.out.h[1]), G(a.out.h[2]), G(a.cp.psi[7]), G(a.cp.psi[9]),
-                        G(a.psi_h_out[1]), G(a.psi_h_out[3]), G(a.cp.psi[1]), G(a.cp.psi[3]), G(a.dft_re[0]),
-                        G(a.dft_im[0]), G(a.dft_re[1]), G(a.dft_im[1]), G(a.dft_re[2]), G(a.dft_im[2]));
+  zedge_march<ADE, DFT, COEF>(a, step, tz, ty, tx, G(wc_t), G(ws_t), G(a.in.e[0]), G(a.in.e[1]), G(a.in.e[2]),
+                              G(a.in.h[0]), G(a.in.h[1]), G(a.in.h[2]), G(a.out.e[0]), G(a.out.e[1]),
+                              G(a.out.e[2]), G(a.out.h[0]), G(a.out.h[1]), G(a.out.h[2]), G(a.cp.psi[7]),
+                              G(a.cp.psi[9]), G(a.psi_h_out[1]), G(a.psi_h_out[3]), G(a.cp.psi[1]), G(a.cp.psi[3]),
+                              G(a.cf.curl[0]), G(a.cf.curl[1]), G(a.cf.curl[2]), G(a.dft_re[0]), G(a.dft_im[0]),
+                              G(a.dft_re[1]), G(a.dft_im[1]), G(a.dft_re[2]), G(a.dft_im[2]));
 }
 
 __host__ __device__ inline int zedge_tiles(const FusedArgs& a) {
   return a.zedge ? (a.lz0 + (a.tiles_z - a.lz1)) * (a.ly1 - a.ly0) * (a.lx1 - a.lx0) : 0;
 }
 
-template <bool ADE, bool DFT>
+template <bool ADE, bool DFT, int COEF>
 __global__ __launch_bounds__(256) void zedge_step_kernel(FusedArgs a) {
   const int logical = xcd_logical_block();
   if (logical >= zedge_tiles(a)) return;
-  zedge_tile<ADE, DFT>(a, a.step, a.dft_cos, a.dft_sin, logical);
+  zedge_tile<ADE, DFT, COEF>(a, a.step, a.dft_cos, a.dft_sin, logical);
 }
 
-template <bool ADE, bool DFT>
+template <bool ADE, bool DFT, int COEF>
 __global__ __launch_bounds__(256) void zedge_batch_kernel(const FusedArgs* __restrict__ args,
                                                           const int* __restrict__ prefix, int nscenes, int total,
                                                           const int* __restrict__ counter) {
@@ -294,7 +306,7 @@ __global__ __launch_bounds__(256) void zedge_batch_kernel(const FusedArgs* __res
   const int n = *counter;
   const FusedArgs& a = args[2 * s + (n & 1)];
   const int64_t w = (int64_t)n * a.nf_dft;
-  zedge_tile<ADE, DFT>(a, n, a.dft_cos + w, a.dft_sin + w, logical - prefix[s]);
+  zedge_tile<ADE, DFT, COEF>(a, n, a.dft_cos + w, a.dft_sin + w, logical - prefix[s]);
 }
 
 }  // namespace mw
