@@ -2,7 +2,7 @@
 with the float64 oracle.  Run by tests/test_emulated_kernels.py in a child process
 (AddressSanitizer must be preloaded into the interpreter that loads the library).
 
-usage: python run_case.py LIB [CASE ...]   (cases: boxes staged dispersive periodic; default all)"""
+usage: python run_case.py LIB [CASE ...]   (cases: boxes staged dispersive periodic adjoint; default all)"""
 import contextlib
 import ctypes
 import inspect
@@ -177,7 +177,52 @@ def case_periodic():
     run("periodic z, z-edge tiles asked", pk, {"frequencies": (configs.F0,), "courant": 0.9}, 10, (0, 4, 1))
 
 
-CASES = {"boxes": case_boxes, "staged": case_staged, "dispersive": case_dispersive, "periodic": case_periodic}
+def case_adjoint():
+    # the reverse sweep (interior and boundary-box reverse kernels, seeds, window
+    # capture and coefficient gradients) against the oracle adjoint: CPML with
+    # Drude + Lorentz poles, and periodic x / PEC y / PMC z faces
+    import test_gpu_adjoint as ta
+    from oracle.driver import OracleAdjoint
+    from witwin_maxwell_amd.fdtd.adjoint import DeviceAdjoint, design_window, restore
+    from conftest import norm_err
+    for bname in ("cpml_ade", "periodic_pec_pmc"):
+        scene, _ = ta._design_case(bname)
+        scene.device = "cpu"
+        steps = 40
+        plan = compile_plan(scene, frequencies=(4e14, 5e14), time_steps=steps, courant=0.9)
+        ref = OracleRun(plan)
+        ref.trajectory = []
+        ref.run(steps)
+        rng = np.random.default_rng(11)
+        g_planes = [(rng.standard_normal(a.shape), rng.standard_normal(a.shape)) for a in ref.plane_re]
+        g_point = (rng.standard_normal(ref.point_re.shape), rng.standard_normal(ref.point_im.shape))
+        radj = OracleAdjoint(ref)
+        radj.run_reverse({"plane_re": [g[0] for g in g_planes], "plane_im": [g[1] for g in g_planes],
+                          "point_re": g_point[0], "point_im": g_point[1]})
+        window = design_window(plan)
+        dev = runtime.DeviceRun(plan, "cpu")
+        dev.run(steps)
+        adj = DeviceAdjoint(dev, window, capacity=steps)
+        adj.zero()
+        t = lambda a: torch.as_tensor(a, dtype=torch.float32)
+        adj.set_seeds(point=(t(g_point[0]), t(g_point[1])), planes=[(t(r), t(i)) for r, i in g_planes])
+        restore(dev, None, 0)
+        adj.capture(0, steps)
+        adj.reverse(0, steps)
+        errs = family_err({c: adj.fields[c].double().numpy() for c in COMPONENTS[:3]},
+                          {c: radj.ae[i] for i, c in enumerate(COMPONENTS[:3])})
+        lo, hi = window
+        box = tuple(slice(l, h) for l, h in zip(lo, hi))
+        for ci in range(3):
+            errs[f"g_curl{ci}"] = norm_err(adj.g_curl[ci].double().numpy(), radj.g_curl[ci][box]) * 1e-2
+            errs[f"g_decay{ci}"] = norm_err(adj.g_decay[ci].double().numpy(), radj.g_decay[ci][box]) * 1e-2
+        worst = max(errs.values())
+        print(f"adjoint {bname}: {plan.shape} max err {worst:.2e}", flush=True)
+        assert worst <= 1e-5, errs
+
+
+CASES = {"boxes": case_boxes, "staged": case_staged, "dispersive": case_dispersive, "periodic": case_periodic,
+         "adjoint": case_adjoint}
 
 for name in (sys.argv[2:] or list(CASES)):
     CASES[name]()

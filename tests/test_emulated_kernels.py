@@ -45,7 +45,7 @@ def test_single_pass_step_matches_oracle_under_emulation(emulated_library):
     """The case groups of run_case.py, each in its own process, side by side."""
     lib, asan = emulated_library
     env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0", OMP_NUM_THREADS="1")
-    groups = ("boxes", "staged", "dispersive", "periodic")
+    groups = ("boxes", "staged", "dispersive", "periodic", "adjoint")
     procs = {g: subprocess.Popen([sys.executable, os.path.join(EMU, "run_case.py"), lib, g], stdout=subprocess.PIPE,
                                  stderr=subprocess.PIPE, text=True, env=env) for g in groups}
     failed = []

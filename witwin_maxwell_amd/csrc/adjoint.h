@@ -277,6 +277,252 @@ __device__ __forceinline__ void reverse_e_at(const RevArgs& a, int i, int j, int
   }
 }
 
+// ---------------------------------------------------------------- boundary boxes
+// The same P1 / P2 sums as reverse_h_at / reverse_e_at, term by term in the same
+// order, with every component and derivative axis fixed at compile time, every
+// load of a node issued before its first store, and no load behind a run-time
+// test: a sample that does not exist (or lies outside a psi slab) is read at
+// offset 0, an array the scene does not have (a scalar coefficient, a psi slab of
+// an axis without PML) through a stand-in buffer, and the value is then scaled by
+// 0 (every value read is a finite field, coefficient or psi sample).  A select
+// would let the compiler move the load behind a branch, and the argument table's
+// pointers carry no aliasing information, so a load after a store waits for it:
+// written the general way, one node cost several memory round trips.
+template <int C, int T> struct EDer {  // term T of the E_C update
+  static constexpr int ax = T == 0 ? (C == 0 ? 1 : 0) : (C == 2 ? 1 : 2);
+  static constexpr int psi = 2 * C + T;
+};
+template <int H, int T> struct HDer {  // term T of the H_H update
+  static constexpr int ax = T == 0 ? (H == 0 ? 1 : 0) : (H == 2 ? 1 : 2);
+  static constexpr int psi = 6 + 2 * H + T;
+};
+
+__device__ __forceinline__ float keep_if(bool c) { return c ? 1.0f : 0.0f; }
+
+// coefficient array value (or its scalar) with the load outside any branch
+__device__ __forceinline__ float coef_nb(const float* arr, const float* stand_in, float value, uint32_t o) {
+  const float v = G(arr ? arr : stand_in)[arr ? o : 0u];
+  const float has = keep_if(arr != nullptr);
+  return v * has + value * (1.0f - has);
+}
+
+// e_mask without early exits: 1 PEC, 2 held by a none/PML face, 0 active
+template <int C>
+__device__ __forceinline__ int e_mask_nb(const mw_grid& g, int i, int j, int k) {
+  bool pec = false, held = false;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int ax = t == 0 ? EDer<C, 0>::ax : EDer<C, 1>::ax;
+    const int x = pick(ax, i, j, k), n = axis_size(g, ax), lo = g.bc_lo[ax], hi = g.bc_hi[ax];
+    const bool at_lo = x == 0, at_hi = x == n - 1;
+    pec = pec | (at_lo & (lo == MW_BC_PEC)) | (at_hi & (hi == MW_BC_PEC));
+    held = held | (at_lo & ((lo == MW_BC_NONE) | (lo == MW_BC_PML))) | (at_hi & ((hi == MW_BC_NONE) | (hi == MW_BC_PML)));
+  }
+  return pec ? 1 : (held ? 2 : 0);
+}
+
+// q of E sample (C; i,j,k) for term T (q_e), 0 when `valid` is false
+template <int C, int T, bool CPML>
+__device__ __forceinline__ float q_nb(const RevArgs& a, int i, int j, int k, bool valid) {
+  constexpr int ax = EDer<C, T>::ax, q = EDer<C, T>::psi;
+  const float s = e_sign(C, T);
+  int sx, sy, sz;
+  e_shape(a.g, C, sx, sy, sz);
+  const uint32_t o = valid ? ((uint32_t)i * sy + j) * sz + k : 0u;
+  const int x = valid ? pick(ax, i, j, k) : 0;
+  const float mu = coef_nb(a.cf.curl[C], a.a.e[C], a.cf.curl_value[C], o) * G(a.a.e[C])[o];
+  float w = s * mu;
+  if (CPML) {
+    const int local = slab_local(x, a.cp.psi_low[q], a.cp.psi_high_start[q], a.cp.psi_high_len[q]);
+    const bool in = valid && local >= 0;
+    const uint32_t po = in ? slab_offset(ax, i, j, k, local, sy, sz, a.cp.psi_low[q] + a.cp.psi_high_len[q]) : 0u;
+    const float lp = G(a.cp.psi[q] ? a.cp.psi[q] : a.a.e[C])[po] * keep_if(in);
+    w = s * mu * G(a.cp.inv_kappa_e[ax])[x] + G(a.cp.c_e[ax])[x] * (lp + s * mu);
+  }
+  return w * G(a.g.inv_e[ax])[x] * keep_if(valid && e_mask_nb<C>(a.g, i, j, k) == 0);
+}
+
+// gather_e for E component C, term T, at H sample (i,j,k) (which exists iff `ok`);
+// PER: the grid has periodic faces (the seam samples are then read as well)
+template <int C, int T, bool CPML, bool PER>
+__device__ __forceinline__ float gather_nb(const RevArgs& a, int i, int j, int k, bool ok) {
+  constexpr int ax = EDer<C, T>::ax;
+  const int n = axis_size(a.g, ax), lo = a.g.bc_lo[ax], hi = a.g.bc_hi[ax];
+  const int m = pick(ax, i, j, k);
+  const bool first = m > 0, inner = m + 1 < n - 1;
+  const float qm = q_nb<C, T, CPML>(a, i, j, k, ok);
+  const float qp = q_nb<C, T, CPML>(a, ax == 0 ? i + 1 : i, ax == 1 ? j + 1 : j, ax == 2 ? k + 1 : k,
+                                    ok && m + 1 <= n - 1);
+  // the samples across a periodic seam (used on the face lanes of periodic axes)
+  float q_first = 0.0f, q_last = 0.0f;
+  const bool use_last = !first && hi == MW_BC_PERIODIC, use_first = !inner && lo == MW_BC_PERIODIC;
+  if (PER) {
+    q_last = q_nb<C, T, CPML>(a, ax == 0 ? n - 1 : i, ax == 1 ? n - 1 : j, ax == 2 ? n - 1 : k, ok && use_last);
+    q_first = q_nb<C, T, CPML>(a, ax == 0 ? 0 : i, ax == 1 ? 0 : j, ax == 2 ? 0 : k, ok && use_first);
+  }
+  // weights of q(m) and q(m+1): 1 inside; at a face the periodic wrap (1), the PMC
+  // mirror (2) or nothing (electric.cu:23-41 transposed)
+  const float wm = first ? 1.0f : (lo == MW_BC_PERIODIC ? 1.0f : (lo == MW_BC_PMC ? 2.0f : 0.0f));
+  const float wp = inner ? 1.0f : (hi == MW_BC_PERIODIC ? 1.0f : (hi == MW_BC_PMC ? 2.0f : 0.0f));
+  float s = 0.0f;
+  s += wm * qm;
+  s += q_last * keep_if(use_last);
+  s -= wp * qp;
+  s -= q_first * keep_if(use_first);
+  return s;
+}
+
+// r of H sample (H; i,j,k) for term T (r_h), 0 when `valid` is false
+template <int H, int T, bool CPML>
+__device__ __forceinline__ float r_nb(const RevArgs& a, int i, int j, int k, bool valid) {
+  constexpr int ax = HDer<H, T>::ax, q = HDer<H, T>::psi;
+  const float s = h_sign(H, T);
+  int sx, sy, sz;
+  h_shape(a.g, H, sx, sy, sz);
+  const uint32_t o = valid ? ((uint32_t)i * sy + j) * sz + k : 0u;
+  const int x = valid ? pick(ax, i, j, k) : 0;
+  const float nu = -coef_nb(a.cf.curl[3 + H], a.a.h[H], a.cf.curl_value[3 + H], o) * G(a.a.h[H])[o];
+  float w = s * nu;
+  if (CPML) {
+    const int local = slab_local(x, a.cp.psi_low[q], a.cp.psi_high_start[q], a.cp.psi_high_len[q]);
+    const bool in = valid && local >= 0;
+    const uint32_t po = in ? slab_offset(ax, i, j, k, local, sy, sz, a.cp.psi_low[q] + a.cp.psi_high_len[q]) : 0u;
+    const float lp = G(a.cp.psi[q] ? a.cp.psi[q] : a.a.h[H])[po] * keep_if(in);
+    w = s * nu * G(a.cp.inv_kappa_h[ax])[x] + G(a.cp.c_h[ax])[x] * (lp + s * nu);
+  }
+  return w * G(a.g.inv_h[ax])[x] * keep_if(valid);
+}
+
+// fwd(H, T) of reverse_e_at for the E sample at (i,j,k) (which exists iff `ok`)
+template <int H, int T, bool CPML>
+__device__ __forceinline__ float fwd_nb(const RevArgs& a, int i, int j, int k, bool ok) {
+  constexpr int ax = HDer<H, T>::ax;
+  const int x = pick(ax, i, j, k), n = axis_size(a.g, ax);
+  const float rm = r_nb<H, T, CPML>(a, ax == 0 ? i - 1 : i, ax == 1 ? j - 1 : j, ax == 2 ? k - 1 : k, ok && x >= 1);
+  const float rc = r_nb<H, T, CPML>(a, i, j, k, ok && x <= n - 2);
+  float s = 0.0f;
+  s += rm;  // 0 when x = 0
+  s -= rc;  // 0 when x = n-1
+  return s;
+}
+
+// A psi sample to rewrite once every load of the node has issued.
+struct PsiStore {
+  float v;
+  uint32_t o;
+  bool in;
+};
+
+template <int H, int T>
+__device__ __forceinline__ PsiStore psi_h_finish(const RevArgs& a, int i, int j, int k, bool ok, int sy, int sz,
+                                                 float nu) {
+  constexpr int ax = HDer<H, T>::ax, q = HDer<H, T>::psi;
+  const int x = ok ? pick(ax, i, j, k) : 0;
+  const int local = slab_local(x, a.cp.psi_low[q], a.cp.psi_high_start[q], a.cp.psi_high_len[q]);
+  PsiStore st;
+  st.in = ok && local >= 0;
+  st.o = st.in ? slab_offset(ax, i, j, k, local, sy, sz, a.cp.psi_low[q] + a.cp.psi_high_len[q]) : 0u;
+  const float pv = G(a.cp.psi[q] ? a.cp.psi[q] : a.a.h[H])[st.o];
+  st.v = G(a.cp.b_h[ax])[x] * (pv + h_sign(H, T) * nu);
+  return st;
+}
+
+template <int C, int T>
+__device__ __forceinline__ PsiStore psi_e_finish(const RevArgs& a, int i, int j, int k, bool upd, int sy, int sz,
+                                                 float mu) {
+  constexpr int ax = EDer<C, T>::ax, q = EDer<C, T>::psi;
+  const int x = upd ? pick(ax, i, j, k) : 0;
+  const int local = slab_local(x, a.cp.psi_low[q], a.cp.psi_high_start[q], a.cp.psi_high_len[q]);
+  PsiStore st;
+  st.in = upd && local >= 0;
+  st.o = st.in ? slab_offset(ax, i, j, k, local, sy, sz, a.cp.psi_low[q] + a.cp.psi_high_len[q]) : 0u;
+  const float pv = G(a.cp.psi[q] ? a.cp.psi[q] : a.a.e[C])[st.o];
+  st.v = G(a.cp.b_e[ax])[x] * (pv + e_sign(C, T) * mu);
+  return st;
+}
+
+// Result of one component at one node: the new adjoint and its psi rewrites.
+struct RevOut {
+  float v;  // P2: the held/decayed part (the pole and curl parts follow)
+  float f;  // P2: the curl-transpose part
+  uint32_t o;
+  bool ok;
+  PsiStore p0, p1;
+};
+
+// P1 for component H at node (i,j,k), stores deferred
+template <int H, bool CPML, bool PER>
+__device__ __forceinline__ RevOut reverse_h_nb(const RevArgs& a, int i, int j, int k) {
+  int sx, sy, sz;
+  h_shape(a.g, H, sx, sy, sz);
+  RevOut r;
+  r.ok = i < sx && j < sy && k < sz;
+  r.o = r.ok ? ((uint32_t)i * sy + j) * sz + k : 0u;
+  const float lam = G(a.a.h[H])[r.o];
+  const float dec = coef_nb(a.cf.decay[3 + H], a.a.h[H], a.cf.decay_value[3 + H], r.o);
+  float g;
+  if (H == 0) g = gather_nb<2, 1, CPML, PER>(a, i, j, k, r.ok) + gather_nb<1, 1, CPML, PER>(a, i, j, k, r.ok);  // Ez b, Ey b
+  else if (H == 1) g = gather_nb<0, 1, CPML, PER>(a, i, j, k, r.ok) + gather_nb<2, 0, CPML, PER>(a, i, j, k, r.ok);
+  else g = gather_nb<0, 0, CPML, PER>(a, i, j, k, r.ok) + gather_nb<1, 0, CPML, PER>(a, i, j, k, r.ok);
+  r.p0.in = r.p1.in = false;
+  if (CPML) {
+    const float nu = -coef_nb(a.cf.curl[3 + H], a.a.h[H], a.cf.curl_value[3 + H], r.o) * lam;
+    r.p0 = psi_h_finish<H, 0>(a, i, j, k, r.ok, sy, sz, nu);
+    r.p1 = psi_h_finish<H, 1>(a, i, j, k, r.ok, sy, sz, nu);
+  }
+  float v = dec * lam;
+  v += g;
+  r.v = v;
+  r.f = 0.0f;
+  return r;
+}
+
+template <int H, bool CPML>
+__device__ __forceinline__ void store_h(const RevArgs& a, const RevOut& r) {
+  constexpr int q0 = HDer<H, 0>::psi, q1 = HDer<H, 1>::psi;
+  if (CPML && r.p0.in) G(a.cp.psi[q0])[r.p0.o] = r.p0.v;
+  if (CPML && r.p1.in) G(a.cp.psi[q1])[r.p1.o] = r.p1.v;
+  if (r.ok) G(a.a.h[H])[r.o] = r.v;
+}
+
+// P2 for component C at node (i,j,k) without ADE, stores deferred
+template <int C, bool CPML>
+__device__ __forceinline__ RevOut reverse_e_nb(const RevArgs& a, int i, int j, int k, float& lam_out) {
+  int sx, sy, sz;
+  e_shape(a.g, C, sx, sy, sz);
+  RevOut r;
+  r.ok = i < sx && j < sy && k < sz;
+  r.o = r.ok ? ((uint32_t)i * sy + j) * sz + k : 0u;
+  const float lam = G(a.a.e[C])[r.o];
+  const float dec = coef_nb(a.cf.decay[C], a.a.e[C], a.cf.decay_value[C], r.o);
+  float f;
+  if (C == 0) f = fwd_nb<1, 1, CPML>(a, i, j, k, r.ok) + fwd_nb<2, 1, CPML>(a, i, j, k, r.ok);       // Hy b, Hz b
+  else if (C == 1) f = fwd_nb<0, 1, CPML>(a, i, j, k, r.ok) + fwd_nb<2, 0, CPML>(a, i, j, k, r.ok);  // Hx b, Hz a
+  else f = fwd_nb<0, 0, CPML>(a, i, j, k, r.ok) + fwd_nb<1, 0, CPML>(a, i, j, k, r.ok);              // Hx a, Hy a
+  const int mask = e_mask_nb<C>(a.g, i, j, k);
+  // PEC: 0; none/PML face: lambda unchanged; active: decay * lambda
+  float out = (dec * lam) * keep_if(mask == 0);
+  out += lam * keep_if(mask == 2);
+  r.p0.in = r.p1.in = false;
+  if (CPML) {
+    const float mu = coef_nb(a.cf.curl[C], a.a.e[C], a.cf.curl_value[C], r.o) * lam;
+    r.p0 = psi_e_finish<C, 0>(a, i, j, k, r.ok && mask == 0, sy, sz, mu);
+    r.p1 = psi_e_finish<C, 1>(a, i, j, k, r.ok && mask == 0, sy, sz, mu);
+  }
+  r.v = out;
+  r.f = f;
+  lam_out = lam;
+  return r;
+}
+
+template <int C, bool CPML>
+__device__ __forceinline__ void store_e(const RevArgs& a, const RevOut& r) {
+  constexpr int q0 = EDer<C, 0>::psi, q1 = EDer<C, 1>::psi;
+  if (CPML && r.p0.in) G(a.cp.psi[q0])[r.p0.o] = r.p0.v;
+  if (CPML && r.p1.in) G(a.cp.psi[q1])[r.p1.o] = r.p1.v;
+  if (r.ok) G(a.a.e[C])[r.o] = r.v;
+}
+
 // Interior forms of P1 and P2 for a wavefront whose samples all lie in the
 // in_lo/in_hi box: no faces, no psi, every neighbour inside the grid, so each
 // gather is a plain difference of mu = curl_E lambda_E (P1) or nu = -curl_H
@@ -407,18 +653,42 @@ __device__ __forceinline__ bool box_node(const RevArgs& a, int& i, int& j, int& 
   return k < a.box_hi[b][2] && j < a.box_hi[b][1];
 }
 
-template <bool CPML>
+template <bool CPML, bool PER>
 __global__ __launch_bounds__(256) void reverse_h_box_kernel(RevArgs a) {
   int i, j, k;
   if (!box_node(a, i, j, k)) return;
-  reverse_h_at<CPML>(a, i, j, k);
+  // P1 writes only lambda_H and the H-side psi of its own samples and reads
+  // neither, so every load of the three components issues before any store
+  const RevOut hx = reverse_h_nb<0, CPML, PER>(a, i, j, k);
+  const RevOut hy = reverse_h_nb<1, CPML, PER>(a, i, j, k);
+  const RevOut hz = reverse_h_nb<2, CPML, PER>(a, i, j, k);
+  store_h<0, CPML>(a, hx);
+  store_h<1, CPML>(a, hy);
+  store_h<2, CPML>(a, hz);
 }
 
 template <bool CPML, bool ADE>
 __global__ __launch_bounds__(256) void reverse_e_box_kernel(RevArgs a) {
   int i, j, k;
   if (!box_node(a, i, j, k)) return;
-  reverse_e_at<CPML, ADE>(a, i, j, k);
+  // P2 writes only lambda_E, the E-side psi and the pole adjoints of its own
+  // samples; the pole transposes (which read and write pole state) follow the
+  // loads of all three components
+  float lx, ly, lz;
+  RevOut ex = reverse_e_nb<0, CPML>(a, i, j, k, lx);
+  RevOut ey = reverse_e_nb<1, CPML>(a, i, j, k, ly);
+  RevOut ez = reverse_e_nb<2, CPML>(a, i, j, k, lz);
+  if (ADE) {
+    if (ex.ok) ex.v += reverse_ade(a, 0, ex.o, lx);
+    if (ey.ok) ey.v += reverse_ade(a, 1, ey.o, ly);
+    if (ez.ok) ez.v += reverse_ade(a, 2, ez.o, lz);
+  }
+  ex.v += ex.f;
+  ey.v += ey.f;
+  ez.v += ez.f;
+  store_e<0, CPML>(a, ex);
+  store_e<1, CPML>(a, ey);
+  store_e<2, CPML>(a, ez);
 }
 
 // Batched reverse phases: block b of the launch belongs to the scene s with

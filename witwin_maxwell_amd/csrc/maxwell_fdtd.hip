@@ -1355,6 +1355,7 @@ int mw_reverse_steps(const mw_program* p, const mw_adjoint* adj, const mw_window
   const dim3 grid((a.in_hi[2] - a.in_lo[2] + 63) / 64, (a.in_hi[1] - a.in_lo[1] + 3) / 4, a.in_hi[0] - a.in_lo[0]);
   const bool interior = a.in_hi[0] > a.in_lo[0];
   const dim3 bgrid(a.total);
+  const bool per = grid_periodic(*g);
   const mw::WindowArgs wa = w ? window_args(w) : mw::WindowArgs{};
   const int64_t we = w ? window_cells(w, 0) : 0, wh = w ? window_cells(w, 1) : 0;
   for (int n = first + count - 1; n >= first; --n) {
@@ -1365,8 +1366,13 @@ int mw_reverse_steps(const mw_program* p, const mw_adjoint* adj, const mw_window
       if ((rc = check_launch("reverse_h"))) return rc;
     }
     if (a.total > 0) {
-      if (a.cpml) hipLaunchKernelGGL(mw::reverse_h_box_kernel<true>, bgrid, block, 0, s, a);
-      else hipLaunchKernelGGL(mw::reverse_h_box_kernel<false>, bgrid, block, 0, s, a);
+      if (per) {
+        if (a.cpml) hipLaunchKernelGGL((mw::reverse_h_box_kernel<true, true>), bgrid, block, 0, s, a);
+        else hipLaunchKernelGGL((mw::reverse_h_box_kernel<false, true>), bgrid, block, 0, s, a);
+      } else {
+        if (a.cpml) hipLaunchKernelGGL((mw::reverse_h_box_kernel<true, false>), bgrid, block, 0, s, a);
+        else hipLaunchKernelGGL((mw::reverse_h_box_kernel<false, false>), bgrid, block, 0, s, a);
+      }
       if ((rc = check_launch("reverse_h (boundary boxes)"))) return rc;
     }
     if ((rc = seeds(n, 1))) return rc;
