@@ -256,7 +256,9 @@ typedef struct mw_program {
   int32_t nf_obs, n_planes, n_points;
   /* single-pass tiling of the z ends: 0 or 2 = interior tiles aligned to the z
    * PML slabs with the z ends in the per-cell shell, 1 = z-edge tiles beside the
-   * interior tiles */
+   * interior tiles, 3 = interior tiles aligned as for 2 and both z ends of two
+   * rows in one wavefront (when each end and a halo lane fit 32 lanes and the z
+   * faces are not periodic; otherwise as 2) */
   int32_t z_tiling;
   const float* obs_e_cos;
   const float* obs_e_sin;

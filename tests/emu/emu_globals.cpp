@@ -1,6 +1,10 @@
 // Lockstep wavefront execution for the CPU emulation header (hip/hip_runtime.h).
 #include "hip/hip_runtime.h"
 #include <barrier>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
 #include <thread>
 #include <vector>
 thread_local uint3e blockIdx, threadIdx, blockDim, gridDim;
@@ -71,4 +75,23 @@ void emu_run_grid(dim3 g, dim3 b, const std::function<void()>& job) {
           p.start.arrive_and_wait();
           p.end.arrive_and_wait();
         }
+}
+
+namespace {
+std::mutex launch_mutex;
+std::map<std::string, long>& launch_counts() { static std::map<std::string, long> m; return m; }
+}  // namespace
+
+void emu_note_launch(const char* name) {
+  std::lock_guard<std::mutex> lock(launch_mutex);
+  ++launch_counts()[name];
+}
+
+// launches so far of the kernels whose launch expression contains `part`
+extern "C" long emu_kernel_launches(const char* part) {
+  std::lock_guard<std::mutex> lock(launch_mutex);
+  long n = 0;
+  for (const auto& kv : launch_counts())
+    if (std::strstr(kv.first.c_str(), part)) n += kv.second;
+  return n;
 }

@@ -69,4 +69,7 @@ template <typename F, typename... A>
 void emu_launch(F f, dim3 g, dim3 b, A... args) {
   emu_run_grid(g, b, [=]() { f(args...); });
 }
-#define hipLaunchKernelGGL(K, G, B, SH, ST, ...) emu_launch(K, dim3(G), dim3(B), __VA_ARGS__)
+// every launch is counted by kernel name (emu_kernel_launches), so a test can
+// tell which kernels a case ran
+void emu_note_launch(const char* name);
+#define hipLaunchKernelGGL(K, G, B, SH, ST, ...) (emu_note_launch(#K), emu_launch(K, dim3(G), dim3(B), __VA_ARGS__))

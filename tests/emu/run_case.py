@@ -65,9 +65,19 @@ def _monitor_errs(ref, dev, need_field=False):
     return errs
 
 
-def run(name, scene, kw, steps, tiling, splits=(), stage_depth=32, need_field=False, fused=True):
+lib.emu_kernel_launches.argtypes = [ctypes.c_char_p]
+lib.emu_kernel_launches.restype = ctypes.c_long
+
+
+def launches(part):
+    return lib.emu_kernel_launches(part.encode())
+
+
+def run(name, scene, kw, steps, tiling, splits=(), stage_depth=32, need_field=False, fused=True, expect=None):
     """Run `steps` steps in calls ending at `splits` (then the rest) with z planes
-    staged `stage_depth` steps at a time, and compare fields, psi and monitors."""
+    staged `stage_depth` steps at a time, and compare fields, psi and monitors;
+    `expect`: a kernel name that the run must launch."""
+    before = launches(expect) if expect else 0
     scene.device = "cpu"
     kw = dict(kw)
     kw.pop("theta", None)
@@ -92,6 +102,8 @@ def run(name, scene, kw, steps, tiling, splits=(), stage_depth=32, need_field=Fa
     worst = max(errs.values())
     print(f"{name}: {plan.shape} tiling {tiling} max err {worst:.2e}", flush=True)
     assert worst <= 1e-5, errs
+    if expect:
+        assert launches(expect) > before, f"{name}: no {expect} launch"
 
 
 dx = configs.DX
@@ -111,7 +123,8 @@ def case_boxes():
                                   frequencies=kw["frequencies"]))
     s.add_monitor(mw.PlaneMonitor("x_mid", axis="x", position=0.0, fields=("Ey",), frequencies=kw["frequencies"]))
     run("boxes, aligned z tiling", s, kw, 8, (3, 2, 2), splits=(2, 6), stage_depth=3)
-    run("boxes, z-edge tiles", s, kw, 8, (0, 4, 1), splits=(3,), stage_depth=4)
+    run("boxes, z-edge tiles", s, kw, 8, (0, 4, 1), splits=(3,), stage_depth=4, expect="zedge_step_kernel")
+    run("boxes, z-end tiles", s, kw, 8, (3, 2, 3), splits=(5,), stage_depth=3, expect="zends_step_kernel")
 
 
 def case_staged():
@@ -134,6 +147,8 @@ def case_staged():
     skw = {"frequencies": freqs, "courant": 0.9, "window": "none"}
     run("staged planes, aligned z tiling", st, skw, 11, (3, 2, 2), splits=(2, 6, 9), stage_depth=4, need_field=True)
     run("staged planes, z-edge tiles", st, skw, 11, (0, 4, 1), splits=(3, 5), stage_depth=4, need_field=True)
+    run("staged planes, z-end tiles", st, skw, 11, (0, 4, 3), splits=(3, 5), stage_depth=4, need_field=True,
+        expect="zends_step_kernel")
     run("staged planes, two-kernel step", st, skw, 11, (0, 0, 0), splits=(4,), stage_depth=3, need_field=True,
         fused=False)
 
@@ -155,6 +170,7 @@ def case_dispersive():
     dkw = {"frequencies": (configs.F0,), "courant": 0.9}
     run("dispersive, aligned z tiling", d, dkw, 12, (3, 2, 2))
     run("dispersive, z-edge tiles", d, dkw, 12, (0, 4, 1))
+    run("dispersive, z-end tiles", d, dkw, 12, (0, 4, 3), expect="zends_step_kernel")
 
 
 def case_periodic():
@@ -168,6 +184,8 @@ def case_periodic():
                                  source_time=mw.GaussianPulse(configs.F0, configs.F0 / 4)))
     run("periodic x, PMC y, aligned z tiling", pz, {"frequencies": (configs.F0,), "courant": 0.9}, 10, (3, 2, 2))
     run("periodic x, PMC y, z-edge tiles", pz, {"frequencies": (configs.F0,), "courant": 0.9}, 10, (0, 4, 1))
+    run("periodic x, PMC y, z-end tiles", pz, {"frequencies": (configs.F0,), "courant": 0.9}, 10, (0, 4, 3),
+        expect="zends_step_kernel")
     # periodic z faces: the z ends stay shell cells even when z-edge tiles are asked for
     pk = mw.Scene(domain=mw.Domain(((-6 * dx, 6 * dx), (-7 * dx, 7 * dx), (-37 * dx, 37 * dx))),
                   grid=mw.GridSpec.uniform(dx), boundary=mw.BoundarySpec.faces(default="pml", z="periodic",
@@ -175,6 +193,7 @@ def case_periodic():
     pk.add_source(mw.PointDipole((0.2 * dx, 0.3 * dx, 36.6 * dx), (0.5, 0.3, 1.0), profile="ideal",
                                  source_time=mw.GaussianPulse(configs.F0, configs.F0 / 4)))
     run("periodic z, z-edge tiles asked", pk, {"frequencies": (configs.F0,), "courant": 0.9}, 10, (0, 4, 1))
+    run("periodic z, z-end tiles asked", pk, {"frequencies": (configs.F0,), "courant": 0.9}, 10, (0, 4, 3))
 
 
 def case_adjoint():

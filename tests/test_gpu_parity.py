@@ -202,7 +202,7 @@ def test_boundary_modes(kind, fused):
     _check_monitors(ref, dev)
 
 
-@pytest.mark.parametrize("tiling", [(3, 2, 2), (0, 4, 1)], ids=["aligned", "zedge-asked"])
+@pytest.mark.parametrize("tiling", [(3, 2, 2), (0, 4, 1), (0, 4, 3)], ids=["aligned", "zedge-asked", "zends-asked"])
 @pytest.mark.parametrize("faces", ["x_periodic_y_pmc", "z_periodic", "xy_periodic"])
 def test_periodic_faces_single_pass(faces, tiling):
     """Periodic faces in the single-pass step (the shell's E pass reads H across
@@ -279,6 +279,52 @@ def test_lean_interior_kernel(case, tiling):
     _check_monitors(ref, dev)
 
 
+def _z_ends_fit(plan):
+    """Do both z ends of this grid, with the interior tiles aligned to the z PML
+    (63 owned columns per tile, one halo column before), fit half a wavefront with
+    a halo lane each?  (The condition under which z_tiling 3 takes effect.)"""
+    nz = plan.shape[2]
+    lay = {n: plan.psi_layout[n] for n in ("psi_hx_z", "psi_ex_z")}
+    klo = max(1, lay["psi_hx_z"][1], lay["psi_ex_z"][1])
+    khi = min(nz - 1, lay["psi_hx_z"][2] if lay["psi_hx_z"][3] > 0 else nz - 1,
+              lay["psi_ex_z"][2] if lay["psi_ex_z"][3] > 0 else nz)
+    first = klo + 1
+    off = first % 63 - 63 if first % 63 else 0
+    z0 = off + 63 * ((first - off) // 63)
+    n = 0
+    while first + 63 * (n + 1) <= khi:
+        n += 1
+    z1 = z0 + 63 * n
+    return n > 0 and (z0 + 1) + (nz - z1 + 1) <= 32
+
+
+@pytest.mark.parametrize("case", ["vacuum_dft", "boxes_soft_source", "dispersive"])
+def test_z_end_tiles(case):
+    """z_tiling 3: interior tiles aligned to the z PML, and both z ends (CPML
+    layers and faces) of two rows advanced by one wavefront, on grids whose z ends
+    fit that layout; homogeneous + full-field DFT, per-sample curl with soft
+    source planes, fused ADE."""
+    print(f"case: z-end tiles, {case}", flush=True)
+    dx = configs.DX
+    if case == "vacuum_dft":
+        scene, kw = configs.vacuum_dipole(cells=20, pml=8)
+        scene.domain = mw.Domain(((-12 * dx, 12 * dx), (-14 * dx, 14 * dx), (-34 * dx, 34 * dx)))
+        steps = 160
+    elif case == "boxes_soft_source":
+        scene, kw = configs.dielectric_boxes(cells=24, pml=8, n_boxes=8, box_range=(4, 12), seed=5)
+        scene.domain = mw.Domain(((-12 * dx, 12 * dx), (-14 * dx, 14 * dx), (-34 * dx, 34 * dx)))
+        steps = 200
+    else:
+        scene, kw = configs.dispersive_spheres(cells=(24, 26, 68), pml=8, fill=0.3, radius=(3, 7), seed=2,
+                                               injection="soft", pulse_delay=3e-15)
+        steps = 1400
+    plan, ref, dev = _run_both(scene, kw, steps, True, (4, 2, 3))
+    assert _z_ends_fit(plan), plan.shape
+    _check_fields(ref, dev)
+    _check_psi(plan, ref, dev)
+    _check_monitors(ref, dev)
+
+
 @pytest.mark.parametrize("mode,fused", [("slab", True), ("slab", False), ("box", True), ("box", False)])
 def test_tfsf_plane_wave(mode, fused):
     """TFSF faces driven by the auxiliary line: slab (dispersive spheres, CPML
@@ -304,7 +350,7 @@ def test_tfsf_plane_wave(mode, fused):
     _check_monitors(ref, dev)
 
 
-@pytest.mark.parametrize("tiling", [None, (3, 1, 1)], ids=["aligned", "zedge"])
+@pytest.mark.parametrize("tiling", [None, (3, 1, 1), (3, 1, 3)], ids=["aligned", "zedge", "zends"])
 def test_staged_planes_equal_per_step_sums(tiling):
     """z-normal plane monitors staged and folded a block at a time (in calls that
     end inside blocks and at their ends) give the per-step sums to the bit: the

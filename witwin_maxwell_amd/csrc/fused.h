@@ -67,6 +67,10 @@ struct FusedArgs {
   // 1: the z tiles outside [lz0, lz1) over the lean y-x range are z-edge tiles
   // (zedge.h); 0: those columns belong to the per-cell shell (shell.h)
   int zedge;
+  // 1: the z ends [0, zend_lo) and [zend_hi, nz) over the lean y-x range are
+  // z-end tiles (zedge.h: both ends of two rows in one wavefront); 0: see zedge
+  int zends;
+  int zend_lo, zend_hi;
 };
 
 __device__ __forceinline__ float lane_up(float v) { return __shfl_up(v, 1, 64); }      // lane-1's value

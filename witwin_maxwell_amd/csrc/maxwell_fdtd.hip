@@ -236,6 +236,12 @@ void dispatch_lean(dim3 grid, dim3 block, hipStream_t s, const mw::FusedArgs& a,
 
 template <bool ADE, bool DFT>
 void dispatch_zedge(dim3 grid, dim3 block, hipStream_t s, const mw::FusedArgs& a, int mode) {
+  if (a.zends) {
+    if (mode == 2) hipLaunchKernelGGL((mw::zends_step_kernel<ADE, DFT, 2>), grid, block, 0, s, a);
+    else if (mode == 1) hipLaunchKernelGGL((mw::zends_step_kernel<ADE, DFT, 1>), grid, block, 0, s, a);
+    else hipLaunchKernelGGL((mw::zends_step_kernel<ADE, DFT, 0>), grid, block, 0, s, a);
+    return;
+  }
   if (mode == 2) hipLaunchKernelGGL((mw::zedge_step_kernel<ADE, DFT, 2>), grid, block, 0, s, a);
   else if (mode == 1) hipLaunchKernelGGL((mw::zedge_step_kernel<ADE, DFT, 1>), grid, block, 0, s, a);
   else hipLaunchKernelGGL((mw::zedge_step_kernel<ADE, DFT, 0>), grid, block, 0, s, a);
@@ -383,6 +389,18 @@ int build_fused_args(const mw_program* p, int n, int chunk_override, mw::FusedAr
   a.zedge = zedge ? 1 : 0;
   int nlean = (a.lz1 - a.lz0) * (a.ly1 - a.ly0) * (a.lx1 - a.lx0);
   if (nlean <= 0) { nlean = 0; a.lz0 = a.lz1 = a.ly0 = a.ly1 = a.lx0 = a.lx1 = 0; }
+  // z-end tiles (z_tiling 3): both z ends of two rows per wavefront, when each end
+  // plus its halo lane fits half a wavefront; otherwise the z ends stay shell cells
+  a.zends = 0;
+  a.zend_lo = a.zend_hi = 0;
+  if (p->z_tiling == 3 && nlean > 0 && g->bc_lo[2] != MW_BC_PERIODIC) {
+    const int z0 = a.z_off + mw::kFOwnCols * a.lz0, z1 = a.z_off + mw::kFOwnCols * a.lz1;
+    if (z0 >= 1 && z1 < g->nz && (z0 + 1) + (g->nz - z1 + 1) <= 32) {
+      a.zends = 1;
+      a.zend_lo = z0;
+      a.zend_hi = z1;
+    }
+  }
   *nlean_out = nlean;
   return MW_OK;
 }
@@ -409,11 +427,11 @@ void build_shell_args(const mw::FusedArgs& a, int nlean, int chunk, mw::ShellArg
     Z0 = a.z_off + mw::kFOwnCols * a.lz0; Z1 = a.z_off + mw::kFOwnCols * a.lz1;
   }
   // with a lean box, its z ends [0, Z0) and [Z1, nz) over the lean x-y range are
-  // the z-edge tiles (zedge.h), not shell boxes
+  // the z-edge or z-end tiles (zedge.h) when those are on, not shell boxes
   const int boxes[6][6] = {{0, X0, 0, ny, 0, nz},  {X1, nx, 0, ny, 0, nz},  {X0, X1, 0, Y0, 0, nz},
                            {X0, X1, Y1, ny, 0, nz}, {X0, X1, Y0, Y1, 0, Z0}, {X0, X1, Y0, Y1, Z1, nz}};
   const int min_shift = (a.ad.count[0] + a.ad.count[1] + a.ad.count[2]) > 0 ? 4 : 5;
-  for (int bi = 0; bi < (a.zedge ? 4 : 6); ++bi) {
+  for (int bi = 0; bi < ((a.zedge || a.zends) ? 4 : 6); ++bi) {
     const int* bb = boxes[bi];
     const int xe = bb[1] - bb[0], ye = bb[3] - bb[2], ze = bb[5] - bb[4];
     if (xe <= 0 || ye <= 0 || ze <= 0) continue;
@@ -780,7 +798,10 @@ int collect_batch(const mw_program* const* programs, int count, int xchunk, Sink
     for (int par = 0; par < 2; ++par) {
       mw::FusedArgs a;
       int nlean = 0;
-      build_fused_args(p, par, xchunk > 0 ? xchunk : p->xchunk, &a, &nlean);
+      // the batch launches have no z-end tiles: z_tiling 3 runs as 2 there
+      mw_program q = *p;
+      if (q.z_tiling == 3) q.z_tiling = 2;
+      build_fused_args(&q, par, xchunk > 0 ? xchunk : p->xchunk, &a, &nlean);
       a.dft_cos = p->dft_cos;  // the batch kernels add step * nf
       a.dft_sin = p->dft_sin;
       mw::ShellArgs sh;
@@ -1015,7 +1036,7 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
     return fail(MW_ERR_ARGUMENT, "steps [%d, %d) outside the program's %d steps", first, first + count, p->total_steps);
   if (p->xchunk < 0 || p->xchunk > 256 || p->shell_chunk < 0 || p->shell_chunk > 256)
     return fail(MW_ERR_ARGUMENT, "xchunk %d / shell_chunk %d outside [0, 256]", p->xchunk, p->shell_chunk);
-  if (p->z_tiling < 0 || p->z_tiling > 2) return fail(MW_ERR_ARGUMENT, "z_tiling %d outside [0, 2]", p->z_tiling);
+  if (p->z_tiling < 0 || p->z_tiling > 3) return fail(MW_ERR_ARGUMENT, "z_tiling %d outside [0, 3]", p->z_tiling);
   if ((rc = check_terms(g, p->h_terms, p->n_h_terms, "h_terms", p->total_steps)) ||
       (rc = check_terms(g, p->e_terms, p->n_e_terms, "e_terms", p->total_steps)))
     return rc;
