@@ -276,7 +276,7 @@ def bench_gradient(args):
                                                    eps_bounds=region.eps_bounds)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        plan, mons, session = differentiable_run(scene, stride=stride, **kw)
+        plan, mons, session = differentiable_run(scene, stride=stride, tune=True, **kw)
         e = mons["objective"]
         loss = 0.0
         for c in ("Ex", "Ey"):
@@ -319,6 +319,7 @@ def bench_gradient(args):
                        "nodes": list(plan.shape)},
             "roofline": roofline, "cpu_baseline": cpu, "host": host_info(),
             "detail": {"forward_s": t_fwd, "backward_s": total - t_fwd,
+                       "tiling": [int(run.program.xchunk), int(run.program.shell_chunk), int(run.program.z_tiling)],
                        "forward_gnodes_per_s": nodes * T / t_fwd / 1e9,
                        "grad_norm": float(theta.grad.norm())}}
     print(json.dumps(line), flush=True)
@@ -472,7 +473,8 @@ def _short(name):
     step kernels, whose variants are timed separately)."""
     if "__amd_rocclr_copyBuffer" in name:
         return "copy"
-    for key in ("lean_step_kernel", "lean_batch_kernel", "zedge_step_kernel", "zedge_batch_kernel"):
+    for key in ("lean_step_kernel", "lean_batch_kernel", "zedge_step_kernel", "zedge_batch_kernel",
+                "zends_step_kernel"):
         if key in name:
             return name[name.index(key):].split("(")[0]
     return name.split("(")[0][:80]
@@ -522,9 +524,7 @@ def summarize(argv):
             # the committed form that the forward workload reads back as roofline.traffic:
             # the step kernels' bytes per step, keyed by the tiling they were measured at
             calib = summary.pop("_calibration")
-            step = sum(v["total_bytes"] for k, v in summary.items() if any(
-                f in k for f in ("lean_step_kernel", "shell_h_kernel", "shell_e_kernel", "zedge_step_kernel",
-                                 "shifted_values_kernel")))
+            step = step_traffic(summary)
             summary = {"by_tiling": {argv[3]: {STEP_KERNEL: step, "_per_kernel_bytes": summary,
                                                "_calibration": calib}}}
         print(json.dumps(summary, indent=1))
@@ -532,7 +532,15 @@ def summarize(argv):
         raise SystemExit("--summarize stats DIR | traffic FETCH_DIR WRITE_DIR [XCHUNK,SHELL_CHUNK,Z_TILING]")
 
 
-STEP_KERNEL = "single-pass step (lean_step_kernel + shell_h_kernel + shell_e_kernel)"
+STEP_KERNEL = "single-pass step (interior, z-end / z-edge, PML-shell H and E, term and aux-line kernels)"
+_STEP_FAMILIES = ("lean_step_kernel", "zends_step_kernel", "zedge_step_kernel", "shell_h_kernel", "shell_e_kernel",
+                  "shifted_values_kernel", "aux_line_h_kernel", "aux_line_e_kernel")
+
+
+def step_traffic(per_kernel):
+    """HBM bytes of one single-pass step: every kernel mw_run_steps launches for
+    the step itself (the monitors' gather/fold excluded, as in the step timing)."""
+    return sum(v["total_bytes"] for k, v in per_kernel.items() if any(f in k for f in _STEP_FAMILIES))
 
 
 def _fastest(timings, n=6):

@@ -334,10 +334,13 @@ class _FDTDGradientFunction(torch.autograd.Function):
         return (None, *[g.to(device=d, dtype=t) for g, (d, t) in zip(list(g_decay) + list(g_curl), ctx.coef_meta)])
 
 
-def differentiable_run(scene, region_densities=None, device="cuda", stride=None, fused=None, **plan_kwargs):
+def differentiable_run(scene, region_densities=None, device="cuda", stride=None, fused=None, tune=False,
+                       **plan_kwargs):
     """Run ``scene`` on the device with gradients enabled for its material
     regions.  Returns (plan, monitors) where the monitor payloads are torch
-    tensors differentiable in the regions' densities."""
+    tensors differentiable in the regions' densities.  ``tune``: time a few
+    single-pass tilings on the run first (the choice is kept for later runs of
+    the same signature in this process)."""
     from .plan import compile_plan
     from .spectral import assemble_monitor_results
     from ..scene import prepare_scene
@@ -349,7 +352,7 @@ def differentiable_run(scene, region_densities=None, device="cuda", stride=None,
     eps_node = mat.node_permittivity(prepared, model, region_densities)
     plan = compile_plan(scene, region_densities=[d.detach() for d in region_densities], **plan_kwargs)
     window = design_window(plan)
-    session = GradientSession(plan, device, window=window, stride=stride, fused=fused)
+    session = GradientSession(plan, device, window=window, stride=stride, fused=fused, tune=tune)
     lo, hi = window
     coefs = window_coefficients(plan, eps_node, lo, hi)
     # the device runs the plan's float32 coefficients; they must be the values

@@ -21,6 +21,9 @@ from .plan import COMPONENTS, FDTDPlan
 
 _KIND = {"debye": 0, "drude": 1, "lorentz": 2}
 
+# tilings chosen by DeviceRun.autotune in this process, per run signature
+_TUNED: dict = {}
+
 
 def _f32(a, device):
     return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float32), device=device)
@@ -361,9 +364,9 @@ class DeviceRun:
         candidate on this run, then reset the run to step 0.  Among candidates
         within ``margin`` of the fastest, the first of ``prefer`` (tilings the
         caller has other data for) is taken.  The choice is cached per (shape,
-        CPML, dispersive, DFT) signature in the JSON file named by
-        WITWIN_MAXWELL_AMD_TUNE_CACHE, when set.  Returns (xchunk, shell_chunk,
-        z_tiling)."""
+        CPML, dispersive, DFT) signature for the rest of the process and in the
+        JSON file named by WITWIN_MAXWELL_AMD_TUNE_CACHE, when set.  Returns
+        (xchunk, shell_chunk, z_tiling)."""
         import json
         import os
         if not self.fused:
@@ -375,7 +378,11 @@ class DeviceRun:
         if cache and path and os.path.exists(path):
             with open(path) as f:
                 table = json.load(f)
+        if cache and sig in _TUNED:
+            self.set_tiling(*_TUNED[sig])
+            return tuple(_TUNED[sig])
         if sig in table:
+            _TUNED[sig] = tuple(table[sig])
             self.set_tiling(*table[sig])
             return tuple(table[sig])
         steps = min(int(trial_steps), max(1, p.time_steps - 1))
@@ -403,6 +410,8 @@ class DeviceRun:
                 break
         self.set_tiling(*best)
         self.reset()
+        if cache:
+            _TUNED[sig] = tuple(best)
         if cache and path:
             table[sig] = list(best)
             with open(path, "w") as f:
