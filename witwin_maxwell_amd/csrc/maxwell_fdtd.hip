@@ -234,6 +234,18 @@ void dispatch_lean(dim3 grid, dim3 block, hipStream_t s, const mw::FusedArgs& a,
   else hipLaunchKernelGGL((mw::lean_step_kernel<ADE, DFT, 0>), grid, block, 0, s, a);
 }
 
+#ifndef MW_MERGE_SHELL_H
+#define MW_MERGE_SHELL_H 1
+#endif
+
+template <bool CPML, bool ADE, bool DFT>
+void dispatch_shell_h_lean(int sb, int lb, hipStream_t s, const mw::ShellArgs& sa, int mode) {
+  const dim3 grid(sb + lb), block(mw::kFLanes, mw::kFRows, 1);
+  if (mode == 2) hipLaunchKernelGGL((mw::shell_h_lean_kernel<CPML, ADE, DFT, 2>), grid, block, 0, s, sa, sb, lb);
+  else if (mode == 1) hipLaunchKernelGGL((mw::shell_h_lean_kernel<CPML, ADE, DFT, 1>), grid, block, 0, s, sa, sb, lb);
+  else hipLaunchKernelGGL((mw::shell_h_lean_kernel<CPML, ADE, DFT, 0>), grid, block, 0, s, sa, sb, lb);
+}
+
 template <bool ADE, bool DFT>
 void dispatch_zedge(dim3 grid, dim3 block, hipStream_t s, const mw::FusedArgs& a, int mode) {
   if (a.zends) {
@@ -492,13 +504,26 @@ int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_field
   const int mode = coef_mode(p->coeffs);
   const dim3 block(mw::kFLanes, mw::kFRows, 1);
   const dim3 sgrid(((sa.total + 7) / 8) * 8);
-  // shell H and lean read only this step's inputs and write disjoint cells
-  if (sa.total > 0) {
+  // shell H and lean read only this step's inputs and write disjoint cells: one
+  // launch when both have work
+  const bool merged = MW_MERGE_SHELL_H && sa.total > 0 && nlean > 0;
+  if (merged) {
+    const int sb = ((sa.total + 7) / 8) * 8, lb = ((nlean + 7) / 8) * 8;
+    if (cpml) {
+      if (ade) { if (dft) dispatch_shell_h_lean<true, true, true>(sb, lb, s, sa, mode); else dispatch_shell_h_lean<true, true, false>(sb, lb, s, sa, mode); }
+      else { if (dft) dispatch_shell_h_lean<true, false, true>(sb, lb, s, sa, mode); else dispatch_shell_h_lean<true, false, false>(sb, lb, s, sa, mode); }
+    } else {
+      if (ade) { if (dft) dispatch_shell_h_lean<false, true, true>(sb, lb, s, sa, mode); else dispatch_shell_h_lean<false, true, false>(sb, lb, s, sa, mode); }
+      else { if (dft) dispatch_shell_h_lean<false, false, true>(sb, lb, s, sa, mode); else dispatch_shell_h_lean<false, false, false>(sb, lb, s, sa, mode); }
+    }
+    if ((rc = check_launch("fused_step (shell H + lean)"))) return rc;
+  }
+  if (sa.total > 0 && !merged) {
     if (cpml) hipLaunchKernelGGL(mw::shell_h_kernel<true>, sgrid, dim3(64, 4, 1), 0, s, sa);
     else hipLaunchKernelGGL(mw::shell_h_kernel<false>, sgrid, dim3(64, 4, 1), 0, s, sa);
     if ((rc = check_launch("fused_step (shell H)"))) return rc;
   }
-  if (nlean > 0) {
+  if (nlean > 0 && !merged) {
     dim3 lgrid(((nlean + 7) / 8) * 8);
     if (ade) { if (dft) dispatch_lean<true, true>(lgrid, block, s, a, mode); else dispatch_lean<true, false>(lgrid, block, s, a, mode); }
     else { if (dft) dispatch_lean<false, true>(lgrid, block, s, a, mode); else dispatch_lean<false, false>(lgrid, block, s, a, mode); }

@@ -493,6 +493,28 @@ __global__ __launch_bounds__(256) void shell_e_kernel(ShellArgs sa) {
   shell_e_tile<CPML, ADE, DFT, PER>(sa, xcd_logical_block(), sa.f.step, sa.f.dft_cos, sa.f.dft_sin);
 }
 
+// Shell H and the interior tiles in one launch (both read only the step's inputs
+// and write disjoint cells): blocks [0, shell_blocks) do shell-H work, the rest
+// interior tiles, each range dealt to the XCDs in contiguous runs as the separate
+// launches deal them (both counts are multiples of 8).  The shell blocks are
+// dispatched first and overlap the start of the interior tiles instead of ending
+// in a launch of their own.
+template <bool CPML, bool ADE, bool DFT, int COEF>
+__global__ __launch_bounds__(256) void shell_h_lean_kernel(ShellArgs sa, int shell_blocks, int lean_blocks) {
+  const int w = blockIdx.x;
+  if (w < shell_blocks) {
+    shell_h_tile<CPML>(sa, (w % 8) * (shell_blocks / 8) + w / 8, sa.f.step);
+    return;
+  }
+  const int v = w - shell_blocks;
+  const int logical = (v % 8) * (lean_blocks / 8) + v / 8;
+  const FusedArgs& a = sa.f;
+  const int bz = a.lz1 - a.lz0, by = a.ly1 - a.ly0, bx = a.lx1 - a.lx0;
+  if (logical >= bz * by * bx) return;
+  lean_tile<ADE, DFT, COEF>(a, a.step, a.dft_cos, a.dft_sin, a.lz0 + logical % bz, a.ly0 + (logical / bz) % by,
+                            a.lx0 + logical / (bz * by));
+}
+
 // Batches: args[2 s + p] is scene s's shell for steps of parity p (dft_cos /
 // dft_sin the base of its weight tables), prefix[s] its first block; the step
 // index comes from the device counter.
