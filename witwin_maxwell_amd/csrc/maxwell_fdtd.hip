@@ -308,17 +308,30 @@ AxisTiling tile_axis_z(int size, int width, int lo, int hi) {
   return best;
 }
 
-int launch_shifted_values(const mw_term* t, int n, int step, double dt, hipStream_t s) {
+// This step's values of the time-shifted terms of both families (H then E), in
+// as few launches as the term set's capacity allows (one on every workload here).
+int launch_shifted_values(const mw_term* h, int nh, const mw_term* e, int ne, int step, double dt, hipStream_t s) {
   mw::TermSet set{};
   int64_t cells = 0;
-  for (int i = 0; i < n; ++i) {
-    if (t[i].kind != MW_TERM_SHIFTED) continue;
-    set.t[set.count++] = t[i];
-    cells = std::max<int64_t>(cells, (int64_t)t[i].shape[0] * t[i].shape[1] * t[i].shape[2]);
+  auto flush = [&]() {
+    if (set.count == 0) return MW_OK;
+    hipLaunchKernelGGL(mw::shifted_values_kernel, dim3(grid_blocks(cells), set.count), dim3(256), 0, s, set, step, dt);
+    set.count = 0;
+    cells = 0;
+    return check_launch("shifted_values");
+  };
+  for (int f = 0; f < 2; ++f) {
+    const mw_term* t = f ? e : h;
+    const int n = f ? ne : nh;
+    for (int i = 0; i < n; ++i) {
+      if (t[i].kind != MW_TERM_SHIFTED) continue;
+      if (set.count == MW_MAX_TERMS)
+        if (int rc = flush()) return rc;
+      set.t[set.count++] = t[i];
+      cells = std::max<int64_t>(cells, (int64_t)t[i].shape[0] * t[i].shape[1] * t[i].shape[2]);
+    }
   }
-  if (set.count == 0) return MW_OK;
-  hipLaunchKernelGGL(mw::shifted_values_kernel, dim3(grid_blocks(cells), set.count), dim3(256), 0, s, set, step, dt);
-  return check_launch("shifted_values");
+  return flush();
 }
 
 // Arguments of the single-pass step for step n (buffers by parity, this step's
@@ -492,8 +505,7 @@ int launch_shell_e(dim3 grid, hipStream_t s, const mw::ShellArgs& sa) {
 
 int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_fields) {
   int rc;
-  if ((rc = launch_shifted_values(p->h_terms, p->n_h_terms, n, p->dt, s))) return rc;
-  if ((rc = launch_shifted_values(p->e_terms, p->n_e_terms, n, p->dt, s))) return rc;
+  if ((rc = launch_shifted_values(p->h_terms, p->n_h_terms, p->e_terms, p->n_e_terms, n, p->dt, s))) return rc;
   mw::FusedArgs a;
   int nlean = 0;
   build_fused_args(p, n, p->xchunk, &a, &nlean);
