@@ -533,8 +533,9 @@ def summarize(argv):
 
 
 STEP_KERNEL = "single-pass step (interior, z-end / z-edge, PML-shell H and E, term and aux-line kernels)"
-_STEP_FAMILIES = ("lean_step_kernel", "zends_step_kernel", "zedge_step_kernel", "shell_h_kernel", "shell_e_kernel",
-                  "shifted_values_kernel", "aux_line_h_kernel", "aux_line_e_kernel")
+_STEP_FAMILIES = ("lean_step_kernel", "shell_h_lean_kernel", "zends_step_kernel", "zedge_step_kernel",
+                  "shell_h_kernel", "shell_e_kernel", "shifted_values_kernel", "aux_line_h_kernel",
+                  "aux_line_e_kernel")
 
 
 def step_traffic(per_kernel):
