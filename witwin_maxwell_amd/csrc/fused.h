@@ -71,6 +71,15 @@ struct FusedArgs {
   // z-end tiles (zedge.h: both ends of two rows in one wavefront); 0: see zedge
   int zends;
   int zend_lo, zend_hi;
+  // z-normal plane samples that the interior tiles store into this step's stage
+  // row (zrow == nullptr: none): zrow[zoff[q] + i * rows(zcomp[q]) + j] = component
+  // zcomp[q] at (i, j, zk[q]) for the interior cells of the launch; the gather
+  // takes the other samples
+  float* zrow;
+  int zcount;
+  int zcomp[MW_MAX_PLANES];
+  int zk[MW_MAX_PLANES];
+  int64_t zoff[MW_MAX_PLANES];
 };
 
 __device__ __forceinline__ float lane_up(float v) { return __shfl_up(v, 1, 64); }      // lane-1's value
@@ -144,6 +153,7 @@ template <int COEF, bool ADE, bool DFT>
 __device__ __forceinline__ void lean_plane(const FusedArgs& a, int i, int j, int k, bool store, const ERow& em,
                                            const ERow& ec, const ERow& ep, const ERow& nm, const ERow& nc,
                                            float& hy_prev, float& hz_prev, bool h_terms, bool e_terms, int step,
+                                           uint32_t zmask,
                                            gptr<const float> __restrict__ wc_t, gptr<const float> __restrict__ ws_t,
                                            gptr<const float> __restrict__ hxi, gptr<const float> __restrict__ hyi,
                                            gptr<const float> __restrict__ hzi, gptr<float> __restrict__ exo,
@@ -214,6 +224,15 @@ __device__ __forceinline__ void lean_plane(const FusedArgs& a, int i, int j, int
     exo[ox] = ex;
     eyo[oy] = ey;
     ezo[oz] = ez;
+    // staged z-normal plane samples of this tile's columns (zmask: wave-uniform)
+    for (uint32_t m = zmask; m != 0u; m &= m - 1u) {
+      const int q = __builtin_ctz(m);
+      if (k != a.zk[q]) continue;
+      const int c = a.zcomp[q];
+      const float v = c == 0 ? ex : c == 1 ? ey : c == 2 ? ez : c == 3 ? hx_c : c == 4 ? hy_c : hz_c;
+      const int rows = (c == 1 || c == 3 || c == 5) ? ny - 1 : ny;
+      G(a.zrow)[a.zoff[q] + (int64_t)i * rows + j] = v;
+    }
     if (DFT) {
       const uint64_t nxs = (uint64_t)sEx * (nx - 1), nys = (uint64_t)sEy * nx, nzs = (uint64_t)sEz * nx;
       for (int f = 0; f < a.nf_dft; ++f) {
@@ -296,11 +315,17 @@ __device__ __forceinline__ void lean_march(
   };
   ERow em = load3(i0 - 1, -1, true), ec = load3(i0 - 1, 0, true), ep = load3(i0 - 1, 1, false);
   float hy_prev = 0.0f, hz_prev = 0.0f;
+  // staged planes whose column this tile owns (lanes 1..63)
+  uint32_t zmask = 0u;
+  if (a.zrow)
+    for (int q = 0; q < a.zcount; ++q)
+      if (a.zk[q] > kc0 && a.zk[q] < kc0 + kFLanes) zmask |= 1u << q;
   for (int i = i0 - 1; i < i1; ++i) {
     const ERow nm = load3(i + 1, -1, true), nc = load3(i + 1, 0, true), np = load3(i + 1, 1, false);
     const bool ht = terms_touch(a.h_terms, a.n_h_terms, i, j - 1, j, kc0, kc0 + kFLanes - 1);
     const bool et = terms_touch(a.e_terms, a.n_e_terms, i, j, j, kc0, kc0 + kFLanes - 1);
     lean_plane<COEF, ADE, DFT>(a, i, j, k, lane > 0 && i >= i0, em, ec, ep, nm, nc, hy_prev, hz_prev, ht, et, step,
+                               zmask,
                                wc_t, ws_t, hxi, hyi, hzi, exo, eyo, ezo, hxo, hyo, hzo, cex, cey, cez, dre0, dim0,
                                dre1, dim1, dre2, dim2);
     em = nm;

@@ -503,12 +503,31 @@ int launch_shell_e(dim3 grid, hipStream_t s, const mw::ShellArgs& sa) {
   return check_launch("fused_step (shell E)");
 }
 
-int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_fields) {
+// One single-pass step.  With `zs` (its row set to this step's stage row), the
+// interior tiles store the staged z-plane samples of their cells, and `zs`'s
+// own box is set to those cells (empty when there are no interior tiles) so the
+// gather skips them.
+int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_fields, mw::ZGatherSet* zs = nullptr) {
   int rc;
   if ((rc = launch_shifted_values(p->h_terms, p->n_h_terms, p->e_terms, p->n_e_terms, n, p->dt, s))) return rc;
   mw::FusedArgs a;
   int nlean = 0;
   build_fused_args(p, n, p->xchunk, &a, &nlean);
+  if (zs) {
+    for (int q = 0; q < 3; ++q) zs->own_lo[q] = zs->own_hi[q] = 0;
+    if (nlean > 0 && zs->count > 0) {
+      a.zrow = zs->row;
+      a.zcount = zs->count;
+      for (int q = 0; q < zs->count; ++q) {
+        a.zcomp[q] = zs->comp[q];
+        a.zk[q] = zs->k[q];
+        a.zoff[q] = zs->off[q];
+      }
+      zs->own_lo[0] = a.x_off + a.xchunk * a.lx0; zs->own_hi[0] = a.x_off + a.xchunk * a.lx1;
+      zs->own_lo[1] = a.y_off + mw::kFRows * a.ly0; zs->own_hi[1] = a.y_off + mw::kFRows * a.ly1;
+      zs->own_lo[2] = a.z_off + mw::kFOwnCols * a.lz0; zs->own_hi[2] = a.z_off + mw::kFOwnCols * a.lz1;
+    }
+  }
   mw::ShellArgs sa;
   build_shell_args(a, nlean, p->shell_chunk, &sa);
   const bool ade = p->has_ade && (p->ade.count[0] + p->ade.count[1] + p->ade.count[2]) > 0;
@@ -1158,8 +1177,12 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
       if ((rc = check_launch("aux_line_h"))) return rc;
     }
     mw_fields cur = p->fields;
+    if (zstage.count > 0) {
+      zstage.row = p->plane_stage + (int64_t)(n % p->plane_stage_depth) * row_floats;
+      for (int q = 0; q < 3; ++q) zstage.own_lo[q] = zstage.own_hi[q] = 0;
+    }
     if (p->fused) {
-      if ((rc = launch_fused(p, n, s, &cur))) return rc;
+      if ((rc = launch_fused(p, n, s, &cur, zstage.count > 0 ? &zstage : nullptr))) return rc;
     } else {
       if ((rc = launch_h(g, &p->coeffs, &p->cpml, &p->fields, 7, s))) return rc;
       if ((rc = launch_terms(g, &p->fields, p->h_terms, p->n_h_terms, n, p->dt, s))) return rc;
@@ -1191,7 +1214,6 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
     }
     if (zstage.count > 0) {
       const int row = n % p->plane_stage_depth;
-      zstage.row = p->plane_stage + (int64_t)row * row_floats;
       hipLaunchKernelGGL(mw::gather_zplanes_kernel, dim3(grid_blocks(zstage_cells * zstage.count)), dim3(256), 0, s,
                          zstage, fs, zstage_cells);
       if ((rc = check_launch("gather_zplanes"))) return rc;

@@ -251,6 +251,9 @@ struct ZGatherSet {
   int32_t comp[MW_MAX_PLANES];
   int32_t k[MW_MAX_PLANES];
   int64_t off[MW_MAX_PLANES];
+  // samples in [own_lo, own_hi) (x, y, z node box) were stored by the interior
+  // tiles of the step and are skipped (an empty box: every sample is gathered)
+  int32_t own_lo[3], own_hi[3];
 };
 
 // row[off + i * sy + j] = component(i, j, k) for each staged pair.  Thread t
@@ -265,7 +268,13 @@ __global__ __launch_bounds__(256) void gather_zplanes_kernel(ZGatherSet set, Fie
   const int64_t l = t / set.count;
   const int c = set.comp[q];
   if (l >= (int64_t)fs.shape[c][0] * fs.shape[c][1]) return;
-  set.row[set.off[q] + l] = fs.ptr[c][l * fs.shape[c][2] + set.k[q]];
+  const int kq = set.k[q];
+  if (kq >= set.own_lo[2] && kq < set.own_hi[2]) {
+    const uint32_t lu = (uint32_t)l, rows = (uint32_t)fs.shape[c][1];  // a plane holds < 2^32 samples
+    const int i = (int)(lu / rows), j = (int)(lu % rows);
+    if (i >= set.own_lo[0] && i < set.own_hi[0] && j >= set.own_lo[1] && j < set.own_hi[1]) return;
+  }
+  set.row[set.off[q] + l] = fs.ptr[c][l * fs.shape[c][2] + kq];
 }
 
 struct ZFlushPlane {
