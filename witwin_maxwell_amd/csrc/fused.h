@@ -175,6 +175,17 @@ __device__ __forceinline__ void lean_plane(const FusedArgs& a, int i, int j, int
   float eym_k1 = lane_down(em.ey), exm_k1 = lane_down(em.ex);
   float eyc_k1 = lane_down(ec.ey), exc_k1 = lane_down(ec.ex);
   if (l63) { eym_k1 = em.ey_k1; exm_k1 = em.ex_k1; eyc_k1 = ec.ey_k1; exc_k1 = ec.ex_k1; }
+  // the pole weight codes of row j's E samples, read with the plane's other loads
+  // (a code read after the H stores would wait for them: three more memory round
+  // trips per plane in every wavefront, pole-free ones included)
+  uint32_t code_x = 0u, code_y = 0u, code_z = 0u;
+  if (ADE) {
+    const uint32_t cx = i * sEx + (uint32_t)j * nz + k, cy = i * sEy + (uint32_t)j * nz + k;
+    const uint32_t cz = i * sEz + (uint32_t)j * (nz - 1) + k;
+    if (a.ad.count[0] > 0 && a.ad.weight[0]) code_x = G(a.ad.weight[0])[cx];
+    if (a.ad.count[1] > 0 && a.ad.weight[1]) code_y = G(a.ad.weight[1])[cy];
+    if (a.ad.count[2] > 0 && a.ad.weight[2]) code_z = G(a.ad.weight[2])[cz];
+  }
   // row j-1 (recomputed halo) -- only Hx and Hz feed row j's E
   const uint32_t oxm = i * sHx + (uint32_t)(j - 1) * (nz - 1) + k;
   const uint32_t ozm = i * sHz + (uint32_t)(j - 1) * nz + k;
@@ -210,8 +221,8 @@ __device__ __forceinline__ void lean_plane(const FusedArgs& a, int i, int j, int
     float ey = lean_e<COEF>(a, 1, oy, ec.ey, (hx_c - hx_km) * iez - (hz_c - hz_prev) * iex, cey);
     float ez = lean_e<COEF>(a, 2, oz, ec.ez, (hy_c - hy_prev) * iex - (hx_c - hx_m) * iey, cez);
     if (ADE) {  // dt/eps_inf: the curl coefficient on lossless samples
-      const float jx = ade_advance(a.ad, 0, ox, ec.ex), jy = ade_advance(a.ad, 1, oy, ec.ey);
-      const float jz = ade_advance(a.ad, 2, oz, ec.ez);
+      const float jx = ade_advance_coded(a.ad, 0, ox, ec.ex, code_x), jy = ade_advance_coded(a.ad, 1, oy, ec.ey, code_y);
+      const float jz = ade_advance_coded(a.ad, 2, oz, ec.ez, code_z);
       if (jx != 0.0f) ex -= lean_ade_gain<COEF>(a, 0, ox, cex) * jx;
       if (jy != 0.0f) ey -= lean_ade_gain<COEF>(a, 1, oy, cey) * jy;
       if (jz != 0.0f) ez -= lean_ade_gain<COEF>(a, 2, oz, cez) * jz;

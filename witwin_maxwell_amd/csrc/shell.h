@@ -346,6 +346,13 @@ __device__ __forceinline__ void shell_e_cells(
     float hx_km = hx[(jH && k > 0) ? ohx - 1 : 0];
     float hx_jm = hx[(kH && j > 0) ? ohx - (nz - 1) : 0];
     const float exv = exi[ox], eyv = eyi[oy], ezv = ezi[oz];
+    // pole weight codes with the plane's other loads (read before any store)
+    uint32_t code_x = 0u, code_y = 0u, code_z = 0u;
+    if (ADE) {
+      if (a.ad.count[0] > 0 && a.ad.weight[0]) code_x = G(a.ad.weight[0])[ox];
+      if (a.ad.count[1] > 0 && a.ad.weight[1]) code_y = G(a.ad.weight[1])[oy];
+      if (a.ad.count[2] > 0 && a.ad.weight[2]) code_z = G(a.ad.weight[2])[oz];
+    }
     const float cdx = dx_ ? dx_[ox] : a.cf.decay_value[0], ccx = cx_ ? cx_[ox] : a.cf.curl_value[0];
     const float cdy = dy_ ? dy_[oy] : a.cf.decay_value[1], ccy = cy_ ? cy_[oy] : a.cf.curl_value[1];
     const float cdz = dz_ ? dz_[oz] : a.cf.decay_value[2], ccz = cz_ ? cz_[oz] : a.cf.curl_value[2];
@@ -394,7 +401,7 @@ __device__ __forceinline__ void shell_e_cells(
       wx_y = hx[(wj && kH) ? i * sHx + (uint32_t)jw * (nz - 1) + k : 0];           // Ez: Hx across y
     }
     if (iH) {  // Ex (nx-1, ny, nz): a = dHz/dy, b = dHy/dz, curl = a - b
-      const float jsum = ADE ? ade_advance(a.ad, 0, ox, exv) : 0.0f;
+      const float jsum = ADE ? ade_advance_coded(a.ad, 0, ox, exv, code_x) : 0.0f;
       const bool pec = face_pec(j, ny, ly, hym) || face_pec(k, nz, lz, hzm);
       float e_new = exv;
       if (!pec && !(face_inactive(j, ny, ly, hym) || face_inactive(k, nz, lz, hzm))) {
@@ -417,7 +424,7 @@ __device__ __forceinline__ void shell_e_cells(
       shell_dft<DFT>(a, dre0, dim0, (uint64_t)sEx * (nx - 1), ox, e_new, wc, ws);
     }
     if (jH) {  // Ey (nx, ny-1, nz): a = dHz/dx, b = dHx/dz, curl = b - a
-      const float jsum = ADE ? ade_advance(a.ad, 1, oy, eyv) : 0.0f;
+      const float jsum = ADE ? ade_advance_coded(a.ad, 1, oy, eyv, code_y) : 0.0f;
       const bool pec = face_pec(i, nx, lx, hxm) || face_pec(k, nz, lz, hzm);
       float e_new = eyv;
       if (!pec && !(face_inactive(i, nx, lx, hxm) || face_inactive(k, nz, lz, hzm))) {
@@ -440,7 +447,7 @@ __device__ __forceinline__ void shell_e_cells(
       shell_dft<DFT>(a, dre1, dim1, (uint64_t)sEy * nx, oy, e_new, wc, ws);
     }
     if (kH) {  // Ez (nx, ny, nz-1): a = dHy/dx, b = dHx/dy, curl = a - b
-      const float jsum = ADE ? ade_advance(a.ad, 2, oz, ezv) : 0.0f;
+      const float jsum = ADE ? ade_advance_coded(a.ad, 2, oz, ezv, code_z) : 0.0f;
       const bool pec = face_pec(i, nx, lx, hxm) || face_pec(j, ny, ly, hym);
       float e_new = ezv;
       if (!pec && !(face_inactive(i, nx, lx, hxm) || face_inactive(j, ny, ly, hym))) {

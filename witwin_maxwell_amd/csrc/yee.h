@@ -254,6 +254,22 @@ __device__ __forceinline__ float ade_advance(const mw_ade& a, int c, uint32_t o,
   return jsum;
 }
 
+// ade_advance with the weight code already read by the caller (`code`, used when
+// the component has weight codes; otherwise the drive arrays are read as above).
+__device__ __forceinline__ float ade_advance_coded(const mw_ade& a, int c, uint32_t o, float e_old, uint32_t code) {
+  if (!a.weight[c]) return ade_advance(a, c, o, e_old);
+  if (code == 0u) return 0.0f;
+  float jsum = 0.0f;
+  const int n = a.count[c];
+  for (int p = 0; p < n; ++p) {
+    const uint32_t w = (code >> (2 * p)) & 3u;
+    if (w == 0u) continue;
+    const mw_pole& pl = a.poles[c][p];
+    jsum += pole_step(a, pl, o, w == 2u ? pl.drive_scale : 0.5f * pl.drive_scale, e_old);
+  }
+  return jsum;
+}
+
 // dt / eps_inf of sample o, the factor of the polarisation-current subtraction
 // E -= dt J / eps_inf (kernels/dispersive.cu:72-83): on a sample with E decay 1
 // (no conductivity) the E curl coefficient is exactly dt / eps, so it is used and

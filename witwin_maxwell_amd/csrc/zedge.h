@@ -140,6 +140,14 @@ __device__ __forceinline__ void zedge_march(
     const uint32_t q1 = l1 >= 0 ? slab_offset(2, i, j, k, l1, ny, nz, len1) : 0;
     const uint32_t q3 = l3 >= 0 ? slab_offset(2, i, j, k, l3, ny - 1, nz, len3) : 0;
     const float p1v = p1r[q1], p3v = p3r[q3];
+    // pole weight codes with the plane's other loads (offset 0 where the sample
+    // does not exist; read before any store of the plane)
+    uint32_t code_x = 0u, code_y = 0u, code_z = 0u;
+    if (ADE) {
+      if (a.ad.count[0] > 0 && a.ad.weight[0]) code_x = G(a.ad.weight[0])[ox];
+      if (a.ad.count[1] > 0 && a.ad.weight[1]) code_y = G(a.ad.weight[1])[oy];
+      if (a.ad.count[2] > 0 && a.ad.weight[2]) code_z = G(a.ad.weight[2])[oz];
+    }
     float ccx = 0.0f, ccy = 0.0f, ccz = 0.0f;  // E curl coefficients of mode 1
     if (COEF == 1) { ccx = kcx[ox]; ccy = kcy[oy]; ccz = kcz[oz]; }
     // H, row j-1 (Hx, Hz: what row j's E needs)
@@ -220,12 +228,13 @@ __device__ __forceinline__ void zedge_march(
     if (store) {
       if (ADE) {
         if (kin) {
-          const float jx = ade_advance(a.ad, 0, ox, ec.ex), jy = ade_advance(a.ad, 1, oy, ec.ey);
+          const float jx = ade_advance_coded(a.ad, 0, ox, ec.ex, code_x);
+          const float jy = ade_advance_coded(a.ad, 1, oy, ec.ey, code_y);
           if (jx != 0.0f) ex -= lean_ade_gain<COEF>(a, 0, ox, kcx) * jx;
           if (jy != 0.0f) ey -= lean_ade_gain<COEF>(a, 1, oy, kcy) * jy;
         }
         if (kv) {
-          const float jz = ade_advance(a.ad, 2, oz, ec.ez);
+          const float jz = ade_advance_coded(a.ad, 2, oz, ec.ez, code_z);
           if (jz != 0.0f) ez -= lean_ade_gain<COEF>(a, 2, oz, kcz) * jz;
         }
       }
@@ -388,6 +397,14 @@ __device__ __forceinline__ void zends_march(
     const uint32_t q1 = l1 >= 0 ? slab_offset(2, i, jl, k, l1, ny, nz, len1) : 0;
     const uint32_t q3 = l3 >= 0 ? slab_offset(2, i, jl, k, l3, ny - 1, nz, len3) : 0;
     const float p1v = p1r[q1], p3v = p3r[q3];
+    // pole weight codes with the plane's other loads (offset 0 where the sample
+    // does not exist; read before any store of the plane)
+    uint32_t code_x = 0u, code_y = 0u, code_z = 0u;
+    if (ADE) {
+      if (a.ad.count[0] > 0 && a.ad.weight[0]) code_x = G(a.ad.weight[0])[ox];
+      if (a.ad.count[1] > 0 && a.ad.weight[1]) code_y = G(a.ad.weight[1])[oy];
+      if (a.ad.count[2] > 0 && a.ad.weight[2]) code_z = G(a.ad.weight[2])[oz];
+    }
     float ccx = 0.0f, ccy = 0.0f, ccz = 0.0f;  // E curl coefficients of mode 1
     if (COEF == 1) { ccx = kcx[ox]; ccy = kcy[oy]; ccz = kcz[oz]; }
     float hx_m, hz_m, hx_c, hy_c, hz_c;
@@ -465,12 +482,13 @@ __device__ __forceinline__ void zends_march(
     if (store) {
       if (ADE) {
         if (kin) {
-          const float jx = ade_advance(a.ad, 0, ox, ec.ex), jy = ade_advance(a.ad, 1, oy, ec.ey);
+          const float jx = ade_advance_coded(a.ad, 0, ox, ec.ex, code_x);
+          const float jy = ade_advance_coded(a.ad, 1, oy, ec.ey, code_y);
           if (jx != 0.0f) ex -= lean_ade_gain<COEF>(a, 0, ox, kcx) * jx;
           if (jy != 0.0f) ey -= lean_ade_gain<COEF>(a, 1, oy, kcy) * jy;
         }
         if (kv) {
-          const float jz = ade_advance(a.ad, 2, oz, ec.ez);
+          const float jz = ade_advance_coded(a.ad, 2, oz, ec.ez, code_z);
           if (jz != 0.0f) ez -= lean_ade_gain<COEF>(a, 2, oz, kcz) * jz;
         }
       }
