@@ -714,6 +714,21 @@ int upload(const std::vector<T>& host, T** dst, const char* what) {
 
 // One step of every scene, reading the step index from the device counter
 // (observers off for the adjoint's replay: the forward sums are already taken).
+template <bool CPML, bool ADE, bool DFT>
+void launch_batch_shell_h_lean(const mw_batch* b, hipStream_t s) {
+  const int lb = ((b->total_lean + 7) / 8) * 8, sb = ((b->total_shell + 7) / 8) * 8;
+  const dim3 grid(lb + sb), block(mw::kFLanes, mw::kFRows, 1);
+  if (b->mode == 2)
+    hipLaunchKernelGGL((mw::shell_h_lean_batch_kernel<CPML, ADE, DFT, 2>), grid, block, 0, s, b->d_args, b->d_shell,
+                       b->d_lean_prefix, b->d_shell_prefix, b->count, b->total_lean, b->total_shell, lb, sb, b->d_counter);
+  else if (b->mode == 1)
+    hipLaunchKernelGGL((mw::shell_h_lean_batch_kernel<CPML, ADE, DFT, 1>), grid, block, 0, s, b->d_args, b->d_shell,
+                       b->d_lean_prefix, b->d_shell_prefix, b->count, b->total_lean, b->total_shell, lb, sb, b->d_counter);
+  else
+    hipLaunchKernelGGL((mw::shell_h_lean_batch_kernel<CPML, ADE, DFT, 0>), grid, block, 0, s, b->d_args, b->d_shell,
+                       b->d_lean_prefix, b->d_shell_prefix, b->count, b->total_lean, b->total_shell, lb, sb, b->d_counter);
+}
+
 int batch_step(const mw_batch* b, hipStream_t s, bool observe = true) {
   int rc;
   if (b->n_aux > 0) {
@@ -728,14 +743,25 @@ int batch_step(const mw_batch* b, hipStream_t s, bool observe = true) {
   }
   // shell H and lean read only the step's inputs; shell E reads the complete H
   const dim3 sgrid(((b->total_shell + 7) / 8) * 8);
-  if (b->total_shell > 0) {
+  const bool merged = MW_MERGE_SHELL_H && b->total_shell > 0 && b->total_lean > 0;
+  if (merged) {
+    if (b->cpml) {
+      if (b->ade) { if (b->dft) launch_batch_shell_h_lean<true, true, true>(b, s); else launch_batch_shell_h_lean<true, true, false>(b, s); }
+      else { if (b->dft) launch_batch_shell_h_lean<true, false, true>(b, s); else launch_batch_shell_h_lean<true, false, false>(b, s); }
+    } else {
+      if (b->ade) { if (b->dft) launch_batch_shell_h_lean<false, true, true>(b, s); else launch_batch_shell_h_lean<false, true, false>(b, s); }
+      else { if (b->dft) launch_batch_shell_h_lean<false, false, true>(b, s); else launch_batch_shell_h_lean<false, false, false>(b, s); }
+    }
+    if ((rc = check_launch("batch fused_step (shell H + lean)"))) return rc;
+  }
+  if (b->total_shell > 0 && !merged) {
     if (b->cpml) hipLaunchKernelGGL(mw::shell_h_batch_kernel<true>, sgrid, dim3(64, 4, 1), 0, s, b->d_shell,
                                     b->d_shell_prefix, b->count, b->total_shell, b->d_counter);
     else hipLaunchKernelGGL(mw::shell_h_batch_kernel<false>, sgrid, dim3(64, 4, 1), 0, s, b->d_shell,
                             b->d_shell_prefix, b->count, b->total_shell, b->d_counter);
     if ((rc = check_launch("batch fused_step (shell H)"))) return rc;
   }
-  if (b->total_lean > 0) {
+  if (b->total_lean > 0 && !merged) {
     const dim3 lgrid(((b->total_lean + 7) / 8) * 8);
     const int* lp = b->d_lean_prefix;
     if (b->ade) {

@@ -523,6 +523,35 @@ __global__ __launch_bounds__(256) void shell_h_lean_kernel(ShellArgs sa, int she
                             a.lx0 + logical / (bz * by));
 }
 
+// The merged launch for a batch of scenes: interior tiles of every scene first
+// (lean_prefix), then shell-H blocks (shell_prefix), each range XCD-mapped as the
+// separate batch launches map it; the step index comes from the device counter.
+template <bool CPML, bool ADE, bool DFT, int COEF>
+__global__ __launch_bounds__(256) void shell_h_lean_batch_kernel(
+    const FusedArgs* __restrict__ args, const ShellArgs* __restrict__ shells, const int* __restrict__ lean_prefix,
+    const int* __restrict__ shell_prefix, int nscenes, int lean_total, int shell_total, int lean_blocks,
+    int shell_blocks, const int* __restrict__ counter) {
+  const int w0 = blockIdx.x;
+  const int n = *counter;
+  if (w0 >= lean_blocks) {
+    const int w = w0 - lean_blocks;
+    const int logical = (w % 8) * (shell_blocks / 8) + w / 8;
+    if (logical >= shell_total) return;
+    const int s = table_scene(shell_prefix, nscenes, logical);
+    shell_h_tile<CPML>(shells[2 * s + (n & 1)], logical - shell_prefix[s], n);
+    return;
+  }
+  const int logical = (w0 % 8) * (lean_blocks / 8) + w0 / 8;
+  if (logical >= lean_total) return;
+  const int s = table_scene(lean_prefix, nscenes, logical);
+  const FusedArgs& a = args[2 * s + (n & 1)];
+  const int t = logical - lean_prefix[s];
+  const int bz = a.lz1 - a.lz0, by = a.ly1 - a.ly0;
+  const int64_t wofs = (int64_t)n * a.nf_dft;
+  lean_tile<ADE, DFT, COEF>(a, n, a.dft_cos + wofs, a.dft_sin + wofs, a.lz0 + t % bz, a.ly0 + (t / bz) % by,
+                            a.lx0 + t / (bz * by));
+}
+
 // Batches: args[2 s + p] is scene s's shell for steps of parity p (dft_cos /
 // dft_sin the base of its weight tables), prefix[s] its first block; the step
 // index comes from the device counter.
