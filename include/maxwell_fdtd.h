@@ -258,7 +258,10 @@ typedef struct mw_program {
    * PML slabs with the z ends in the per-cell shell, 1 = z-edge tiles beside the
    * interior tiles, 3 = interior tiles aligned as for 2 and both z ends of two
    * rows in one wavefront (when each end and a halo lane fit 32 lanes and the z
-   * faces are not periodic; otherwise as 2) */
+   * faces are not periodic; otherwise as 2), 4 = as 3 with the z-end launch on a
+   * second stream beside the interior launch (a stream and two events per
+   * concurrent caller, created on first use and kept by the library; the z ends
+   * stay on the caller's stream while it is being captured) */
   int32_t z_tiling;
   const float* obs_e_cos;
   const float* obs_e_sin;

@@ -125,6 +125,8 @@ def case_boxes():
     run("boxes, aligned z tiling", s, kw, 8, (3, 2, 2), splits=(2, 6), stage_depth=3)
     run("boxes, z-edge tiles", s, kw, 8, (0, 4, 1), splits=(3,), stage_depth=4, expect="zedge_step_kernel")
     run("boxes, z-end tiles", s, kw, 8, (3, 2, 3), splits=(5,), stage_depth=3, expect="zends_step_kernel")
+    run("boxes, z-end tiles, second stream", s, kw, 8, (3, 2, 4), splits=(5,), stage_depth=3,
+        expect="zends_step_kernel")
 
 
 def case_staged():

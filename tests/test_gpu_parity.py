@@ -298,13 +298,15 @@ def _z_ends_fit(plan):
     return n > 0 and (z0 + 1) + (nz - z1 + 1) <= 32
 
 
+@pytest.mark.parametrize("z_tiling", [3, 4], ids=["one-stream", "side-stream"])
 @pytest.mark.parametrize("case", ["vacuum_dft", "boxes_soft_source", "dispersive"])
-def test_z_end_tiles(case):
+def test_z_end_tiles(case, z_tiling):
     """z_tiling 3: interior tiles aligned to the z PML, and both z ends (CPML
     layers and faces) of two rows advanced by one wavefront, on grids whose z ends
     fit that layout; homogeneous + full-field DFT, per-sample curl with soft
-    source planes, fused ADE."""
-    print(f"case: z-end tiles, {case}", flush=True)
+    source planes, fused ADE.  z_tiling 4 runs the z-end launch on a second
+    stream beside the interior launch, joined before shell E."""
+    print(f"case: z-end tiles, {case}, z_tiling {z_tiling}", flush=True)
     dx = configs.DX
     if case == "vacuum_dft":
         scene, kw = configs.vacuum_dipole(cells=20, pml=8)
@@ -318,7 +320,7 @@ def test_z_end_tiles(case):
         scene, kw = configs.dispersive_spheres(cells=(24, 26, 68), pml=8, fill=0.3, radius=(3, 7), seed=2,
                                                injection="soft", pulse_delay=3e-15)
         steps = 1400
-    plan, ref, dev = _run_both(scene, kw, steps, True, (4, 2, 3))
+    plan, ref, dev = _run_both(scene, kw, steps, True, (4, 2, z_tiling))
     assert _z_ends_fit(plan), plan.shape
     _check_fields(ref, dev)
     _check_psi(plan, ref, dev)
@@ -350,7 +352,8 @@ def test_tfsf_plane_wave(mode, fused):
     _check_monitors(ref, dev)
 
 
-@pytest.mark.parametrize("tiling", [None, (3, 1, 1), (3, 1, 3)], ids=["aligned", "zedge", "zends"])
+@pytest.mark.parametrize("tiling", [None, (3, 1, 1), (3, 1, 3), (3, 1, 4)], ids=["aligned", "zedge", "zends",
+                                                                            "zends-side-stream"])
 def test_staged_planes_equal_per_step_sums(tiling):
     """z-normal plane monitors staged and folded a block at a time (in calls that
     end inside blocks and at their ends) give the per-step sums to the bit: the

@@ -414,11 +414,11 @@ int build_fused_args(const mw_program* p, int n, int chunk_override, mw::FusedAr
   a.zedge = zedge ? 1 : 0;
   int nlean = (a.lz1 - a.lz0) * (a.ly1 - a.ly0) * (a.lx1 - a.lx0);
   if (nlean <= 0) { nlean = 0; a.lz0 = a.lz1 = a.ly0 = a.ly1 = a.lx0 = a.lx1 = 0; }
-  // z-end tiles (z_tiling 3): both z ends of two rows per wavefront, when each end
-  // plus its halo lane fits half a wavefront; otherwise the z ends stay shell cells
+  // z-end tiles (z_tiling 3 and 4): both z ends of two rows per wavefront, when each
+  // end plus its halo lane fits half a wavefront; otherwise the z ends stay shell cells
   a.zends = 0;
   a.zend_lo = a.zend_hi = 0;
-  if (p->z_tiling == 3 && nlean > 0 && g->bc_lo[2] != MW_BC_PERIODIC) {
+  if ((p->z_tiling == 3 || p->z_tiling == 4) && nlean > 0 && g->bc_lo[2] != MW_BC_PERIODIC) {
     const int z0 = a.z_off + mw::kFOwnCols * a.lz0, z1 = a.z_off + mw::kFOwnCols * a.lz1;
     if (z0 >= 1 && z1 < g->nz && (z0 + 1) + (g->nz - z1 + 1) <= 32) {
       a.zends = 1;
@@ -503,11 +503,81 @@ int launch_shell_e(dim3 grid, hipStream_t s, const mw::ShellArgs& sa) {
   return check_launch("fused_step (shell E)");
 }
 
+// A second stream for the z-end launch (z_tiling 4).  The z-end tiles, like the
+// interior + shell-H launch, read only the step's inputs and write their own
+// cells, so the two launches run side by side; shell E reads the z ends' H and
+// waits for them.  A call takes a stream and its fork/join events from a
+// per-device pool for its whole duration and gives them back at its end, so
+// callers on different host threads never share one and the pool grows only to
+// the number of concurrent callers.  The streams are non-blocking and of default
+// priority; the library keeps them until the process ends.
+struct SideLane {
+  int device;
+  hipStream_t stream;
+  hipEvent_t fork, join;
+};
+
+std::mutex g_side_mu;
+std::vector<SideLane*> g_side_free;
+
+// *out stays NULL (the z ends run on `s`) while `s` is being captured or when
+// `s` belongs to another device than the current one.
+int side_acquire(hipStream_t s, SideLane** out) {
+  *out = nullptr;
+  hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &st) != hipSuccess) {
+    (void)hipGetLastError();
+    return MW_OK;
+  }
+  if (st != hipStreamCaptureStatusNone) return MW_OK;
+  int dev = 0;
+  hipDevice_t sdev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || hipStreamGetDevice(s, &sdev) != hipSuccess) {
+    (void)hipGetLastError();
+    return MW_OK;
+  }
+  if ((int)sdev != dev) return MW_OK;
+  {
+    std::lock_guard<std::mutex> lock(g_side_mu);
+    for (size_t i = 0; i < g_side_free.size(); ++i)
+      if (g_side_free[i]->device == dev) {
+        *out = g_side_free[i];
+        g_side_free.erase(g_side_free.begin() + (std::ptrdiff_t)i);
+        return MW_OK;
+      }
+  }
+  SideLane* lane = new SideLane{dev, nullptr, nullptr, nullptr};
+  if (hipStreamCreateWithFlags(&lane->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&lane->fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&lane->join, hipEventDisableTiming) != hipSuccess) {
+    const hipError_t err = hipGetLastError();
+    if (lane->join) (void)hipEventDestroy(lane->join);
+    if (lane->fork) (void)hipEventDestroy(lane->fork);
+    if (lane->stream) (void)hipStreamDestroy(lane->stream);
+    delete lane;
+    return fail(MW_ERR_RUNTIME, "z-end stream: %s", hipGetErrorString(err));
+  }
+  *out = lane;
+  return MW_OK;
+}
+
+struct SideGuard {
+  SideLane* lane = nullptr;
+  ~SideGuard() {
+    if (!lane) return;
+    std::lock_guard<std::mutex> lock(g_side_mu);
+    g_side_free.push_back(lane);
+  }
+};
+
 // One single-pass step.  With `zs` (its row set to this step's stage row), the
 // interior tiles store the staged z-plane samples of their cells, and `zs`'s
 // own box is set to those cells (empty when there are no interior tiles) so the
 // gather skips them.
-int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_fields, mw::ZGatherSet* zs = nullptr) {
+// With `side`, the z-end launch runs on its stream between a fork after the term
+// values and a join before shell E.
+int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_fields, mw::ZGatherSet* zs = nullptr,
+                 SideLane* side = nullptr) {
   int rc;
   if ((rc = launch_shifted_values(p->h_terms, p->n_h_terms, p->e_terms, p->n_e_terms, n, p->dt, s))) return rc;
   mw::FusedArgs a;
@@ -535,6 +605,26 @@ int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_field
   const int mode = coef_mode(p->coeffs);
   const dim3 block(mw::kFLanes, mw::kFRows, 1);
   const dim3 sgrid(((sa.total + 7) / 8) * 8);
+  const int nze = mw::zedge_tiles(a);
+  auto launch_zends = [&](hipStream_t zs_stream) {
+    const dim3 zgrid(((nze + 7) / 8) * 8);
+    if (ade) { if (dft) dispatch_zedge<true, true>(zgrid, block, zs_stream, a, mode); else dispatch_zedge<true, false>(zgrid, block, zs_stream, a, mode); }
+    else { if (dft) dispatch_zedge<false, true>(zgrid, block, zs_stream, a, mode); else dispatch_zedge<false, false>(zgrid, block, zs_stream, a, mode); }
+    return check_launch("fused_step (z edges)");
+  };
+  // the z ends on the side stream: they start with the launch below, and the join
+  // is recorded right after them so that every later return path can wait for it
+  const bool forked = side && a.zends && nze > 0;
+  if (forked) {
+    if (hipEventRecord(side->fork, s) != hipSuccess || hipStreamWaitEvent(side->stream, side->fork, 0) != hipSuccess)
+      return fail(MW_ERR_RUNTIME, "z-end stream: fork failed");
+    const int zrc = launch_zends(side->stream);
+    if (hipEventRecord(side->join, side->stream) != hipSuccess) return fail(MW_ERR_RUNTIME, "z-end stream: join failed");
+    if (zrc) {
+      (void)hipStreamWaitEvent(s, side->join, 0);
+      return zrc;
+    }
+  }
   // shell H and lean read only this step's inputs and write disjoint cells: one
   // launch when both have work
   const bool merged = MW_MERGE_SHELL_H && sa.total > 0 && nlean > 0;
@@ -547,25 +637,33 @@ int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_field
       if (ade) { if (dft) dispatch_shell_h_lean<false, true, true>(sb, lb, s, sa, mode); else dispatch_shell_h_lean<false, true, false>(sb, lb, s, sa, mode); }
       else { if (dft) dispatch_shell_h_lean<false, false, true>(sb, lb, s, sa, mode); else dispatch_shell_h_lean<false, false, false>(sb, lb, s, sa, mode); }
     }
-    if ((rc = check_launch("fused_step (shell H + lean)"))) return rc;
+    if ((rc = check_launch("fused_step (shell H + lean)"))) {
+      if (forked) (void)hipStreamWaitEvent(s, side->join, 0);
+      return rc;
+    }
   }
   if (sa.total > 0 && !merged) {
     if (cpml) hipLaunchKernelGGL(mw::shell_h_kernel<true>, sgrid, dim3(64, 4, 1), 0, s, sa);
     else hipLaunchKernelGGL(mw::shell_h_kernel<false>, sgrid, dim3(64, 4, 1), 0, s, sa);
-    if ((rc = check_launch("fused_step (shell H)"))) return rc;
+    if ((rc = check_launch("fused_step (shell H)"))) {
+      if (forked) (void)hipStreamWaitEvent(s, side->join, 0);
+      return rc;
+    }
   }
   if (nlean > 0 && !merged) {
     dim3 lgrid(((nlean + 7) / 8) * 8);
     if (ade) { if (dft) dispatch_lean<true, true>(lgrid, block, s, a, mode); else dispatch_lean<true, false>(lgrid, block, s, a, mode); }
     else { if (dft) dispatch_lean<false, true>(lgrid, block, s, a, mode); else dispatch_lean<false, false>(lgrid, block, s, a, mode); }
-    if ((rc = check_launch("fused_step (lean)"))) return rc;
+    if ((rc = check_launch("fused_step (lean)"))) {
+      if (forked) (void)hipStreamWaitEvent(s, side->join, 0);
+      return rc;
+    }
   }
-  const int nze = mw::zedge_tiles(a);
-  if (nze > 0) {
-    const dim3 zgrid(((nze + 7) / 8) * 8);
-    if (ade) { if (dft) dispatch_zedge<true, true>(zgrid, block, s, a, mode); else dispatch_zedge<true, false>(zgrid, block, s, a, mode); }
-    else { if (dft) dispatch_zedge<false, true>(zgrid, block, s, a, mode); else dispatch_zedge<false, false>(zgrid, block, s, a, mode); }
-    if ((rc = check_launch("fused_step (z edges)"))) return rc;
+  if (forked) {
+    // shell E reads the z ends' H^{n+1/2}
+    if (hipStreamWaitEvent(s, side->join, 0) != hipSuccess) return fail(MW_ERR_RUNTIME, "z-end stream: join failed");
+  } else if (nze > 0) {
+    if ((rc = launch_zends(s))) return rc;
   }
   // shell E reads the complete H^{n+1/2}
   if (sa.total > 0) {
@@ -880,9 +978,9 @@ int collect_batch(const mw_program* const* programs, int count, int xchunk, Sink
     for (int par = 0; par < 2; ++par) {
       mw::FusedArgs a;
       int nlean = 0;
-      // the batch launches have no z-end tiles: z_tiling 3 runs as 2 there
+      // the batch launches have no z-end tiles: z_tiling 3 and 4 run as 2 there
       mw_program q = *p;
-      if (q.z_tiling == 3) q.z_tiling = 2;
+      if (q.z_tiling >= 3) q.z_tiling = 2;
       build_fused_args(&q, par, xchunk > 0 ? xchunk : p->xchunk, &a, &nlean);
       a.dft_cos = p->dft_cos;  // the batch kernels add step * nf
       a.dft_sin = p->dft_sin;
@@ -1118,7 +1216,7 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
     return fail(MW_ERR_ARGUMENT, "steps [%d, %d) outside the program's %d steps", first, first + count, p->total_steps);
   if (p->xchunk < 0 || p->xchunk > 256 || p->shell_chunk < 0 || p->shell_chunk > 256)
     return fail(MW_ERR_ARGUMENT, "xchunk %d / shell_chunk %d outside [0, 256]", p->xchunk, p->shell_chunk);
-  if (p->z_tiling < 0 || p->z_tiling > 3) return fail(MW_ERR_ARGUMENT, "z_tiling %d outside [0, 3]", p->z_tiling);
+  if (p->z_tiling < 0 || p->z_tiling > 4) return fail(MW_ERR_ARGUMENT, "z_tiling %d outside [0, 4]", p->z_tiling);
   if ((rc = check_terms(g, p->h_terms, p->n_h_terms, "h_terms", p->total_steps)) ||
       (rc = check_terms(g, p->e_terms, p->n_e_terms, "e_terms", p->total_steps)))
     return rc;
@@ -1197,6 +1295,8 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
     return fail(MW_ERR_ARGUMENT, "auxiliary line arrays missing");
   hipStream_t s = (hipStream_t)stream;
   const mw_ade* ade = p->has_ade ? &p->ade : nullptr;
+  SideGuard side;
+  if (p->fused && p->z_tiling == 4 && count > 0 && (rc = side_acquire(s, &side.lane))) return rc;
   for (int n = first; n < first + count; ++n) {
     if (aux.n_e > 0) {
       hipLaunchKernelGGL(mw::aux_line_h_kernel, dim3((aux.n_e + 254) / 256), dim3(256), 0, s, aux);
@@ -1208,7 +1308,7 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
       for (int q = 0; q < 3; ++q) zstage.own_lo[q] = zstage.own_hi[q] = 0;
     }
     if (p->fused) {
-      if ((rc = launch_fused(p, n, s, &cur, zstage.count > 0 ? &zstage : nullptr))) return rc;
+      if ((rc = launch_fused(p, n, s, &cur, zstage.count > 0 ? &zstage : nullptr, side.lane))) return rc;
     } else {
       if ((rc = launch_h(g, &p->coeffs, &p->cpml, &p->fields, 7, s))) return rc;
       if ((rc = launch_terms(g, &p->fields, p->h_terms, p->n_h_terms, n, p->dt, s))) return rc;

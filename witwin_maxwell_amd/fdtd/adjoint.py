@@ -248,7 +248,7 @@ class GradientSession:
         self.run = DeviceRun(plan, device, fused=fused)
         if tune and self.run.fused and plan.time_steps >= 200:
             # a few trial steps of each candidate tiling (the run is reset afterwards)
-            self.run.autotune(trial_steps=8, xchunks=(0, 6, 8), shell_chunks=(2, 4), z_tilings=(1, 2, 3))
+            self.run.autotune(trial_steps=8, xchunks=(0, 6, 8), shell_chunks=(2, 4), z_tilings=(1, 2, 3, 4))
         self.window = window if window is not None else design_window(plan)
         self.stride = int(stride or default_stride(plan.time_steps))
         self.checkpoints = {}

@@ -352,13 +352,14 @@ class DeviceRun:
         launches (0 = the library's choice) and how the z ends are tiled (0 or 2:
         interior tiles aligned to the z PML slabs, z ends in the per-cell shell;
         1: z-edge tiles beside the interior tiles; 3: aligned interior tiles and
-        both z ends of two rows per wavefront, where they fit, else as 2)."""
+        both z ends of two rows per wavefront, where they fit, else as 2; 4: as 3
+        with the z-end launch on a second stream beside the interior launch)."""
         self.program.xchunk = int(xchunk)
         self.program.shell_chunk = int(shell_chunk)
         self.program.z_tiling = int(z_tiling)
         return self
 
-    def autotune(self, trial_steps: int = 12, xchunks=(0, 3, 4, 6, 8), shell_chunks=(2, 4), z_tilings=(1, 2, 3),
+    def autotune(self, trial_steps: int = 12, xchunks=(0, 3, 4, 6, 8), shell_chunks=(2, 4), z_tilings=(1, 2, 3, 4),
                  cache=True, prefer=(), margin=0.01):
         """Pick the single-pass tiling by timing ``trial_steps`` steps of each
         candidate on this run, then reset the run to step 0.  Among candidates
