@@ -568,9 +568,12 @@ def _traffic_for(tiling, path, kernel):
     it was measured at the tiling this run uses.  z_tiling 4 launches the kernels
     of z_tiling 3 (the z-end launch on a second stream) and moves the same bytes
     per launch."""
-    if tiling is not None and len(tiling) == 3 and int(tiling[2]) == 4:
-        tiling = (tiling[0], tiling[1], 3)
-    entry = _traffic_table(path).get(tuple(tiling)) if tiling is not None else None
+    if tiling is None:
+        return None
+    table = _traffic_table(path)
+    entry = table.get(tuple(tiling))
+    if entry is None and len(tiling) == 3 and int(tiling[2]) == 4:
+        entry = table.get((tiling[0], tiling[1], 3))
     return None if entry is None else entry.get(kernel)
 
 
