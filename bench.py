@@ -515,10 +515,46 @@ def traffic_summary(fetch_dir, write_dir):
     return out
 
 
+def step_busy(directory):
+    """(steps, union, sum): the single-pass step kernels of a kernel trace, steps
+    counted by the shell-E dispatches; ``union`` is the time in microseconds per
+    step during which at least one of them runs (launches that overlap on two
+    streams count once), ``sum`` the sum of their durations per step."""
+    spans, steps = [], 0
+    for path in _find(directory, "kernel_trace.csv"):
+        with open(path) as fh:
+            for r in csv.DictReader(fh):
+                name = r.get("Kernel_Name", "")
+                if "_batch_kernel" in name or not any(f in name for f in _STEP_FAMILIES):
+                    continue
+                steps += "shell_e_kernel" in name
+                spans.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"])))
+    if not steps:
+        return None
+    spans.sort()
+    union, total, end = 0, 0, None
+    for a, b in spans:
+        total += b - a
+        if end is None or a >= end:
+            union += b - a
+            end = b
+        elif b > end:
+            union += b - end
+            end = b
+    return steps, union / steps / 1e3, total / steps / 1e3
+
+
 def summarize(argv):
     if argv[0] == "stats":
         for k, v in sorted(kernel_stats(argv[1]).items(), key=lambda kv: -kv[1]["total_ms"]):
             print(f"{v['calls']:8d} calls  avg {v['avg_us']:10.2f} us  total {v['total_ms']:10.2f} ms  {k}")
+        try:
+            busy = step_busy(argv[1])
+        except FileNotFoundError:
+            busy = None
+        if busy:
+            print(f"step kernels over {busy[0]} steps: {busy[1]:.2f} us per step with at least one running "
+                  f"(sum of their durations {busy[2]:.2f} us)")
     elif argv[0] == "traffic":
         summary = traffic_summary(argv[1], argv[2])
         if len(argv) > 3:
