@@ -239,11 +239,12 @@ void dispatch_lean(dim3 grid, dim3 block, hipStream_t s, const mw::FusedArgs& a,
 #endif
 
 template <bool CPML, bool ADE, bool DFT>
-void dispatch_shell_h_lean(int sb, int lb, hipStream_t s, const mw::ShellArgs& sa, int mode) {
-  const dim3 grid(sb + lb), block(mw::kFLanes, mw::kFRows, 1);
-  if (mode == 2) hipLaunchKernelGGL((mw::shell_h_lean_kernel<CPML, ADE, DFT, 2>), grid, block, 0, s, sa, sb, lb);
-  else if (mode == 1) hipLaunchKernelGGL((mw::shell_h_lean_kernel<CPML, ADE, DFT, 1>), grid, block, 0, s, sa, sb, lb);
-  else hipLaunchKernelGGL((mw::shell_h_lean_kernel<CPML, ADE, DFT, 0>), grid, block, 0, s, sa, sb, lb);
+void dispatch_shell_h_lean(int sb, int lb, hipStream_t s, const mw::ShellArgs& sa, int mode, const mw::PointSet& pts) {
+  const int pb = pts.count > 0 ? (pts.count + 255) / 256 : 0;
+  const dim3 grid(sb + lb + pb), block(mw::kFLanes, mw::kFRows, 1);
+  if (mode == 2) hipLaunchKernelGGL((mw::shell_h_lean_kernel<CPML, ADE, DFT, 2>), grid, block, 0, s, sa, sb, lb, pts);
+  else if (mode == 1) hipLaunchKernelGGL((mw::shell_h_lean_kernel<CPML, ADE, DFT, 1>), grid, block, 0, s, sa, sb, lb, pts);
+  else hipLaunchKernelGGL((mw::shell_h_lean_kernel<CPML, ADE, DFT, 0>), grid, block, 0, s, sa, sb, lb, pts);
 }
 
 template <bool ADE, bool DFT>
@@ -575,9 +576,12 @@ struct SideGuard {
 // own box is set to those cells (empty when there are no interior tiles) so the
 // gather skips them.
 // With `side`, the z-end launch runs on its stream between a fork after the term
-// values and a join before shell E.
+// values and a join before shell E.  With `pts` (the previous step's point
+// observers), the merged interior + shell-H launch takes them and `*pts_taken` is
+// set; otherwise the caller launches them.
 int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_fields, mw::ZGatherSet* zs = nullptr,
-                 SideLane* side = nullptr) {
+                 SideLane* side = nullptr, const mw::PointSet* pts = nullptr, bool* pts_taken = nullptr) {
+  if (pts_taken) *pts_taken = false;
   int rc;
   if ((rc = launch_shifted_values(p->h_terms, p->n_h_terms, p->e_terms, p->n_e_terms, n, p->dt, s))) return rc;
   mw::FusedArgs a;
@@ -630,13 +634,16 @@ int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_field
   const bool merged = MW_MERGE_SHELL_H && sa.total > 0 && nlean > 0;
   if (merged) {
     const int sb = ((sa.total + 7) / 8) * 8, lb = ((nlean + 7) / 8) * 8;
+    mw::PointSet none{};
+    const mw::PointSet& ps = pts ? *pts : none;
     if (cpml) {
-      if (ade) { if (dft) dispatch_shell_h_lean<true, true, true>(sb, lb, s, sa, mode); else dispatch_shell_h_lean<true, true, false>(sb, lb, s, sa, mode); }
-      else { if (dft) dispatch_shell_h_lean<true, false, true>(sb, lb, s, sa, mode); else dispatch_shell_h_lean<true, false, false>(sb, lb, s, sa, mode); }
+      if (ade) { if (dft) dispatch_shell_h_lean<true, true, true>(sb, lb, s, sa, mode, ps); else dispatch_shell_h_lean<true, true, false>(sb, lb, s, sa, mode, ps); }
+      else { if (dft) dispatch_shell_h_lean<true, false, true>(sb, lb, s, sa, mode, ps); else dispatch_shell_h_lean<true, false, false>(sb, lb, s, sa, mode, ps); }
     } else {
-      if (ade) { if (dft) dispatch_shell_h_lean<false, true, true>(sb, lb, s, sa, mode); else dispatch_shell_h_lean<false, true, false>(sb, lb, s, sa, mode); }
-      else { if (dft) dispatch_shell_h_lean<false, false, true>(sb, lb, s, sa, mode); else dispatch_shell_h_lean<false, false, false>(sb, lb, s, sa, mode); }
+      if (ade) { if (dft) dispatch_shell_h_lean<false, true, true>(sb, lb, s, sa, mode, ps); else dispatch_shell_h_lean<false, true, false>(sb, lb, s, sa, mode, ps); }
+      else { if (dft) dispatch_shell_h_lean<false, false, true>(sb, lb, s, sa, mode, ps); else dispatch_shell_h_lean<false, false, false>(sb, lb, s, sa, mode, ps); }
     }
+    if (pts_taken) *pts_taken = pts != nullptr;
     if ((rc = check_launch("fused_step (shell H + lean)"))) {
       if (forked) (void)hipStreamWaitEvent(s, side->join, 0);
       return rc;
@@ -1297,6 +1304,31 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
   const mw_ade* ade = p->has_ade ? &p->ade : nullptr;
   SideGuard side;
   if (p->fused && p->z_tiling == 4 && count > 0 && (rc = side_acquire(s, &side.lane))) return rc;
+  // the point observers of step n ride in step n+1's merged interior + shell-H
+  // launch when there is one (that step reads the fields step n wrote and writes
+  // the other buffers); the rest, and the call's last step, get a launch of their own
+  mw::PointSet pts{};
+  bool pts_pending = false;
+  auto point_set = [&](const mw::FieldSet& fs, int n) {
+    mw::PointSet ps{};
+    for (int c = 0; c < 6; ++c) ps.ptr[c] = fs.ptr[c];
+    const int64_t off = (int64_t)n * p->nf_obs;
+    ps.comp = p->point_component;
+    ps.lin = p->point_linear;
+    ps.e_cos = p->obs_e_cos + off;
+    ps.e_sin = p->obs_e_sin + off;
+    ps.h_cos = p->obs_h_cos + off;
+    ps.h_sin = p->obs_h_sin + off;
+    ps.re = p->point_re;
+    ps.im = p->point_im;
+    ps.count = p->n_points;
+    ps.nf = p->nf_obs;
+    return ps;
+  };
+  auto launch_points = [&](const mw::PointSet& ps) {
+    hipLaunchKernelGGL(mw::accumulate_points_kernel, dim3((ps.count + 63) / 64), dim3(64), 0, s, ps);
+    return check_launch("accumulate_points");
+  };
   for (int n = first; n < first + count; ++n) {
     if (aux.n_e > 0) {
       hipLaunchKernelGGL(mw::aux_line_h_kernel, dim3((aux.n_e + 254) / 256), dim3(256), 0, s, aux);
@@ -1308,7 +1340,12 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
       for (int q = 0; q < 3; ++q) zstage.own_lo[q] = zstage.own_hi[q] = 0;
     }
     if (p->fused) {
-      if ((rc = launch_fused(p, n, s, &cur, zstage.count > 0 ? &zstage : nullptr, side.lane))) return rc;
+      bool taken = false;
+      if ((rc = launch_fused(p, n, s, &cur, zstage.count > 0 ? &zstage : nullptr, side.lane,
+                             pts_pending ? &pts : nullptr, &taken)))
+        return rc;
+      if (pts_pending && !taken && (rc = launch_points(pts))) return rc;
+      pts_pending = false;
     } else {
       if ((rc = launch_h(g, &p->coeffs, &p->cpml, &p->fields, 7, s))) return rc;
       if ((rc = launch_terms(g, &p->fields, p->h_terms, p->n_h_terms, n, p->dt, s))) return rc;
@@ -1361,13 +1398,12 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
       }
     }
     if (p->n_points > 0) {
-      const int64_t off = (int64_t)n * p->nf_obs;
-      hipLaunchKernelGGL(mw::accumulate_points_kernel, dim3((p->n_points + 63) / 64), dim3(64), 0, s, fs, p->n_points,
-                         p->point_component, p->point_linear, p->nf_obs, p->obs_e_cos + off, p->obs_e_sin + off,
-                         p->obs_h_cos + off, p->obs_h_sin + off, p->point_re, p->point_im);
-      if ((rc = check_launch("accumulate_points"))) return rc;
+      pts = point_set(fs, n);
+      if (p->fused) pts_pending = true;
+      else if ((rc = launch_points(pts))) return rc;
     }
   }
+  if (pts_pending && (rc = launch_points(pts))) return rc;
   return MW_OK;
 }
 

@@ -334,20 +334,37 @@ __global__ __launch_bounds__(256) void clear_floats_kernel(float* __restrict__ p
   if (i < n) p[i] = 0.0f;
 }
 
-__global__ __launch_bounds__(64) void accumulate_points_kernel(FieldSet fs, int count, const int32_t* comp,
-                                                               const int32_t* lin, int nf, const float* e_cos,
-                                                               const float* e_sin, const float* h_cos,
-                                                               const float* h_sin, float* re, float* im) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= count) return;
-  const int c = comp[p];
-  const float x = fs.ptr[c][lin[p]];
-  const float* wc = c < 3 ? e_cos : h_cos;
-  const float* ws = c < 3 ? e_sin : h_sin;
-  for (int f = 0; f < nf; ++f) {
-    re[f * count + p] += x * wc[f];
-    im[f * count + p] += x * ws[f];
+// The point observers of one step: the fields that step wrote and the step's
+// weight rows.  Point p adds x * w[f] to its sums (one fused multiply-add each,
+// wherever the sample is taken).
+struct PointSet {
+  const float* ptr[6];
+  const int32_t* comp;
+  const int32_t* lin;
+  const float* e_cos;
+  const float* e_sin;
+  const float* h_cos;
+  const float* h_sin;
+  float* re;
+  float* im;
+  int32_t count, nf;
+};
+
+__device__ __forceinline__ void accumulate_point(const PointSet& ps, int p) {
+  const int c = ps.comp[p];
+  const float x = ps.ptr[c][ps.lin[p]];
+  const float* wc = c < 3 ? ps.e_cos : ps.h_cos;
+  const float* ws = c < 3 ? ps.e_sin : ps.h_sin;
+  for (int f = 0; f < ps.nf; ++f) {
+    const int o = f * ps.count + p;
+    ps.re[o] = fmaf(x, wc[f], ps.re[o]);
+    ps.im[o] = fmaf(x, ws[f], ps.im[o]);
   }
+}
+
+__global__ __launch_bounds__(64) void accumulate_points_kernel(PointSet ps) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < ps.count) accumulate_point(ps, p);
 }
 
 __global__ __launch_bounds__(64) void accumulate_points_single_kernel(const float* field, const int32_t* lin,
