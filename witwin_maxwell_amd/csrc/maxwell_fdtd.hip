@@ -239,12 +239,13 @@ void dispatch_lean(dim3 grid, dim3 block, hipStream_t s, const mw::FusedArgs& a,
 #endif
 
 template <bool CPML, bool ADE, bool DFT>
-void dispatch_shell_h_lean(int sb, int lb, hipStream_t s, const mw::ShellArgs& sa, int mode, const mw::PointSet& pts) {
+void dispatch_shell_h_lean(int sb, int lb, hipStream_t s, const mw::ShellArgs& sa, int mode, const mw::PointSet& pts,
+                           const mw::ZGatherJob& zg) {
   const int pb = pts.count > 0 ? (pts.count + 255) / 256 : 0;
-  const dim3 grid(sb + lb + pb), block(mw::kFLanes, mw::kFRows, 1);
-  if (mode == 2) hipLaunchKernelGGL((mw::shell_h_lean_kernel<CPML, ADE, DFT, 2>), grid, block, 0, s, sa, sb, lb, pts);
-  else if (mode == 1) hipLaunchKernelGGL((mw::shell_h_lean_kernel<CPML, ADE, DFT, 1>), grid, block, 0, s, sa, sb, lb, pts);
-  else hipLaunchKernelGGL((mw::shell_h_lean_kernel<CPML, ADE, DFT, 0>), grid, block, 0, s, sa, sb, lb, pts);
+  const dim3 grid(sb + lb + zg.blocks + pb), block(mw::kFLanes, mw::kFRows, 1);
+  if (mode == 2) hipLaunchKernelGGL((mw::shell_h_lean_kernel<CPML, ADE, DFT, 2>), grid, block, 0, s, sa, sb, lb, pts, zg);
+  else if (mode == 1) hipLaunchKernelGGL((mw::shell_h_lean_kernel<CPML, ADE, DFT, 1>), grid, block, 0, s, sa, sb, lb, pts, zg);
+  else hipLaunchKernelGGL((mw::shell_h_lean_kernel<CPML, ADE, DFT, 0>), grid, block, 0, s, sa, sb, lb, pts, zg);
 }
 
 template <bool ADE, bool DFT>
@@ -577,11 +578,13 @@ struct SideGuard {
 // gather skips them.
 // With `side`, the z-end launch runs on its stream between a fork after the term
 // values and a join before shell E.  With `pts` (the previous step's point
-// observers), the merged interior + shell-H launch takes them and `*pts_taken` is
-// set; otherwise the caller launches them.
+// observers) and `zg` (the previous step's z-plane gather), the merged interior +
+// shell-H launch takes them and `*obs_taken` is set; otherwise the caller
+// launches them.
 int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_fields, mw::ZGatherSet* zs = nullptr,
-                 SideLane* side = nullptr, const mw::PointSet* pts = nullptr, bool* pts_taken = nullptr) {
-  if (pts_taken) *pts_taken = false;
+                 SideLane* side = nullptr, const mw::PointSet* pts = nullptr, const mw::ZGatherJob* zg = nullptr,
+                 bool* obs_taken = nullptr) {
+  if (obs_taken) *obs_taken = false;
   int rc;
   if ((rc = launch_shifted_values(p->h_terms, p->n_h_terms, p->e_terms, p->n_e_terms, n, p->dt, s))) return rc;
   mw::FusedArgs a;
@@ -636,14 +639,16 @@ int launch_fused(const mw_program* p, int n, hipStream_t s, mw_fields* out_field
     const int sb = ((sa.total + 7) / 8) * 8, lb = ((nlean + 7) / 8) * 8;
     mw::PointSet none{};
     const mw::PointSet& ps = pts ? *pts : none;
+    mw::ZGatherJob no_gather{};
+    const mw::ZGatherJob& zj = zg ? *zg : no_gather;
     if (cpml) {
-      if (ade) { if (dft) dispatch_shell_h_lean<true, true, true>(sb, lb, s, sa, mode, ps); else dispatch_shell_h_lean<true, true, false>(sb, lb, s, sa, mode, ps); }
-      else { if (dft) dispatch_shell_h_lean<true, false, true>(sb, lb, s, sa, mode, ps); else dispatch_shell_h_lean<true, false, false>(sb, lb, s, sa, mode, ps); }
+      if (ade) { if (dft) dispatch_shell_h_lean<true, true, true>(sb, lb, s, sa, mode, ps, zj); else dispatch_shell_h_lean<true, true, false>(sb, lb, s, sa, mode, ps, zj); }
+      else { if (dft) dispatch_shell_h_lean<true, false, true>(sb, lb, s, sa, mode, ps, zj); else dispatch_shell_h_lean<true, false, false>(sb, lb, s, sa, mode, ps, zj); }
     } else {
-      if (ade) { if (dft) dispatch_shell_h_lean<false, true, true>(sb, lb, s, sa, mode, ps); else dispatch_shell_h_lean<false, true, false>(sb, lb, s, sa, mode, ps); }
-      else { if (dft) dispatch_shell_h_lean<false, false, true>(sb, lb, s, sa, mode, ps); else dispatch_shell_h_lean<false, false, false>(sb, lb, s, sa, mode, ps); }
+      if (ade) { if (dft) dispatch_shell_h_lean<false, true, true>(sb, lb, s, sa, mode, ps, zj); else dispatch_shell_h_lean<false, true, false>(sb, lb, s, sa, mode, ps, zj); }
+      else { if (dft) dispatch_shell_h_lean<false, false, true>(sb, lb, s, sa, mode, ps, zj); else dispatch_shell_h_lean<false, false, false>(sb, lb, s, sa, mode, ps, zj); }
     }
-    if (pts_taken) *pts_taken = pts != nullptr;
+    if (obs_taken) *obs_taken = true;
     if ((rc = check_launch("fused_step (shell H + lean)"))) {
       if (forked) (void)hipStreamWaitEvent(s, side->join, 0);
       return rc;
@@ -1329,6 +1334,15 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
     hipLaunchKernelGGL(mw::accumulate_points_kernel, dim3((ps.count + 63) / 64), dim3(64), 0, s, ps);
     return check_launch("accumulate_points");
   };
+  // likewise the z-plane gather of a step inside a block of staged steps (its
+  // stage row is not the next step's); a block's last step and the call's last
+  // step gather in a launch of their own, ahead of the fold
+  mw::ZGatherJob zjob{};
+  bool zjob_pending = false;
+  auto launch_gather = [&](const mw::ZGatherJob& j) {
+    hipLaunchKernelGGL(mw::gather_zplanes_kernel, dim3(j.blocks), dim3(256), 0, s, j.set, j.fs, j.samples);
+    return check_launch("gather_zplanes");
+  };
   for (int n = first; n < first + count; ++n) {
     if (aux.n_e > 0) {
       hipLaunchKernelGGL(mw::aux_line_h_kernel, dim3((aux.n_e + 254) / 256), dim3(256), 0, s, aux);
@@ -1342,10 +1356,13 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
     if (p->fused) {
       bool taken = false;
       if ((rc = launch_fused(p, n, s, &cur, zstage.count > 0 ? &zstage : nullptr, side.lane,
-                             pts_pending ? &pts : nullptr, &taken)))
+                             pts_pending ? &pts : nullptr, zjob_pending ? &zjob : nullptr, &taken)))
         return rc;
-      if (pts_pending && !taken && (rc = launch_points(pts))) return rc;
-      pts_pending = false;
+      if (!taken) {
+        if (zjob_pending && (rc = launch_gather(zjob))) return rc;
+        if (pts_pending && (rc = launch_points(pts))) return rc;
+      }
+      pts_pending = zjob_pending = false;
     } else {
       if ((rc = launch_h(g, &p->coeffs, &p->cpml, &p->fields, 7, s))) return rc;
       if ((rc = launch_terms(g, &p->fields, p->h_terms, p->n_h_terms, n, p->dt, s))) return rc;
@@ -1377,12 +1394,16 @@ int mw_run_steps(const mw_program* p, int32_t first, int32_t count, void* stream
     }
     if (zstage.count > 0) {
       const int row = n % p->plane_stage_depth;
-      hipLaunchKernelGGL(mw::gather_zplanes_kernel, dim3(grid_blocks(zstage_cells * zstage.count)), dim3(256), 0, s,
-                         zstage, fs, zstage_cells);
-      if ((rc = check_launch("gather_zplanes"))) return rc;
+      const bool block_end = row == p->plane_stage_depth - 1, call_end = n == first + count - 1;
+      const mw::ZGatherJob job{zstage, fs, zstage_cells, grid_blocks(zstage_cells * zstage.count), 0};
+      if (p->fused && !block_end && !call_end) {
+        zjob = job;
+        zjob_pending = true;
+      } else if ((rc = launch_gather(job))) {
+        return rc;
+      }
       // fold the block's rows when it completes or the call ends inside it (then
       // clear them: rows of steps already folded must add nothing at the block's end)
-      const bool block_end = row == p->plane_stage_depth - 1, call_end = n == first + count - 1;
       if (block_end || call_end) {
         const int64_t off = (int64_t)(n - row) * p->nf_obs;
         hipLaunchKernelGGL(mw::flush_zstage_kernel, dim3(grid_blocks(zstage_cells), zflush.count), dim3(256), 0, s,

@@ -507,15 +507,23 @@ __global__ __launch_bounds__(256) void shell_e_kernel(ShellArgs sa) {
 // interior tiles' tail instead of a launch of their own (dispatching them first
 // measured slower: C2 544.4 vs 542.3 us, design region 216.9 vs 209.6 us).
 //
-// With `pts.count` > 0 the launch ends with blocks that take the previous step's
-// point observers: the fields that step wrote are this step's inputs, which no
-// block of the launch writes.
+// The launch can end with blocks that take the previous step's observers: first
+// its z-plane gather (`zg.blocks` blocks; its stage row is not the one this
+// step's interior tiles write), then its point observers (`pts.count` > 0).  The
+// fields that step wrote are this step's inputs, which no block of the launch
+// writes.
 template <bool CPML, bool ADE, bool DFT, int COEF>
 __global__ __launch_bounds__(256) void shell_h_lean_kernel(ShellArgs sa, int shell_blocks, int lean_blocks,
-                                                           PointSet pts) {
+                                                           PointSet pts, ZGatherJob zg) {
   const int w0 = blockIdx.x;
   if (w0 >= lean_blocks + shell_blocks) {
-    const int p = (w0 - lean_blocks - shell_blocks) * 256 + (int)threadIdx.x + 64 * (int)threadIdx.y;
+    const int tid = (int)threadIdx.x + 64 * (int)threadIdx.y;
+    const int e = w0 - lean_blocks - shell_blocks;
+    if (e < zg.blocks) {
+      gather_zplane_sample(zg.set, zg.fs, zg.samples, (int64_t)e * 256 + tid);
+      return;
+    }
+    const int p = (e - zg.blocks) * 256 + tid;
     if (p < pts.count) accumulate_point(pts, p);
     return;
   }

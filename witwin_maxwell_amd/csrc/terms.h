@@ -261,8 +261,8 @@ struct ZGatherSet {
 // wavefront, so pairs that share a cache line (one component at neighbouring
 // columns, as a flux plane's H samples) fetch it once, when it is read, instead of
 // once per pair in passes far apart.
-__global__ __launch_bounds__(256) void gather_zplanes_kernel(ZGatherSet set, FieldSet fs, int64_t samples) {
-  const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void gather_zplane_sample(const ZGatherSet& set, const FieldSet& fs, int64_t samples,
+                                                     int64_t t) {
   if (t >= samples * set.count) return;
   const int q = (int)(t % set.count);
   const int64_t l = t / set.count;
@@ -276,6 +276,20 @@ __global__ __launch_bounds__(256) void gather_zplanes_kernel(ZGatherSet set, Fie
   }
   set.row[set.off[q] + l] = fs.ptr[c][l * fs.shape[c][2] + kq];
 }
+
+__global__ __launch_bounds__(256) void gather_zplanes_kernel(ZGatherSet set, FieldSet fs, int64_t samples) {
+  gather_zplane_sample(set, fs, samples, (int64_t)blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// One step's z-plane gather handed to a later launch: the stage row of that
+// step, the fields it wrote and the samples per plane (count = 0: none).
+struct ZGatherJob {
+  ZGatherSet set;
+  FieldSet fs;
+  int64_t samples;
+  int32_t blocks;  // 256-thread blocks that cover samples * set.count
+  int32_t pad;
+};
 
 struct ZFlushPlane {
   int32_t component;
